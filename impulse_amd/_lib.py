@@ -1,0 +1,94 @@
+"""ctypes binding of the C ABI declared in include/impulse_fft.h.
+
+The native library (``impulse_amd/libimpulse_fft.so``, HIP for gfx950) is the
+only compute path: if it is missing, ``lib()`` raises; there is no CPU
+fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libimpulse_fft.so")
+
+# Every function declared in include/impulse_fft.h.
+EXPORTS = (
+    "impulse_fft_plan_create", "impulse_fft_execute", "impulse_fft_execute_host",
+    "impulse_fft_plan_device_bytes", "impulse_fft_plan_num_launches", "impulse_fft_plan_destroy",
+    "impulse_fft_c2c", "impulse_fft_r2c", "impulse_fft_c2r",
+    "impulse_fft_cache_set_capacity", "impulse_fft_cache_size", "impulse_fft_cache_clear",
+    "impulse_make_cfft_plan", "impulse_destroy_cfft_plan", "impulse_cfft_forward",
+    "impulse_cfft_backward", "impulse_cfft_length",
+    "impulse_make_rfft_plan", "impulse_destroy_rfft_plan", "impulse_rfft_forward",
+    "impulse_rfft_backward", "impulse_rfft_length",
+    "impulse_fft_status_string", "impulse_fft_last_error", "impulse_fft_device_check",
+)
+
+C2C, R2C, C2R = 0, 1, 2
+F64 = 0
+
+_szp = ctypes.POINTER(ctypes.c_size_t)
+_pdp = ctypes.POINTER(ctypes.c_ssize_t)
+_vp = ctypes.c_void_p
+_LIB = None
+
+
+def _sig(L):
+    i, sz, d = ctypes.c_int, ctypes.c_size_t, ctypes.c_double
+    L.impulse_fft_plan_create.argtypes = [ctypes.POINTER(_vp), i, i, sz, _szp, _pdp, _pdp, sz, _szp, i]
+    L.impulse_fft_execute.argtypes = [_vp, _vp, _vp, d, _vp]
+    L.impulse_fft_execute_host.argtypes = [_vp, _vp, _vp, d]
+    L.impulse_fft_plan_device_bytes.argtypes = [_vp]
+    L.impulse_fft_plan_device_bytes.restype = sz
+    L.impulse_fft_plan_num_launches.argtypes = [_vp]
+    L.impulse_fft_plan_destroy.argtypes = [_vp]
+    for n in ("impulse_fft_c2c", "impulse_fft_r2c", "impulse_fft_c2r"):
+        getattr(L, n).argtypes = [sz, _szp, _pdp, _pdp, sz, _szp, i, i, _vp, _vp, d, sz]
+    L.impulse_fft_cache_set_capacity.argtypes = [sz]
+    L.impulse_fft_cache_size.restype = sz
+    for kind in ("cfft", "rfft"):
+        getattr(L, f"impulse_make_{kind}_plan").argtypes = [sz]
+        getattr(L, f"impulse_make_{kind}_plan").restype = _vp
+        getattr(L, f"impulse_destroy_{kind}_plan").argtypes = [_vp]
+        getattr(L, f"impulse_destroy_{kind}_plan").restype = None
+        for d_ in ("forward", "backward"):
+            getattr(L, f"impulse_{kind}_{d_}").argtypes = [_vp, _vp, d]
+        getattr(L, f"impulse_{kind}_length").argtypes = [_vp]
+        getattr(L, f"impulse_{kind}_length").restype = sz
+    L.impulse_fft_status_string.argtypes = [i]
+    L.impulse_fft_status_string.restype = ctypes.c_char_p
+    L.impulse_fft_last_error.restype = ctypes.c_char_p
+
+
+def lib():
+    """Load the HIP library (raises if it was not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"impulse_amd native library missing: {LIB_PATH}; build it with "
+                "`make -C impulse_amd/csrc` (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        _sig(L)
+        _LIB = L
+    return _LIB
+
+
+class FFTError(RuntimeError):
+    def __init__(self, status, message=""):
+        self.status = status
+        text = lib().impulse_fft_status_string(status).decode()
+        super().__init__(f"{text} ({status}): {message}" if message else f"{text} ({status})")
+
+
+def check(status):
+    if status != 0:
+        raise FFTError(status, lib().impulse_fft_last_error().decode())
+    return status
+
+
+def sizes(vals):
+    return (ctypes.c_size_t * len(vals))(*[int(v) for v in vals])
+
+
+def strides(vals):
+    return (ctypes.c_ssize_t * len(vals))(*[int(v) for v in vals])

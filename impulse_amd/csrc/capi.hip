@@ -1,0 +1,355 @@
+// extern "C" implementation of include/impulse_fft.h.
+//
+// Every entry point converts internal errors to status codes and records a
+// message retrievable with impulse_fft_last_error(); nothing throws across
+// the ABI.
+#include <list>
+#include <sstream>
+#include <unordered_map>
+
+#include "../../include/impulse_fft.h"
+#include "engine.h"
+
+struct impulse_fft_plan_s {
+    std::shared_ptr<ifft::Plan> plan;
+};
+
+struct impulse_cfft_plan_s {
+    size_t length = 0;
+    std::unique_ptr<ifft::Plan> fwd, bwd;
+};
+
+struct impulse_rfft_plan_s {
+    size_t length = 0;
+    std::unique_ptr<ifft::Plan> r2c, c2r;
+    ifft::DevBuf x, bins, hc;   // device staging: reals, complex bins, halfcomplex
+    std::mutex mtx;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(const ifft::Error &e)
+{
+    g_last_error = e.msg;
+    return e.code ? e.code : ifft::E_HIP;
+}
+
+int fail(int code, const char *msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+// ---- LRU plan cache for the one-shot calls -------------------------------
+struct Cache {
+    std::mutex mtx;
+    size_t capacity = 16;
+    std::list<std::pair<std::string, std::shared_ptr<ifft::Plan>>> lru;
+    std::unordered_map<std::string, decltype(lru)::iterator> map;
+
+    void trim()
+    {
+        while (lru.size() > capacity) {
+            map.erase(lru.back().first);
+            lru.pop_back();
+        }
+    }
+};
+
+Cache &cache()
+{
+    static Cache *c = new Cache();   // never destroyed: plans may outlive static teardown
+    return *c;
+}
+
+std::string cache_key(int kind, int dtype, size_t ndim, const size_t *shape, const ptrdiff_t *sin,
+                      const ptrdiff_t *sout, size_t naxes, const size_t *axes, int forward, int dev)
+{
+    std::ostringstream k;
+    k << kind << '|' << dtype << '|' << (forward ? 1 : 0) << '|' << dev << '|';
+    for (size_t d = 0; d < ndim; ++d) k << shape[d] << ',' << sin[d] << ',' << sout[d] << ';';
+    k << '|';
+    for (size_t a = 0; a < naxes; ++a) k << axes[a] << ',';
+    return k.str();
+}
+
+int cached_plan(std::shared_ptr<ifft::Plan> &out, int kind, int dtype, size_t ndim, const size_t *shape,
+                const ptrdiff_t *sin, const ptrdiff_t *sout, size_t naxes, const size_t *axes, int forward)
+{
+    if (ndim < 1 || !shape || !sin || !sout || naxes < 1 || !axes)
+        return fail(ifft::E_INVALID, "invalid argument");
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const std::string key = cache_key(kind, dtype, ndim, shape, sin, sout, naxes, axes, forward, dev);
+    Cache &c = cache();
+    {
+        std::lock_guard<std::mutex> lk(c.mtx);
+        auto it = c.map.find(key);
+        if (it != c.map.end()) {
+            c.lru.splice(c.lru.begin(), c.lru, it->second);
+            out = it->second->second;
+            return ifft::OK;
+        }
+    }
+    std::unique_ptr<ifft::Plan> p;
+    ifft::Error err{0, ""};
+    int rc = ifft::plan_create(p, kind, dtype, ndim, shape, sin, sout, naxes, axes, forward, err);
+    if (rc != ifft::OK) return fail(err);
+    std::shared_ptr<ifft::Plan> sp(std::move(p));
+    {
+        std::lock_guard<std::mutex> lk(c.mtx);
+        auto it = c.map.find(key);
+        if (it != c.map.end()) {   // another thread won the race
+            out = it->second->second;
+            return ifft::OK;
+        }
+        c.lru.emplace_front(key, sp);
+        c.map[key] = c.lru.begin();
+        c.trim();
+    }
+    out = sp;
+    return ifft::OK;
+}
+
+int oneshot(int kind, size_t ndim, const size_t *shape, const ptrdiff_t *sin, const ptrdiff_t *sout,
+            size_t naxes, const size_t *axes, int forward, int dtype, const void *in, void *out, double fct)
+{
+    if (shape)
+        for (size_t d = 0; d < ndim; ++d)
+            if (shape[d] == 0) return ifft::OK;   // pocketfft returns early on empty arrays
+    std::shared_ptr<ifft::Plan> p;
+    int rc = cached_plan(p, kind, dtype, ndim, shape, sin, sout, naxes, axes, forward);
+    if (rc != ifft::OK) return rc;
+    ifft::Error err{0, ""};
+    rc = ifft::plan_execute_host(*p, in, out, fct, err);
+    return rc == ifft::OK ? rc : fail(err);
+}
+
+int make_1d(std::unique_ptr<ifft::Plan> &p, int kind, size_t n, int forward)
+{
+    const size_t shape[1] = {n};
+    const ptrdiff_t sin[1] = {kind == IMPULSE_FFT_R2C ? 8 : 16};
+    const ptrdiff_t sout[1] = {kind == IMPULSE_FFT_C2R ? 8 : 16};
+    const size_t axes[1] = {0};
+    ifft::Error err{0, ""};
+    int rc = ifft::plan_create(p, kind, 0, 1, shape, sin, sout, 1, axes, forward, err);
+    return rc == ifft::OK ? rc : fail(err);
+}
+
+}  // namespace
+
+extern "C" {
+
+int impulse_fft_plan_create(impulse_fft_plan *plan, int kind, int dtype, size_t ndim, const size_t *shape,
+                            const ptrdiff_t *stride_in, const ptrdiff_t *stride_out, size_t naxes, const size_t *axes,
+                            int forward)
+{
+    if (!plan) return fail(ifft::E_INVALID, "plan pointer is NULL");
+    *plan = nullptr;
+    std::unique_ptr<ifft::Plan> p;
+    ifft::Error err{0, ""};
+    int rc = ifft::plan_create(p, kind, dtype, ndim, shape, stride_in, stride_out, naxes, axes, forward, err);
+    if (rc != ifft::OK) return fail(err);
+    auto *h = new (std::nothrow) impulse_fft_plan_s;
+    if (!h) return fail(ifft::E_NOMEM, "out of host memory");
+    h->plan = std::move(p);
+    *plan = h;
+    return ifft::OK;
+}
+
+int impulse_fft_execute(impulse_fft_plan plan, const void *data_in, void *data_out, double fct, void *stream)
+{
+    if (!plan || !plan->plan) return fail(ifft::E_INVALID, "plan is NULL");
+    ifft::Error err{0, ""};
+    int rc = ifft::plan_execute(*plan->plan, data_in, data_out, fct, (hipStream_t)stream, err);
+    return rc == ifft::OK ? rc : fail(err);
+}
+
+int impulse_fft_execute_host(impulse_fft_plan plan, const void *data_in, void *data_out, double fct)
+{
+    if (!plan || !plan->plan) return fail(ifft::E_INVALID, "plan is NULL");
+    ifft::Error err{0, ""};
+    int rc = ifft::plan_execute_host(*plan->plan, data_in, data_out, fct, err);
+    return rc == ifft::OK ? rc : fail(err);
+}
+
+size_t impulse_fft_plan_device_bytes(impulse_fft_plan plan)
+{
+    return (plan && plan->plan) ? plan->plan->device_bytes() : 0;
+}
+
+int impulse_fft_plan_num_launches(impulse_fft_plan plan)
+{
+    return (plan && plan->plan) ? (int)plan->plan->passes.size() : -1;
+}
+
+int impulse_fft_plan_destroy(impulse_fft_plan plan)
+{
+    delete plan;
+    return ifft::OK;
+}
+
+int impulse_fft_c2c(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in, const ptrdiff_t *stride_out,
+                    size_t naxes, const size_t *axes, int forward, int dtype, const void *data_in, void *data_out,
+                    double fct, size_t nthreads)
+{
+    (void)nthreads;
+    return oneshot(IMPULSE_FFT_C2C, ndim, shape, stride_in, stride_out, naxes, axes, forward, dtype, data_in,
+                   data_out, fct);
+}
+
+int impulse_fft_r2c(size_t ndim, const size_t *shape_in, const ptrdiff_t *stride_in, const ptrdiff_t *stride_out,
+                    size_t naxes, const size_t *axes, int forward, int dtype, const void *data_in, void *data_out,
+                    double fct, size_t nthreads)
+{
+    (void)nthreads;
+    return oneshot(IMPULSE_FFT_R2C, ndim, shape_in, stride_in, stride_out, naxes, axes, forward, dtype, data_in,
+                   data_out, fct);
+}
+
+int impulse_fft_c2r(size_t ndim, const size_t *shape_out, const ptrdiff_t *stride_in, const ptrdiff_t *stride_out,
+                    size_t naxes, const size_t *axes, int forward, int dtype, const void *data_in, void *data_out,
+                    double fct, size_t nthreads)
+{
+    (void)nthreads;
+    return oneshot(IMPULSE_FFT_C2R, ndim, shape_out, stride_in, stride_out, naxes, axes, forward, dtype, data_in,
+                   data_out, fct);
+}
+
+int impulse_fft_cache_set_capacity(size_t entries)
+{
+    Cache &c = cache();
+    std::lock_guard<std::mutex> lk(c.mtx);
+    c.capacity = entries;
+    c.trim();
+    return ifft::OK;
+}
+
+size_t impulse_fft_cache_size(void)
+{
+    Cache &c = cache();
+    std::lock_guard<std::mutex> lk(c.mtx);
+    return c.lru.size();
+}
+
+int impulse_fft_cache_clear(void)
+{
+    Cache &c = cache();
+    std::lock_guard<std::mutex> lk(c.mtx);
+    c.lru.clear();
+    c.map.clear();
+    return ifft::OK;
+}
+
+// ---- C path -----------------------------------------------------------------
+
+impulse_cfft_plan impulse_make_cfft_plan(size_t length)
+{
+    if (length == 0) { fail(ifft::E_INVALID, "length must be >= 1"); return nullptr; }
+    auto *h = new (std::nothrow) impulse_cfft_plan_s;
+    if (!h) return nullptr;
+    h->length = length;
+    if (make_1d(h->fwd, IMPULSE_FFT_C2C, length, 1) != ifft::OK || make_1d(h->bwd, IMPULSE_FFT_C2C, length, 0) != ifft::OK) {
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+
+void impulse_destroy_cfft_plan(impulse_cfft_plan plan) { delete plan; }
+
+static int cfft_run(impulse_cfft_plan plan, double c[], double fct, bool forward)
+{
+    if (!plan || !c) { fail(ifft::E_INVALID, "NULL plan or data"); return -1; }
+    if (plan->length == 1) return 0;   // PocketFFT's C path returns unscaled here (pocketfft.c:874)
+    ifft::Error err{0, ""};
+    int rc = ifft::plan_execute_host(forward ? *plan->fwd : *plan->bwd, c, c, fct, err);
+    if (rc != ifft::OK) { fail(err); return -1; }
+    return 0;
+}
+
+int impulse_cfft_forward(impulse_cfft_plan plan, double c[], double fct) { return cfft_run(plan, c, fct, true); }
+int impulse_cfft_backward(impulse_cfft_plan plan, double c[], double fct) { return cfft_run(plan, c, fct, false); }
+size_t impulse_cfft_length(impulse_cfft_plan plan) { return plan ? plan->length : 0; }
+
+impulse_rfft_plan impulse_make_rfft_plan(size_t length)
+{
+    if (length == 0) { fail(ifft::E_INVALID, "length must be >= 1"); return nullptr; }
+    auto *h = new (std::nothrow) impulse_rfft_plan_s;
+    if (!h) return nullptr;
+    h->length = length;
+    const size_t nb = length / 2 + 1;
+    if (make_1d(h->r2c, IMPULSE_FFT_R2C, length, 1) != ifft::OK ||
+        make_1d(h->c2r, IMPULSE_FFT_C2R, length, 0) != ifft::OK || h->x.alloc(length * 8) != ifft::OK ||
+        h->bins.alloc(nb * 16) != ifft::OK || h->hc.alloc(length * 8) != ifft::OK) {
+        if (g_last_error.empty()) g_last_error = "rfft plan allocation failed";
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+
+void impulse_destroy_rfft_plan(impulse_rfft_plan plan) { delete plan; }
+
+static int rfft_run(impulse_rfft_plan plan, double c[], double fct, bool forward)
+{
+    if (!plan || !c) { fail(ifft::E_INVALID, "NULL plan or data"); return -1; }
+    std::lock_guard<std::mutex> lk(plan->mtx);
+    const size_t n = plan->length;
+    if (n == 1) return 0;   // PocketFFT's C path returns unscaled here (pocketfft.c:1703,1739)
+    ifft::Error err{0, ""};
+    hipError_t e = hipSuccess;
+    int rc = ifft::OK;
+    if (forward) {
+        e = hipMemcpy(plan->x.p, c, n * 8, hipMemcpyHostToDevice);
+        if (e == hipSuccess) rc = ifft::plan_execute(*plan->r2c, plan->x.p, plan->bins.p, fct, nullptr, err);
+        if (e == hipSuccess && rc == ifft::OK)
+            e = ifft::launch_pack_halfcomplex((const double2 *)plan->bins.p, (double *)plan->hc.p, (long long)n, nullptr);
+        if (e == hipSuccess && rc == ifft::OK) e = hipMemcpy(c, plan->hc.p, n * 8, hipMemcpyDeviceToHost);
+    } else {
+        e = hipMemcpy(plan->hc.p, c, n * 8, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = ifft::launch_unpack_halfcomplex((const double *)plan->hc.p, (double2 *)plan->bins.p, (long long)n, nullptr);
+        if (e == hipSuccess) rc = ifft::plan_execute(*plan->c2r, plan->bins.p, plan->x.p, fct, nullptr, err);
+        if (e == hipSuccess && rc == ifft::OK) e = hipMemcpy(c, plan->x.p, n * 8, hipMemcpyDeviceToHost);
+    }
+    if (rc != ifft::OK) { fail(err); return -1; }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(ifft::E_HIP, hipGetErrorString(e));
+        return -1;
+    }
+    return 0;
+}
+
+int impulse_rfft_forward(impulse_rfft_plan plan, double c[], double fct) { return rfft_run(plan, c, fct, true); }
+int impulse_rfft_backward(impulse_rfft_plan plan, double c[], double fct) { return rfft_run(plan, c, fct, false); }
+size_t impulse_rfft_length(impulse_rfft_plan plan) { return plan ? plan->length : 0; }
+
+// ---- diagnostics ------------------------------------------------------------
+
+const char *impulse_fft_status_string(int status)
+{
+    switch (status) {
+        case IMPULSE_FFT_OK: return "ok";
+        case IMPULSE_FFT_ERR_INVALID_ARG: return "invalid argument";
+        case IMPULSE_FFT_ERR_NOMEM: return "out of memory";
+        case IMPULSE_FFT_ERR_HIP: return "HIP runtime error";
+        case IMPULSE_FFT_ERR_UNSUPPORTED: return "unsupported request";
+        case IMPULSE_FFT_ERR_NO_DEVICE: return "no usable gfx950 device";
+        default: return "unknown status";
+    }
+}
+
+const char *impulse_fft_last_error(void) { return g_last_error.c_str(); }
+
+int impulse_fft_device_check(void)
+{
+    ifft::Error err{0, ""};
+    int rc = ifft::device_check(err);
+    return rc == ifft::OK ? rc : fail(err);
+}
+
+}  // extern "C"

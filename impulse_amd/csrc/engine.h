@@ -1,0 +1,88 @@
+// Host-side engine: plans, device tables and the pass list one execute runs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fft_params.h"
+
+namespace ifft {
+
+// launchers (fft_lds.hip)
+hipError_t launch_fft_lds(const FftParams &P, unsigned grid, unsigned block, size_t lds_bytes, hipStream_t s);
+hipError_t launch_real_post(const RealPostParams &P, hipStream_t s);
+hipError_t launch_pack_halfcomplex(const double2 *bins, double *hc, long long n, hipStream_t s);
+hipError_t launch_unpack_halfcomplex(const double *hc, double2 *bins, long long n, hipStream_t s);
+
+// status codes (mirror impulse_fft_status)
+enum Status { OK = 0, E_INVALID = 1, E_NOMEM = 2, E_HIP = 3, E_UNSUPPORTED = 4, E_NODEVICE = 5 };
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+// A device allocation owned by a plan (freed on destruction).
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf();
+    int alloc(size_t n);
+};
+
+// Which buffer a pass reads or writes; resolved at execute time.
+enum BufRef : int { REF_IN = 0, REF_OUT = 1, REF_S0 = 2, REF_S1 = 3, REF_TMP = 4, REF_NONE = 5 };
+
+struct Pass {
+    enum Kind { FFT_LDS, REAL_POST } kind;
+    FftParams fp{};
+    RealPostParams rp{};
+    unsigned grid = 1, block = 64;
+    size_t lds = 0;
+    BufRef src = REF_NONE, dst = REF_NONE;
+    bool uses_fct = false;
+};
+
+struct Plan {
+    int kind = 0, dtype = 0, forward = 1;
+    int device = 0;
+    std::vector<size_t> shape;          // shape as given (real shape for r2c/c2r)
+    std::vector<ptrdiff_t> sin, sout;   // byte strides
+    std::vector<size_t> axes;
+    bool empty = false;                 // prod(shape) == 0: execute is a no-op
+    std::vector<Pass> passes;
+    std::vector<std::shared_ptr<DevBuf>> tables;
+    DevBuf s0, s1, tmp;                 // workspace
+    size_t s0_bytes = 0, s1_bytes = 0, tmp_bytes = 0;
+    // host staging (execute_host)
+    std::mutex host_mtx;
+    DevBuf stage_in, stage_out;
+    ptrdiff_t in_lo = 0, in_hi = 0, out_lo = 0, out_hi = 0;   // byte spans relative to element 0
+    bool out_dense = false;
+    size_t device_bytes() const;
+};
+
+int plan_create(std::unique_ptr<Plan> &out, int kind, int dtype, size_t ndim, const size_t *shape,
+                const ptrdiff_t *stride_in, const ptrdiff_t *stride_out, size_t naxes, const size_t *axes,
+                int forward, Error &err);
+int plan_execute(Plan &plan, const void *in, void *out, double fct, hipStream_t stream, Error &err);
+int plan_execute_host(Plan &plan, const void *in, void *out, double fct, Error &err);
+
+// host FP64 twiddles (twiddle.hip)
+void twiddle_table(std::vector<double2> &out, uint64_t n, uint64_t count, uint64_t step);
+double2 twiddle_at(uint64_t k, uint64_t n);   // exp(-2 pi i k / n), exact at multiples of pi/4
+
+// device info
+int device_check(Error &err);
+int max_lds_bytes();
+
+}  // namespace ifft

@@ -1,0 +1,448 @@
+// Generic single-pass FP64 FFT kernel: a batch of length-N transforms staged
+// through LDS, mixed-radix Stockham autosort stages with runtime radices
+// (2, 3, 4, 5, 7, 8, 11, 13, 16), and fused load/store modes for real
+// transforms, four-step passes and Bluestein.  See fft_params.h for the
+// geometry model.
+//
+// Stockham stage (radix R, p = product of the radices already applied):
+//   butterfly b in [0, N/R): k = b mod p,
+//   inputs  x[b + q N/R] * W_{pR}^{q k},  q < R
+//   outputs y[(b - k) R + k + q p] = DFT_R(inputs)[q]
+// which leaves the result in natural order after the last stage.  Each stage
+// reads its inputs into registers, synchronises, and writes back in place.
+#include "butterflies.h"
+#include "fft_params.h"
+
+namespace ifft {
+
+// ---------------------------------------------------------------------------
+// batch geometry helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ long long dims_off(const Side &s, long long p)
+{
+    long long off = 0;
+    for (int d = 0; d < s.nbd; ++d) {
+        long long c = s.cnt[d];
+        long long idx = p % c;
+        p /= c;
+        off += idx * s.str[d];
+    }
+    return off;
+}
+
+// base offsets of logical outer index u: row a (and row b for pair sides)
+__device__ __forceinline__ void side_base(const Side &s, long long u, long long &a, long long &b, int &hasb)
+{
+    if (s.kind == SIDE_LINEAR) {
+        a = u * s.lin;
+        b = 0;
+        hasb = 0;
+    } else if (s.kind == SIDE_DIMS_PAIR) {
+        long long p = 2 * u;
+        a = dims_off(s, p);
+        hasb = (p + 1) < s.nphys;
+        b = hasb ? dims_off(s, p + 1) : 0;
+    } else {
+        a = dims_off(s, u);
+        b = 0;
+        hasb = 0;
+    }
+}
+
+__device__ __forceinline__ double2 twolevel(const double2 *lo, const double2 *hi, long long S, long long m)
+{
+    return cmul(lo[m % S], hi[m / S]);
+}
+
+__device__ __forceinline__ double2 real_tw(const FftParams &P, long long k)
+{
+    // W_{2M}^k for the real-half modes
+    return P.rtw ? P.rtw[k] : twolevel(P.rtw_lo, P.rtw_hi, P.rtw_S, k);
+}
+
+// ---------------------------------------------------------------------------
+// Stockham stages in LDS
+// ---------------------------------------------------------------------------
+
+template <int R, int NB>
+__device__ __forceinline__ void stage(double2 *__restrict__ buf, int N, int p, int i, int NT,
+                                      const double2 *__restrict__ tw, double s)
+{
+    const int nb = N / R;
+    const int twstep = N / (p * R);
+    double2 v[NB][R];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        const int b = i + c * NT;
+        if (b < nb) {
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[c][q] = buf[b + q * nb];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        const int b = i + c * NT;
+        if (b < nb) {
+            const int k = b % p;
+            if (p > 1) {
+#pragma unroll
+                for (int q = 1; q < R; ++q) v[c][q] = cmul(v[c][q], tw_sign(tw[q * k * twstep], s));
+            }
+            dft<R>(v[c], s);
+            const int j = (b - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) buf[j + q * p] = v[c][q];
+        }
+    }
+    __syncthreads();
+}
+
+__device__ void run_stages(double2 *buf, const FftParams &P, int i, double s)
+{
+    int p = 1;
+    for (int st = 0; st < P.nst; ++st) {
+        const int r = P.radix[st];
+        switch (r) {
+            case 2: stage<2, nb_max(2)>(buf, P.N, p, i, P.NT, P.tw, s); break;
+            case 3: stage<3, nb_max(3)>(buf, P.N, p, i, P.NT, P.tw, s); break;
+            case 4: stage<4, nb_max(4)>(buf, P.N, p, i, P.NT, P.tw, s); break;
+            case 5: stage<5, nb_max(5)>(buf, P.N, p, i, P.NT, P.tw, s); break;
+            case 7: stage<7, nb_max(7)>(buf, P.N, p, i, P.NT, P.tw, s); break;
+            case 8: stage<8, nb_max(8)>(buf, P.N, p, i, P.NT, P.tw, s); break;
+            case 11: stage<11, nb_max(11)>(buf, P.N, p, i, P.NT, P.tw, s); break;
+            case 13: stage<13, nb_max(13)>(buf, P.N, p, i, P.NT, P.tw, s); break;
+            default: stage<16, nb_max(16)>(buf, P.N, p, i, P.NT, P.tw, s); break;
+        }
+        p *= r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// load / store of one point
+// ---------------------------------------------------------------------------
+
+struct TfInfo {
+    long long ia, ib, oa, ob;   // signal bases (input rows a/b, output rows a/b)
+    long long ip0, op0;         // position of element 0 (input / output side)
+    long long j;                // inner index
+    int flags;                  // 1 valid, 2 input row b exists, 4 output row b exists
+};
+
+__device__ __forceinline__ double2 load_point(const FftParams &P, const TfInfo &T, long long pos)
+{
+    const Side &S = P.in;
+    switch (P.load_mode) {
+        case LD_C2C: {
+            const double2 *x = (const double2 *)P.src;
+            return x[T.ia + pos * S.sax];
+        }
+        case LD_REAL_HALF: {
+            const double *x = (const double *)P.src;
+            return make_double2(x[T.ia + 2 * pos * S.sax], x[T.ia + (2 * pos + 1) * S.sax]);
+        }
+        case LD_R2C_PAIR: {
+            const double *x = (const double *)P.src;
+            double a = x[T.ia + pos * S.sax];
+            double b = (T.flags & 2) ? x[T.ib + pos * S.sax] : 0.0;
+            return make_double2(a, b);
+        }
+        case LD_C2R_HALF: {
+            const double2 *x = (const double2 *)P.src;
+            const long long M = P.M;
+            double2 xp = x[T.ia + pos * S.sax];
+            double2 xq = x[T.ia + (M - pos) * S.sax];
+            if (P.fwd_flag) { xp = cconj(xp); xq = cconj(xq); }
+            if (pos == 0) { xp.y = 0.0; xq.y = 0.0; }
+            double2 e = cadd(xp, cconj(xq));
+            double2 o = cmul(csub(xp, cconj(xq)), cconj(real_tw(P, pos)));
+            return make_double2(e.x - o.y, e.y + o.x);
+        }
+        default: {  // LD_C2R_PAIR
+            const double2 *x = (const double2 *)P.src;
+            const long long L = P.M;
+            const bool mirror = 2 * pos > L;
+            const long long k = mirror ? L - pos : pos;
+            double2 a = x[T.ia + k * S.sax];
+            double2 b = (T.flags & 2) ? x[T.ib + k * S.sax] : make_double2(0.0, 0.0);
+            if (P.fwd_flag != (int)mirror) { a = cconj(a); b = cconj(b); }
+            if (pos == 0) { a.y = 0.0; b.y = 0.0; }
+            return make_double2(a.x - b.y, a.y + b.x);
+        }
+    }
+}
+
+__device__ __forceinline__ void put_c(const FftParams &P, long long off, double2 y)
+{
+    ((double2 *)P.dst)[off] = y;
+}
+
+// store local point n of transform t; `buf` is the transform's LDS row
+__device__ __forceinline__ void store_point(const FftParams &P, const TfInfo &T, const double2 *buf, int n)
+{
+    const Side &S = P.out;
+    const long long pos = T.op0 + (long long)n * S.pst;
+    switch (P.store_mode) {
+        case ST_C2C: {
+            if (pos >= P.olim) return;
+            put_c(P, T.oa + pos * S.sax, cscale(buf[n], P.fct));
+            return;
+        }
+        case ST_TWIDDLE: {
+            double2 w = tw_sign(twolevel(P.tw_lo, P.tw_hi, P.tw_S, (long long)P.tw_mul * T.j * (long long)n), (double)P.sign);
+            put_c(P, T.oa + pos * S.sax, cmul(buf[n], w));
+            return;
+        }
+        case ST_MUL_TABLE: {
+            put_c(P, T.oa + pos * S.sax, cmul(buf[n], P.mtab[pos]));
+            return;
+        }
+        case ST_R2C_HALF: {
+            const int M = P.M;
+            double2 zk = buf[n % M], zm = cconj(buf[(M - n) % M]);
+            double2 e = cscale(cadd(zk, zm), 0.5);
+            double2 d = csub(zk, zm);
+            double2 o = make_double2(0.5 * d.y, -0.5 * d.x);  // -i/2 (zk - zm*)
+            double2 X = cadd(e, cmul(o, real_tw(P, n)));
+            X = cscale(X, P.fct);
+            if (!P.fwd_flag) X = cconj(X);
+            put_c(P, T.oa + pos * S.sax, X);
+            return;
+        }
+        case ST_R2C_PAIR: {
+            const int L = P.M;
+            double2 zk = buf[n], zm = cconj(buf[(L - n) % L]);
+            double2 xa = cscale(cadd(zk, zm), 0.5 * P.fct);
+            double2 d = csub(zk, zm);
+            double2 xb = make_double2(0.5 * P.fct * d.y, -0.5 * P.fct * d.x);
+            if (!P.fwd_flag) { xa = cconj(xa); xb = cconj(xb); }
+            put_c(P, T.oa + pos * S.sax, xa);
+            if (T.flags & 4) put_c(P, T.ob + pos * S.sax, xb);
+            return;
+        }
+        case ST_C2R_HALF: {
+            if (pos >= P.olim) return;
+            double *y = (double *)P.dst;
+            double2 z = buf[n];
+            y[T.oa + 2 * pos * S.sax] = z.x * P.fct;
+            y[T.oa + (2 * pos + 1) * S.sax] = z.y * P.fct;
+            return;
+        }
+        default: {  // ST_C2R_PAIR
+            if (pos >= P.olim) return;
+            double *y = (double *)P.dst;
+            double2 z = buf[n];
+            y[T.oa + pos * S.sax] = z.x * P.fct;
+            if (T.flags & 4) y[T.ob + pos * S.sax] = z.y * P.fct;
+            return;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the kernel
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kMaxThreads) k_fft_lds(const FftParams P)
+{
+    extern __shared__ double2 lds[];
+    __shared__ TfInfo info[kMaxTB];
+
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    const int TB = P.TB, N = P.N, LD = P.LD;
+    const long long g0 = (long long)blockIdx.x * TB;
+
+    if (tid < TB) {
+        TfInfo T;
+        const long long g = g0 + tid;
+        T.flags = 0;
+        T.ia = T.ib = T.oa = T.ob = T.ip0 = T.op0 = T.j = 0;
+        if (g < P.ntf) {
+            const long long j = g % P.ni, u = g / P.ni;
+            int hb_in, hb_out;
+            side_base(P.in, u, T.ia, T.ib, hb_in);
+            side_base(P.out, u, T.oa, T.ob, hb_out);
+            T.ia += j * P.in.isig;
+            T.ib += j * P.in.isig;
+            T.oa += j * P.out.isig;
+            T.ob += j * P.out.isig;
+            T.ip0 = j * P.in.ipos;
+            T.op0 = j * P.out.ipos;
+            T.j = j;
+            T.flags = 1 | (hb_in ? 2 : 0) | (hb_out ? 4 : 0);
+        }
+        info[tid] = T;
+    }
+    __syncthreads();
+
+    // global -> LDS, with zero padding beyond the signal and the Bluestein chirp
+    for (int e = tid; e < TB * N; e += nthr) {
+        int t, n;
+        if (P.col_load) { t = e % TB; n = e / TB; }
+        else { n = e % N; t = e / N; }
+        const TfInfo &T = info[t];
+        double2 z = make_double2(0.0, 0.0);
+        const long long pos = T.ip0 + (long long)n * P.in.pst;
+        if ((T.flags & 1) && n < P.nload && pos < P.ilim) {
+            z = load_point(P, T, pos);
+            if (P.conj_in) z = cconj(z);
+            if (P.chirp_in) z = cmul(z, P.chirp[pos]);
+        }
+        lds[t * LD + n] = z;
+    }
+    __syncthreads();
+
+    const int t = tid / P.NT, i = tid % P.NT;
+    double2 *buf = lds + t * LD;
+    run_stages(buf, P, i, (double)P.sign);
+
+    if (P.conv) {
+        // Bluestein: multiply by the chirp kernel's spectrum, transform back
+        for (int e = tid; e < TB * N; e += nthr) {
+            const int tt = e / N, n = e % N;
+            lds[tt * LD + n] = cmul(lds[tt * LD + n], P.spec[n]);
+        }
+        __syncthreads();
+        run_stages(buf, P, i, 1.0);
+    }
+    if (P.chirp_out || P.conj_out) {
+        for (int e = tid; e < TB * N; e += nthr) {
+            const int tt = e / N, n = e % N;
+            const long long pos = info[tt].op0 + (long long)n * P.out.pst;
+            double2 z = lds[tt * LD + n];
+            if (P.chirp_out && pos < P.olim) z = cmul(z, P.chirp[pos]);
+            if (P.conj_out) z = cconj(z);
+            lds[tt * LD + n] = z;
+        }
+        __syncthreads();
+    }
+
+    // LDS -> global
+    for (int e = tid; e < TB * P.nstore; e += nthr) {
+        int tt, n;
+        if (P.col_store) { tt = e % TB; n = e / TB; }
+        else { n = e % P.nstore; tt = e / P.nstore; }
+        const TfInfo &T = info[tt];
+        if (T.flags & 1) store_point(P, T, lds + tt * LD, n);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// four-step real post-process: Z -> half spectra
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_real_post(const RealPostParams P)
+{
+    const long long per = P.half ? (long long)P.M + 1 : (long long)P.M / 2 + 1;
+    const long long total = per * P.ntf;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long u = e / per, k = e % per;
+        const double2 *Z = P.Z + u * P.zlen;
+        long long oa, ob;
+        int hb;
+        side_base(P.out, u, oa, ob, hb);
+        double2 *y = (double2 *)P.dst;
+        if (P.half) {
+            const long long M = P.M;
+            double2 zk = Z[k % M], zm = cconj(Z[(M - k) % M]);
+            double2 ev = cscale(cadd(zk, zm), 0.5);
+            double2 d = csub(zk, zm);
+            double2 o = make_double2(0.5 * d.y, -0.5 * d.x);
+            double2 w = P.two_level ? twolevel(P.tw_lo, P.tw_hi, P.tw_S, k) : P.rtw[k];
+            double2 X = cscale(cadd(ev, cmul(o, w)), P.fct);
+            if (!P.fwd_flag) X = cconj(X);
+            y[oa + k * P.out.sax] = X;
+        } else {
+            const long long L = P.M;
+            double2 zk = Z[k], zm = cconj(Z[(L - k) % L]);
+            double2 xa = cscale(cadd(zk, zm), 0.5 * P.fct);
+            double2 d = csub(zk, zm);
+            double2 xb = make_double2(0.5 * P.fct * d.y, -0.5 * P.fct * d.x);
+            if (!P.fwd_flag) { xa = cconj(xa); xb = cconj(xb); }
+            y[oa + k * P.out.sax] = xa;
+            if (hb) y[ob + k * P.out.sax] = xb;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// C-path layout kernels: complex bins <-> FFTPACK halfcomplex
+// ---------------------------------------------------------------------------
+
+// bins[0..n/2] -> hc[0..n): [r0, r1, i1, r2, i2, ... (, r_{n/2})]
+__global__ void k_pack_halfcomplex(const double2 *__restrict__ bins, double *__restrict__ hc, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        if (i == 0) hc[0] = bins[0].x;
+        else {
+            const long long k = (i + 1) / 2;
+            hc[i] = (i & 1) ? bins[k].x : bins[k].y;
+        }
+    }
+}
+
+// hc[0..n) -> bins[0..n/2] (imaginary parts of bin 0 and of bin n/2 for even n are zero)
+__global__ void k_unpack_halfcomplex(const double *__restrict__ hc, double2 *__restrict__ bins, long long n)
+{
+    const long long nb = n / 2 + 1;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nb;
+         k += (long long)gridDim.x * blockDim.x) {
+        double2 z;
+        if (k == 0) z = make_double2(hc[0], 0.0);
+        else {
+            z.x = hc[2 * k - 1];
+            z.y = (2 * k < n) ? hc[2 * k] : 0.0;
+        }
+        bins[k] = z;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers (called from the planner / C ABI)
+// ---------------------------------------------------------------------------
+
+hipError_t launch_fft_lds(const FftParams &P, unsigned grid, unsigned block, size_t lds_bytes, hipStream_t s)
+{
+    static bool attr_done = false;
+    if (!attr_done) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_fft_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           160 * 1024 - (int)(sizeof(TfInfo) * kMaxTB) - 1024);
+        if (e != hipSuccess) (void)hipGetLastError();
+        attr_done = true;
+    }
+    hipLaunchKernelGGL(k_fft_lds, dim3(grid), dim3(block), lds_bytes, s, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_real_post(const RealPostParams &P, hipStream_t s)
+{
+    const long long per = P.half ? (long long)P.M + 1 : (long long)P.M / 2 + 1;
+    long long total = per * P.ntf;
+    long long blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_real_post, dim3((unsigned)blocks), dim3(256), 0, s, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_halfcomplex(const double2 *bins, double *hc, long long n, hipStream_t s)
+{
+    long long blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_pack_halfcomplex, dim3((unsigned)blocks), dim3(256), 0, s, bins, hc, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack_halfcomplex(const double *hc, double2 *bins, long long n, hipStream_t s)
+{
+    long long blocks = (n / 2 + 1 + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_unpack_halfcomplex, dim3((unsigned)blocks), dim3(256), 0, s, hc, bins, n);
+    return hipGetLastError();
+}
+
+}  // namespace ifft

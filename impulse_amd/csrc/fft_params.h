@@ -1,0 +1,121 @@
+// Kernel-side parameter blocks of the impulse FFT engine (host and device).
+//
+// Geometry model.  A launch runs a batch of "logical transforms" of length N
+// (complex points) through LDS.  Logical transform g is split into an inner
+// index j = g % ni and an outer index u = g / ni.  Element n of transform g
+// sits at signal position   pos = j*ipos + n*pst   of a signal whose element 0
+// is at   base(u) + j*isig ;  its address is   base + pos*sax   in units of the
+// array's scalar (double for real arrays, double2 for complex arrays).
+// base(u) comes from one of three "side" layouts:
+//   SIDE_DIMS      : u is a physical batch index decomposed over nbd dims
+//   SIDE_DIMS_PAIR : u covers physical rows 2u and 2u+1 (row 2u+1 may not exist)
+//   SIDE_LINEAR    : base = u * lin (a contiguous workspace)
+// One-pass transforms use ni = the batch dim kept for coalescing (isig = its
+// stride, ipos = 0); four-step passes use ni = the other factor (isig = 0,
+// ipos = 1 or N1) so that loads and stores can address the full signal.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace ifft {
+
+constexpr int kMaxStages = 24;
+constexpr int kMaxBatchDims = 8;
+constexpr int kMaxTB = 64;          // transforms per workgroup
+constexpr int kMaxLdsLen = 8192;    // longest single-pass FP64 transform (128 KiB of LDS)
+constexpr int kMaxThreads = 512;       // workgroup size cap (256 VGPRs per lane available)
+
+enum LoadMode : int {
+    LD_C2C = 0,       // z = x[pos]                                  (complex input)
+    LD_REAL_HALF = 1, // z = x[2 pos] + i x[2 pos + 1]               (even-length real input)
+    LD_R2C_PAIR = 2,  // z = xa[pos] + i xb[pos]                     (odd-length real rows 2u, 2u+1)
+    LD_C2R_HALF = 3,  // z = (X[p] + X*[M-p]) + i (X[p] - X*[M-p]) W_{2M}^{-p}  (even c2r)
+    LD_C2R_PAIR = 4   // z = A[pos] + i B[pos], A, B Hermitian-extended half spectra (odd c2r)
+};
+
+enum StoreMode : int {
+    ST_C2C = 0,       // y[pos] = fct z[n]
+    ST_TWIDDLE = 1,   // y[pos] = z[n] W_L^{j n}                     (four-step pass A)
+    ST_R2C_HALF = 2,  // X[k], k = 0..M, from Z[k], Z[M-k]           (even r2c)
+    ST_R2C_PAIR = 3,  // Xa[k], Xb[k], k = 0..L/2, from Z[k], Z[L-k] (odd r2c pair)
+    ST_C2R_HALF = 4,  // x[2 pos] = Re z, x[2 pos + 1] = Im z
+    ST_C2R_PAIR = 5,  // xa[pos] = Re z, xb[pos] = Im z
+    ST_MUL_TABLE = 6  // y[pos] = z[n] tab[pos]                      (Bluestein spectrum product)
+};
+
+enum SideKind : int { SIDE_DIMS = 0, SIDE_DIMS_PAIR = 1, SIDE_LINEAR = 2 };
+
+struct Side {
+    int kind;
+    int nbd;                        // SIDE_DIMS*: number of batch dims (dim 0 fastest)
+    long long cnt[kMaxBatchDims];   // physical counts
+    long long str[kMaxBatchDims];   // strides (scalar units of this side's array)
+    long long nphys;                // physical batch count (pair bound check)
+    long long lin;                  // SIDE_LINEAR stride per logical transform
+    long long isig;                 // added to the base per inner index j
+    long long ipos;                 // added to the position per inner index j
+    long long pst;                  // position step per element n
+    long long sax;                  // scalar stride per position
+};
+
+struct FftParams {
+    // transform in LDS
+    int N;                     // transform length in LDS (points)
+    int nst;                   // number of Stockham stages
+    int radix[kMaxStages];
+    int NT;                    // threads per transform
+    int TB;                    // transforms per workgroup
+    int LD;                    // LDS words (double2) per transform
+    // modes
+    int load_mode, store_mode;
+    int col_load, col_store;   // lane mapping: 1 = lanes walk transforms first
+    int chirp_in, conv, chirp_out;  // Bluestein pieces
+    int conj_in, conj_out;     // backward Bluestein runs the forward tables on conjugates
+    int sign;                  // -1: forward stages, +1: backward stages
+    int fwd_flag;              // r2c: 0 => conjugate output; c2r: 1 => conjugate input
+    long long ilim;            // input positions >= ilim load as zero (Bluestein zero padding)
+    long long olim;            // output positions >= olim are not stored
+    int M;                     // half length (real-half modes) or odd length (pair modes)
+    int nload;                 // local points loaded per transform (the rest of the row is zero)
+    int nstore;                // points stored per transform
+    double fct;
+    // batch
+    long long ntf;             // logical transforms in this launch
+    long long ni;              // inner count
+    Side in, out;
+    // arrays
+    const void *src;
+    void *dst;
+    const double2 *tw;         // W_N^j (forward sign), j < N
+    const double2 *chirp;      // w[n] = exp(-i pi n^2 / nsig), n < nsig
+    const double2 *spec;       // Bluestein kernel spectrum / N, N entries
+    const double2 *rtw;        // W_{2M}^k, k <= M (real-half modes)
+    const double2 *tw_lo;      // two-level W_L^m = lo[m % S] * hi[m / S]
+    const double2 *tw_hi;
+    long long tw_S, tw_L;
+    int tw_mul;                // ST_TWIDDLE uses W_L^{tw_mul j n}
+    const double2 *rtw_lo;     // two-level W_{2M} used by the real-half modes when rtw is null
+    const double2 *rtw_hi;
+    long long rtw_S;
+    const double2 *mtab;       // ST_MUL_TABLE
+};
+
+// Post-process of a four-step real transform: Z (contiguous per logical
+// signal) -> half spectra in the output layout.
+struct RealPostParams {
+    int half;                  // 1: even length, Z has M points; 0: odd pair, Z has L points
+    int M;                     // M (half) or L (pair)
+    int fwd_flag;
+    double fct;
+    long long ntf;             // logical signals
+    long long zlen;            // Z stride per logical signal
+    Side out;
+    const double2 *Z;
+    void *dst;
+    const double2 *rtw;        // W_{2M}^k, k <= M (half)
+    const double2 *tw_lo, *tw_hi;
+    long long tw_S;
+    int two_level;             // rtw replaced by the two-level table
+};
+
+}  // namespace ifft

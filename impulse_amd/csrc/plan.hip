@@ -1,0 +1,921 @@
+// Planner: turns an n-D c2c / r2c / c2r request into a list of GPU passes.
+//
+// Composition over axes follows pocketfft_hdronly.h:3272-3390:
+//   c2c : one job per entry of `axes`, in order; the first reads the input, the
+//         rest work in place on the output; fct is applied once (first job).
+//   r2c : r2c along axes.back() (input -> output, fct), then c2c in place on
+//         the output over the remaining axes.
+//   c2r : c2c over all axes but the last into a contiguous temporary, then
+//         c2r along axes.back() (temporary -> output, fct).
+// Each 1-D job picks one algorithm (see build_job):
+//   length <= 8192 and 13-smooth  : one LDS pass (real transforms fused)
+//   13-smooth, longer             : four-step, two LDS passes through a workspace
+//   otherwise                     : Bluestein chirp-z with a 7-smooth length
+//                                   n2 >= 2L-1 (one LDS pass, or four passes)
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+
+#include "butterflies.h"
+#include "engine.h"
+
+namespace ifft {
+
+// ---------------------------------------------------------------------------
+// device buffers and queries
+// ---------------------------------------------------------------------------
+
+DevBuf::~DevBuf()
+{
+    if (p) (void)hipFree(p);
+}
+
+int DevBuf::alloc(size_t n)
+{
+    if (p) { (void)hipFree(p); p = nullptr; }
+    bytes = 0;
+    if (!n) return OK;
+    if (hipMalloc(&p, n) != hipSuccess) {
+        (void)hipGetLastError();
+        p = nullptr;
+        return E_NOMEM;
+    }
+    bytes = n;
+    return OK;
+}
+
+size_t Plan::device_bytes() const
+{
+    size_t b = s0.bytes + s1.bytes + tmp.bytes;
+    for (auto &t : tables) b += t->bytes;
+    return b;
+}
+
+int max_lds_bytes()
+{
+    static int cached = 0;
+    if (!cached) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+            cached = v;
+        else {
+            (void)hipGetLastError();
+            cached = 64 * 1024;
+        }
+    }
+    return cached;
+}
+
+int device_check(Error &err)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        err = {E_NODEVICE, "no HIP device available"};
+        return E_NODEVICE;
+    }
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        err = {E_NODEVICE, "cannot query the current HIP device"};
+        return E_NODEVICE;
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        err = {E_NODEVICE, std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950"};
+        return E_NODEVICE;
+    }
+    return OK;
+}
+
+// ---------------------------------------------------------------------------
+// arithmetic helpers
+// ---------------------------------------------------------------------------
+
+static bool smooth13(long long n)
+{
+    for (int p : {2, 3, 5, 7, 11, 13})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+static std::vector<int> radices_for(long long n)
+{
+    std::vector<int> r;
+    int twos = 0;
+    while (n % 2 == 0) { n /= 2; ++twos; }
+    while (twos >= 4) { r.push_back(16); twos -= 4; }
+    if (twos == 3) r.push_back(8);
+    else if (twos == 2) r.push_back(4);
+    else if (twos == 1) r.push_back(2);
+    for (int p : {3, 5, 7, 11, 13})
+        while (n % p == 0) { r.push_back(p); n /= p; }
+    return r;
+}
+
+static long long good_size7(long long n)
+{
+    long long best = LLONG_MAX;
+    for (long long f7 = 1; f7 < best; f7 *= 7)
+        for (long long f5 = f7; f5 < best; f5 *= 5)
+            for (long long f3 = f5; f3 < best; f3 *= 3) {
+                long long v = f3;
+                while (v < n) v *= 2;
+                if (v < best) best = v;
+                if (f3 >= n) break;
+            }
+    return best;
+}
+
+static long long bluestein_n2(long long L)
+{
+    const long long need = 2 * L - 1;
+    const long long s = good_size7(need);
+    long long p2 = 1;
+    while (p2 < need) p2 *= 2;
+    return (p2 * 100 <= s * 115) ? p2 : s;
+}
+
+static int threads_per_transform(long long N, const std::vector<int> &rad);
+
+// a length one LDS pass can take: 13-smooth, fits LDS and the workgroup size cap
+static bool lds_len_ok(long long n)
+{
+    return n <= kMaxLdsLen && smooth13(n) && threads_per_transform(n, radices_for(n)) <= kMaxThreads;
+}
+
+// L = L1 * L2 with both factors single-pass lengths; L2 <= L1, as balanced as possible
+static bool split_four(long long L, long long &L1, long long &L2)
+{
+    L1 = L2 = 0;
+    for (long long a = 2; a * a <= L; ++a) {
+        if (L % a) continue;
+        const long long b = L / a;
+        if (lds_len_ok(a) && lds_len_ok(b)) { L2 = a; L1 = b; }
+    }
+    return L2 > 0;
+}
+
+static int threads_per_transform(long long N, const std::vector<int> &rad)
+{
+    int nt = 1;
+    for (int r : rad) {
+        const long long nb = N / r;
+        const int per = nb_max(r);
+        nt = std::max<int>(nt, (int)((nb + per - 1) / per));
+    }
+    return nt;
+}
+
+// ---------------------------------------------------------------------------
+// jobs
+// ---------------------------------------------------------------------------
+
+enum JobKind { J_C2C = 0, J_R2C = 1, J_C2R = 2 };
+
+struct Job {
+    int kind;
+    long long L;         // transform length (real length for r2c / c2r)
+    int forward;
+    bool fct;
+    BufRef src, dst;
+    long long is_ax, os_ax;                       // scalar-unit strides along the axis
+    std::vector<long long> cnt, istr, ostr;       // batch dims, fastest first
+    long long nbatch() const
+    {
+        long long b = 1;
+        for (auto c : cnt) b *= c;
+        return b;
+    }
+};
+
+struct Builder {
+    Plan &plan;
+    Error &err;
+
+    std::shared_ptr<DevBuf> upload(const std::vector<double2> &v)
+    {
+        auto b = std::make_shared<DevBuf>();
+        if (b->alloc(v.size() * sizeof(double2)) != OK) {
+            err = {E_NOMEM, "device allocation for a twiddle table failed"};
+            return nullptr;
+        }
+        if (hipMemcpy(b->p, v.data(), v.size() * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipGetLastError();
+            err = {E_HIP, "hipMemcpy of a twiddle table failed"};
+            return nullptr;
+        }
+        plan.tables.push_back(b);
+        return b;
+    }
+
+    const double2 *table_direct(long long n)   // W_n^j, j < n
+    {
+        std::vector<double2> v;
+        twiddle_table(v, (uint64_t)n, (uint64_t)n, 1);
+        auto b = upload(v);
+        return b ? (const double2 *)b->p : nullptr;
+    }
+
+    const double2 *table_real(long long L)   // W_L^k, k <= L/2
+    {
+        std::vector<double2> v;
+        twiddle_table(v, (uint64_t)L, (uint64_t)(L / 2 + 1), 1);
+        auto b = upload(v);
+        return b ? (const double2 *)b->p : nullptr;
+    }
+
+    bool table_two_level(long long L, const double2 *&lo, const double2 *&hi, long long &S)
+    {
+        S = 1;
+        while (S * S < L) S *= 2;
+        std::vector<double2> a, b;
+        twiddle_table(a, (uint64_t)L, (uint64_t)S, 1);
+        twiddle_table(b, (uint64_t)L, (uint64_t)((L + S - 1) / S + 1), (uint64_t)S);
+        auto ba = upload(a);
+        if (!ba) return false;
+        auto bb = upload(b);
+        if (!bb) return false;
+        lo = (const double2 *)ba->p;
+        hi = (const double2 *)bb->p;
+        return true;
+    }
+
+    static FftParams base(long long N)
+    {
+        FftParams P;
+        std::memset(&P, 0, sizeof(P));
+        P.N = (int)N;
+        auto rad = radices_for(N);
+        P.nst = (int)rad.size();
+        for (size_t s = 0; s < rad.size(); ++s) P.radix[s] = rad[s];
+        P.NT = threads_per_transform(N, rad);
+        P.LD = (int)N;
+        P.ilim = P.olim = LLONG_MAX;
+        P.nload = P.nstore = (int)N;
+        P.fct = 1.0;
+        P.tw_mul = 1;
+        P.sign = -1;
+        P.ni = 1;
+        return P;
+    }
+
+    static void side_dims(Side &s, const Job &J, bool input, bool pair)
+    {
+        std::memset(&s, 0, sizeof(s));
+        s.kind = pair ? SIDE_DIMS_PAIR : SIDE_DIMS;
+        s.nbd = (int)J.cnt.size();
+        s.nphys = 1;
+        for (int d = 0; d < s.nbd; ++d) {
+            s.cnt[d] = J.cnt[d];
+            s.str[d] = input ? J.istr[d] : J.ostr[d];
+            s.nphys *= J.cnt[d];
+        }
+        s.pst = 1;
+        s.sax = input ? J.is_ax : J.os_ax;
+    }
+
+    static void side_lin(Side &s, long long lin, long long ipos, long long pst)
+    {
+        std::memset(&s, 0, sizeof(s));
+        s.kind = SIDE_LINEAR;
+        s.lin = lin;
+        s.ipos = ipos;
+        s.pst = pst;
+        s.sax = 1;
+    }
+
+    int choose_tb(const FftParams &P, long long ntf, bool col)
+    {
+        const long long per = (long long)P.LD * (long long)sizeof(double2);
+        long long budget = col ? 128 * 1024 : 64 * 1024;
+        const long long cap = max_lds_bytes() - (long long)sizeof(long long) * 8 * kMaxTB - 1024;
+        budget = std::min(budget, cap);
+        long long tb = std::max<long long>(1, budget / per);
+        tb = std::min<long long>(tb, std::max(1, kMaxThreads / P.NT));
+        tb = std::min<long long>(tb, kMaxTB);
+        const long long want = std::max<long long>(1, (ntf + 1023) / 1024);
+        if (tb > want) tb = std::max<long long>(want, col ? std::min<long long>(tb, 4) : 1);
+        return (int)tb;
+    }
+
+    bool add_fft(FftParams P, BufRef src, BufRef dst, bool uses_fct)
+    {
+        const long long per = (long long)P.LD * (long long)sizeof(double2) * P.TB;
+        const long long cap = max_lds_bytes() - (long long)sizeof(long long) * 8 * kMaxTB - 1024;
+        if (per > cap) {
+            err = {E_UNSUPPORTED, "transform does not fit in LDS"};
+            return false;
+        }
+        Pass ps;
+        ps.kind = Pass::FFT_LDS;
+        ps.fp = P;
+        ps.block = (unsigned)(P.TB * P.NT);
+        const long long grid = (P.ntf + P.TB - 1) / P.TB;
+        if (grid > INT_MAX) {
+            err = {E_UNSUPPORTED, "batch too large for one launch"};
+            return false;
+        }
+        ps.grid = (unsigned)std::max<long long>(1, grid);
+        ps.lds = (size_t)per;
+        ps.src = src;
+        ps.dst = dst;
+        ps.uses_fct = uses_fct;
+        plan.passes.push_back(ps);
+        return true;
+    }
+
+    // ---- one LDS pass over the whole job --------------------------------
+    bool single_pass(const Job &J)
+    {
+        const long long L = J.L;
+        const bool odd = (L & 1) != 0;
+        const bool real = J.kind != J_C2C;
+        const bool pair = real && odd;
+        const long long M = real ? (odd ? L : L / 2) : L;    // complex points in LDS
+        FftParams P = base(M);
+        P.tw = M > 1 ? table_direct(M) : nullptr;
+        if (M > 1 && !P.tw) return false;
+        P.fwd_flag = J.forward;
+        P.M = (int)M;
+        side_dims(P.in, J, true, pair);
+        side_dims(P.out, J, false, pair);
+        P.ntf = pair ? (J.nbatch() + 1) / 2 : J.nbatch();
+        switch (J.kind) {
+            case J_C2C:
+                P.load_mode = LD_C2C; P.store_mode = ST_C2C;
+                P.sign = J.forward ? -1 : 1;
+                P.olim = L;
+                break;
+            case J_R2C:
+                P.sign = -1;
+                if (pair) { P.load_mode = LD_R2C_PAIR; P.store_mode = ST_R2C_PAIR; P.nstore = (int)(L / 2 + 1); }
+                else {
+                    P.load_mode = LD_REAL_HALF; P.store_mode = ST_R2C_HALF; P.nstore = (int)(M + 1);
+                    P.rtw = table_real(L);
+                    if (!P.rtw) return false;
+                }
+                break;
+            default:
+                P.sign = 1;
+                P.olim = M;
+                if (pair) { P.load_mode = LD_C2R_PAIR; P.store_mode = ST_C2R_PAIR; }
+                else {
+                    P.load_mode = LD_C2R_HALF; P.store_mode = ST_C2R_HALF;
+                    P.rtw = table_real(L);
+                    if (!P.rtw) return false;
+                }
+                break;
+        }
+        const bool colL = !J.cnt.empty() && std::llabs(J.istr[0]) < std::llabs(J.is_ax);
+        const bool colS = !J.cnt.empty() && std::llabs(J.ostr[0]) < std::llabs(J.os_ax);
+        P.TB = choose_tb(P, P.ntf, colL || colS);
+        P.col_load = P.TB > 1 && colL;
+        P.col_store = P.TB > 1 && colS;
+        return add_fft(P, J.src, J.dst, J.fct);
+    }
+
+    // ---- Bluestein tables ------------------------------------------------
+    bool bluestein_tables(long long L, long long n2, const double2 *&chirp, const double2 *&spec)
+    {
+        // w[n] = exp(-i pi n^2 / L) = W_{2L}^{n^2 mod 2L}
+        std::vector<double2> w(L), b(n2, make_double2(0.0, 0.0));
+        for (long long n = 0; n < L; ++n) {
+            const uint64_t idx = (uint64_t)(((unsigned __int128)n * (unsigned __int128)n) % (unsigned __int128)(2 * L));
+            w[n] = twiddle_at(idx, (uint64_t)(2 * L));
+        }
+        for (long long m = 0; m < L; ++m) {
+            const double2 c = make_double2(w[m].x, -w[m].y);
+            b[m] = c;
+            if (m) b[n2 - m] = c;
+        }
+        auto bw = upload(w);
+        if (!bw) return false;
+        auto bs = upload(b);
+        if (!bs) return false;
+        chirp = (const double2 *)bw->p;
+        // spectrum of b, scaled by 1/n2, computed in place on the GPU
+        std::unique_ptr<Plan> sub;
+        const size_t shp[1] = {(size_t)n2};
+        const ptrdiff_t st[1] = {(ptrdiff_t)sizeof(double2)};
+        const size_t ax[1] = {0};
+        Error e2;
+        if (plan_create(sub, J_C2C, 0, 1, shp, st, st, 1, ax, 1, e2) != OK) { err = e2; return false; }
+        if (plan_execute(*sub, bs->p, bs->p, 1.0 / (double)n2, nullptr, e2) != OK) { err = e2; return false; }
+        if (hipDeviceSynchronize() != hipSuccess) {
+            (void)hipGetLastError();
+            err = {E_HIP, "Bluestein spectrum computation failed"};
+            return false;
+        }
+        spec = (const double2 *)bs->p;
+        return true;
+    }
+
+    bool bluestein_single(const Job &J, long long Ls, long long n2)
+    {
+        // Ls: complex signal length the chirp-z runs on (L, L/2 for even real)
+        const double2 *chirp = nullptr, *spec = nullptr;
+        if (!bluestein_tables(Ls, n2, chirp, spec)) return false;
+        const bool real = J.kind != J_C2C;
+        const bool pair = real && (J.L & 1);
+        FftParams P = base(n2);
+        P.tw = table_direct(n2);
+        if (!P.tw) return false;
+        P.chirp = chirp;
+        P.spec = spec;
+        P.chirp_in = P.conv = P.chirp_out = 1;
+        P.sign = -1;
+        P.fwd_flag = J.forward;
+        P.M = (int)Ls;
+        P.ilim = Ls;
+        P.nload = (int)Ls;
+        P.olim = Ls;
+        side_dims(P.in, J, true, pair);
+        side_dims(P.out, J, false, pair);
+        P.ntf = pair ? (J.nbatch() + 1) / 2 : J.nbatch();
+        switch (J.kind) {
+            case J_C2C:
+                P.load_mode = LD_C2C; P.store_mode = ST_C2C;
+                P.conj_in = P.conj_out = J.forward ? 0 : 1;
+                P.nstore = (int)Ls;
+                break;
+            case J_R2C:
+                if (pair) { P.load_mode = LD_R2C_PAIR; P.store_mode = ST_R2C_PAIR; P.nstore = (int)(Ls / 2 + 1); }
+                else {
+                    P.load_mode = LD_REAL_HALF; P.store_mode = ST_R2C_HALF; P.nstore = (int)(Ls + 1);
+                    P.rtw = table_real(J.L);
+                    if (!P.rtw) return false;
+                }
+                break;
+            default:
+                P.conj_in = P.conj_out = 1;
+                P.nstore = (int)Ls;
+                if (pair) { P.load_mode = LD_C2R_PAIR; P.store_mode = ST_C2R_PAIR; }
+                else {
+                    P.load_mode = LD_C2R_HALF; P.store_mode = ST_C2R_HALF;
+                    P.rtw = table_real(J.L);
+                    if (!P.rtw) return false;
+                }
+                break;
+        }
+        const bool colL = !J.cnt.empty() && std::llabs(J.istr[0]) < std::llabs(J.is_ax);
+        const bool colS = !J.cnt.empty() && std::llabs(J.ostr[0]) < std::llabs(J.os_ax);
+        P.TB = choose_tb(P, P.ntf, colL || colS);
+        P.col_load = P.TB > 1 && colL;
+        P.col_store = P.TB > 1 && colS;
+        return add_fft(P, J.src, J.dst, J.fct);
+    }
+
+    // ---- four-step (and four-step Bluestein) -----------------------------
+    bool ensure_ws(size_t &have, size_t need)
+    {
+        have = std::max(have, need);
+        return true;
+    }
+
+    bool four_step(const Job &J)
+    {
+        const bool real = J.kind != J_C2C;
+        const bool pair = real && (J.L & 1);
+        const long long Ls = real ? ((J.L & 1) ? J.L : J.L / 2) : J.L;   // complex signal length
+        const bool blue = !smooth13(Ls);
+        const long long n2 = blue ? bluestein_n2(Ls) : Ls;
+        long long L1, L2;
+        if (!split_four(n2, L1, L2)) {
+            err = {E_UNSUPPORTED, "transform length too long for two passes"};
+            return false;
+        }
+        const long long nlog = pair ? (J.nbatch() + 1) / 2 : J.nbatch();
+        // twiddles: W_{n2} two-level for the pass-A twiddle, W_{2 Ls} two-level
+        // for the even-real pre/post twiddle
+        const bool real_half = real && !pair;
+        const double2 *lo, *hi;
+        long long S;
+        if (!table_two_level(n2, lo, hi, S)) return false;
+        const double2 *rlo = nullptr, *rhi = nullptr;
+        long long rS = 0;
+        if (real_half && !table_two_level(2 * Ls, rlo, rhi, rS)) return false;
+        const double2 *twA = table_direct(L2), *twB = table_direct(L1);
+        if (!twA || !twB) return false;
+        const double2 *chirp = nullptr, *spec = nullptr;
+        if (blue && !bluestein_tables(Ls, n2, chirp, spec)) return false;
+
+        const size_t ws = (size_t)(nlog * n2) * sizeof(double2);
+        ensure_ws(plan.s0_bytes, ws);
+        ensure_ws(plan.s1_bytes, ws);
+
+        auto passA = [&](FftParams P) {
+            P.N = (int)L2;
+            auto rad = radices_for(L2);
+            P.nst = (int)rad.size();
+            for (size_t s = 0; s < rad.size(); ++s) P.radix[s] = rad[s];
+            P.NT = threads_per_transform(L2, rad);
+            P.LD = (int)L2;
+            P.tw = twA;
+            P.ni = L1;
+            P.ntf = L1 * nlog;
+            P.nload = P.nstore = (int)L2;
+            P.store_mode = ST_TWIDDLE;
+            P.tw_lo = lo; P.tw_hi = hi; P.tw_S = S; P.tw_L = n2; P.tw_mul = 1;
+            P.rtw_lo = rlo; P.rtw_hi = rhi; P.rtw_S = rS;
+            side_lin(P.out, n2, 1, L1);
+            P.TB = choose_tb(P, P.ntf, true);
+            P.col_load = P.col_store = P.TB > 1;
+            return P;
+        };
+        auto passB = [&](FftParams P) {
+            P.N = (int)L1;
+            auto rad = radices_for(L1);
+            P.nst = (int)rad.size();
+            for (size_t s = 0; s < rad.size(); ++s) P.radix[s] = rad[s];
+            P.NT = threads_per_transform(L1, rad);
+            P.LD = (int)L1;
+            P.tw = twB;
+            P.ni = L2;
+            P.ntf = L2 * nlog;
+            P.nload = P.nstore = (int)L1;
+            P.load_mode = LD_C2C;
+            side_lin(P.in, n2, L1, 1);
+            P.TB = choose_tb(P, P.ntf, true);
+            P.col_load = 0;
+            P.col_store = P.TB > 1;
+            return P;
+        };
+
+        FftParams P0 = base(1);
+        P0.fwd_flag = J.forward;
+        P0.M = (int)Ls;
+        // input side of pass A: the user's layout, inner index = n1 (position step L1)
+        side_dims(P0.in, J, true, pair);
+        P0.in.ipos = 1;
+        P0.in.pst = L1;
+        // output side of the final pass
+        Side outS;
+        side_dims(outS, J, false, pair);
+        outS.ipos = 1;
+        outS.pst = L2;
+
+        int lmode;
+        int dir_sign;   // sign of the FFT proper
+        switch (J.kind) {
+            case J_C2C: lmode = LD_C2C; dir_sign = J.forward ? -1 : 1; break;
+            case J_R2C: lmode = pair ? LD_R2C_PAIR : LD_REAL_HALF; dir_sign = -1; break;
+            default: lmode = pair ? LD_C2R_PAIR : LD_C2R_HALF; dir_sign = 1; break;
+        }
+        // real-half load twiddle W_{2 Ls}^{-p} is read from the two-level table
+        P0.rtw = nullptr;
+
+        if (!blue) {
+            FftParams A = passA(P0);
+            A.load_mode = lmode;
+            A.sign = dir_sign;
+            if (!add_fft(A, J.src, REF_S0, false)) return false;
+            FftParams B = passB(P0);
+            B.sign = dir_sign;
+            if (J.kind == J_R2C) {
+                side_lin(B.out, n2, 1, L2);
+                B.store_mode = ST_C2C;
+                if (!add_fft(B, REF_S0, REF_S1, false)) return false;
+                return add_post(J, pair, Ls, n2, rlo, rhi, rS);
+            }
+            B.out = outS;
+            B.store_mode = (J.kind == J_C2C) ? ST_C2C : (pair ? ST_C2R_PAIR : ST_C2R_HALF);
+            B.olim = Ls;
+            return add_fft(B, REF_S0, J.dst, J.fct);
+        }
+
+        // Bluestein: A1 (chirp in) -> B1 (x spectrum) -> A2 (inverse) -> B2 (chirp out)
+        const int conj = (J.kind == J_C2R) || (J.kind == J_C2C && !J.forward);
+        FftParams A1 = passA(P0);
+        A1.load_mode = lmode;
+        A1.sign = -1;
+        A1.chirp = chirp;
+        A1.chirp_in = 1;
+        A1.conj_in = conj;
+        A1.ilim = Ls;
+        if (!add_fft(A1, J.src, REF_S0, false)) return false;
+        FftParams B1 = passB(P0);
+        B1.sign = -1;
+        B1.store_mode = ST_MUL_TABLE;
+        B1.mtab = spec;
+        side_lin(B1.out, n2, 1, L2);
+        if (!add_fft(B1, REF_S0, REF_S1, false)) return false;
+        FftParams P1 = P0;
+        side_lin(P1.in, n2, 1, L1);
+        FftParams A2 = passA(P1);
+        A2.load_mode = LD_C2C;
+        A2.sign = 1;
+        if (!add_fft(A2, REF_S1, REF_S0, false)) return false;
+        FftParams B2 = passB(P0);
+        B2.sign = 1;
+        B2.chirp = chirp;
+        B2.chirp_out = 1;
+        B2.conj_out = conj;
+        B2.olim = Ls;
+        if (J.kind == J_R2C) {
+            side_lin(B2.out, n2, 1, L2);
+            B2.store_mode = ST_C2C;
+            if (!add_fft(B2, REF_S0, REF_S1, false)) return false;
+            return add_post(J, pair, Ls, n2, rlo, rhi, rS);
+        }
+        B2.out = outS;
+        B2.store_mode = (J.kind == J_C2C) ? ST_C2C : (pair ? ST_C2R_PAIR : ST_C2R_HALF);
+        return add_fft(B2, REF_S0, J.dst, J.fct);
+    }
+
+    bool add_post(const Job &J, bool pair, long long Ls, long long zlen, const double2 *lo, const double2 *hi,
+                  long long S)
+    {
+        Pass ps;
+        ps.kind = Pass::REAL_POST;
+        RealPostParams &R = ps.rp;
+        std::memset(&R, 0, sizeof(R));
+        R.half = pair ? 0 : 1;
+        R.M = (int)Ls;
+        R.fwd_flag = J.forward;
+        R.fct = 1.0;
+        R.ntf = pair ? (J.nbatch() + 1) / 2 : J.nbatch();
+        R.zlen = zlen;
+        side_dims(R.out, J, false, pair);
+        R.two_level = 1;
+        R.tw_lo = lo;
+        R.tw_hi = hi;
+        R.tw_S = S;
+        ps.src = REF_S1;
+        ps.dst = J.dst;
+        ps.uses_fct = J.fct;
+        plan.passes.push_back(ps);
+        return true;
+    }
+
+    bool build_job(const Job &J)
+    {
+        const bool real = J.kind != J_C2C;
+        const long long Ls = real ? ((J.L & 1) ? J.L : J.L / 2) : J.L;
+        if (lds_len_ok(Ls)) return single_pass(J);
+        if (!smooth13(Ls)) {
+            const long long n2 = bluestein_n2(Ls);
+            if (lds_len_ok(n2)) return bluestein_single(J, Ls, n2);
+        }
+        return four_step(J);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// plan creation
+// ---------------------------------------------------------------------------
+
+static bool merge_dims(Job &J)
+{
+    // drop unit dims, sort by |input stride|, merge dims that are contiguous in both arrays
+    std::vector<size_t> idx;
+    for (size_t d = 0; d < J.cnt.size(); ++d)
+        if (J.cnt[d] > 1) idx.push_back(d);
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](size_t a, size_t b) { return std::llabs(J.istr[a]) < std::llabs(J.istr[b]); });
+    std::vector<long long> c, is, os;
+    for (size_t d : idx) {
+        if (!c.empty() && c.back() * is.back() == J.istr[d] && c.back() * os.back() == J.ostr[d]) {
+            c.back() *= J.cnt[d];
+            continue;
+        }
+        c.push_back(J.cnt[d]);
+        is.push_back(J.istr[d]);
+        os.push_back(J.ostr[d]);
+    }
+    J.cnt = c;
+    J.istr = is;
+    J.ostr = os;
+    return J.cnt.size() <= (size_t)kMaxBatchDims;
+}
+
+static Job make_job(int kind, long long L, int forward, bool fct, BufRef src, BufRef dst, size_t axis,
+                    const std::vector<size_t> &shape, const std::vector<long long> &is,
+                    const std::vector<long long> &os)
+{
+    Job J;
+    J.kind = kind;
+    J.L = L;
+    J.forward = forward;
+    J.fct = fct;
+    J.src = src;
+    J.dst = dst;
+    J.is_ax = is[axis];
+    J.os_ax = os[axis];
+    for (size_t d = 0; d < shape.size(); ++d) {
+        if (d == axis) continue;
+        J.cnt.push_back((long long)shape[d]);
+        J.istr.push_back(is[d]);
+        J.ostr.push_back(os[d]);
+    }
+    return J;
+}
+
+static void span(const std::vector<size_t> &shape, const std::vector<ptrdiff_t> &st, ptrdiff_t elem,
+                 ptrdiff_t &lo, ptrdiff_t &hi)
+{
+    lo = 0;
+    hi = 0;
+    for (size_t d = 0; d < shape.size(); ++d) {
+        const ptrdiff_t e = (ptrdiff_t)(shape[d] - 1) * st[d];
+        if (e < 0) lo += e;
+        else hi += e;
+    }
+    hi += elem;
+}
+
+int plan_create(std::unique_ptr<Plan> &out, int kind, int dtype, size_t ndim, const size_t *shape,
+                const ptrdiff_t *stride_in, const ptrdiff_t *stride_out, size_t naxes, const size_t *axes,
+                int forward, Error &err)
+{
+    if (kind < 0 || kind > 2) { err = {E_INVALID, "unknown transform kind"}; return E_INVALID; }
+    if (dtype != 0) { err = {E_UNSUPPORTED, "only IMPULSE_FFT_F64 is supported"}; return E_UNSUPPORTED; }
+    if (ndim < 1 || !shape) { err = {E_INVALID, "ndim must be >= 1"}; return E_INVALID; }
+    if (!stride_in || !stride_out) { err = {E_INVALID, "stride dimension mismatch"}; return E_INVALID; }
+    if (naxes < 1 || !axes) { err = {E_INVALID, "no axes given"}; return E_INVALID; }
+    std::vector<int> seen(ndim, 0);
+    for (size_t a = 0; a < naxes; ++a) {
+        if (axes[a] >= ndim) { err = {E_INVALID, "bad axis number"}; return E_INVALID; }
+        if (++seen[axes[a]] > 1) { err = {E_INVALID, "axis specified repeatedly"}; return E_INVALID; }
+    }
+    int rc = device_check(err);
+    if (rc != OK) return rc;
+
+    auto p = std::make_unique<Plan>();
+    Plan &P = *p;
+    P.kind = kind;
+    P.dtype = dtype;
+    P.forward = forward ? 1 : 0;
+    (void)hipGetDevice(&P.device);
+    P.shape.assign(shape, shape + ndim);
+    P.sin.assign(stride_in, stride_in + ndim);
+    P.sout.assign(stride_out, stride_out + ndim);
+    P.axes.assign(axes, axes + naxes);
+    for (size_t d = 0; d < ndim; ++d)
+        if (shape[d] == 0) P.empty = true;
+
+    const ptrdiff_t in_elem = (kind == J_R2C) ? 8 : 16, out_elem = (kind == J_C2R) ? 8 : 16;
+    const size_t last = axes[naxes - 1];
+    std::vector<size_t> shape_c(P.shape);   // complex-side shape for r2c / c2r
+    if (kind != J_C2C) shape_c[last] = P.shape[last] / 2 + 1;
+    const std::vector<size_t> &shape_in = (kind == J_C2R) ? shape_c : P.shape;
+    const std::vector<size_t> &shape_out = (kind == J_R2C) ? shape_c : P.shape;
+    if (P.empty) {
+        out = std::move(p);
+        return OK;
+    }
+    for (size_t d = 0; d < ndim; ++d) {
+        if (P.sin[d] % in_elem || P.sout[d] % out_elem) {
+            err = {E_UNSUPPORTED, "strides must be multiples of the element size"};
+            return E_UNSUPPORTED;
+        }
+    }
+    span(shape_in, P.sin, in_elem, P.in_lo, P.in_hi);
+    span(shape_out, P.sout, out_elem, P.out_lo, P.out_hi);
+    {
+        size_t n = 1;
+        for (auto s : shape_out) n *= s;
+        P.out_dense = (size_t)(P.out_hi - P.out_lo) == n * (size_t)out_elem;
+    }
+
+    std::vector<long long> is(ndim), os(ndim);
+    for (size_t d = 0; d < ndim; ++d) {
+        is[d] = P.sin[d] / in_elem;
+        os[d] = P.sout[d] / out_elem;
+    }
+
+    std::vector<Job> jobs;
+    if (kind == J_C2C) {
+        for (size_t a = 0; a < naxes; ++a) {
+            const bool first = a == 0;
+            jobs.push_back(make_job(J_C2C, (long long)shape[axes[a]], P.forward, first, first ? REF_IN : REF_OUT,
+                                    REF_OUT, axes[a], P.shape, first ? is : os, os));
+        }
+    } else if (kind == J_R2C) {
+        jobs.push_back(make_job(J_R2C, (long long)shape[last], P.forward, true, REF_IN, REF_OUT, last, P.shape, is, os));
+        for (size_t a = 0; a + 1 < naxes; ++a)
+            jobs.push_back(make_job(J_C2C, (long long)shape_c[axes[a]], P.forward, false, REF_OUT, REF_OUT, axes[a],
+                                    shape_c, os, os));
+    } else {
+        std::vector<long long> ts(ndim);   // contiguous complex temporary
+        if (naxes > 1) {
+            long long acc = 1;
+            for (size_t d = ndim; d-- > 0;) {
+                ts[d] = acc;
+                acc *= (long long)shape_c[d];
+            }
+            P.tmp_bytes = (size_t)acc * 16;
+            for (size_t a = 0; a + 1 < naxes; ++a) {
+                const bool first = a == 0;
+                jobs.push_back(make_job(J_C2C, (long long)shape_c[axes[a]], P.forward, false, first ? REF_IN : REF_TMP,
+                                        REF_TMP, axes[a], shape_c, first ? is : ts, ts));
+            }
+        }
+        const std::vector<long long> &cis = naxes > 1 ? ts : is;
+        Job J = make_job(J_C2R, (long long)shape[last], P.forward, true, naxes > 1 ? REF_TMP : REF_IN, REF_OUT, last,
+                         P.shape, cis, os);
+        jobs.push_back(J);
+    }
+
+    Builder B{P, err};
+    for (auto &J : jobs) {
+        if (!merge_dims(J)) { err = {E_UNSUPPORTED, "too many batch dimensions"}; return E_UNSUPPORTED; }
+        if (!B.build_job(J)) return err.code ? err.code : E_UNSUPPORTED;
+    }
+    if (P.s0.alloc(P.s0_bytes) != OK || P.s1.alloc(P.s1_bytes) != OK || P.tmp.alloc(P.tmp_bytes) != OK) {
+        err = {E_NOMEM, "device workspace allocation failed"};
+        return E_NOMEM;
+    }
+    out = std::move(p);
+    return OK;
+}
+
+// ---------------------------------------------------------------------------
+// execution
+// ---------------------------------------------------------------------------
+
+int plan_execute(Plan &P, const void *in, void *out, double fct, hipStream_t stream, Error &err)
+{
+    if (P.empty) return OK;
+    if (!in || !out) { err = {E_INVALID, "null data pointer"}; return E_INVALID; }
+    if (in == out && P.kind != J_C2C) {
+        err = {E_INVALID, "in-place r2c/c2r is not supported"};
+        return E_INVALID;
+    }
+    if (in == out && P.sin != P.sout) { err = {E_INVALID, "stride mismatch"}; return E_INVALID; }
+    const uintptr_t ain = (P.kind == J_R2C) ? 8 : 16, aout = (P.kind == J_C2R) ? 8 : 16;
+    if (((uintptr_t)in % ain) || ((uintptr_t)out % aout)) {
+        err = {E_UNSUPPORTED, "data pointers must be aligned to the element size"};
+        return E_UNSUPPORTED;
+    }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev != P.device) { err = {E_INVALID, "plan was created on another device"}; return E_INVALID; }
+    auto resolve = [&](BufRef r) -> void * {
+        switch (r) {
+            case REF_IN: return const_cast<void *>(in);
+            case REF_OUT: return out;
+            case REF_S0: return P.s0.p;
+            case REF_S1: return P.s1.p;
+            case REF_TMP: return P.tmp.p;
+            default: return nullptr;
+        }
+    };
+    for (auto &ps : P.passes) {
+        hipError_t e;
+        if (ps.kind == Pass::FFT_LDS) {
+            FftParams fp = ps.fp;
+            fp.src = resolve(ps.src);
+            fp.dst = resolve(ps.dst);
+            fp.fct = ps.uses_fct ? fct : 1.0;
+            e = launch_fft_lds(fp, ps.grid, ps.block, ps.lds, stream);
+        } else {
+            RealPostParams rp = ps.rp;
+            rp.Z = (const double2 *)resolve(ps.src);
+            rp.dst = resolve(ps.dst);
+            rp.fct = ps.uses_fct ? fct : 1.0;
+            e = launch_real_post(rp, stream);
+        }
+        if (e != hipSuccess) {
+            err = {E_HIP, std::string("kernel launch failed: ") + hipGetErrorString(e)};
+            return E_HIP;
+        }
+    }
+    return OK;
+}
+
+int plan_execute_host(Plan &P, const void *in, void *out, double fct, Error &err)
+{
+    if (P.empty) return OK;
+    if (!in || !out) { err = {E_INVALID, "null data pointer"}; return E_INVALID; }
+    std::lock_guard<std::mutex> lk(P.host_mtx);
+    const size_t nin = (size_t)(P.in_hi - P.in_lo), nout = (size_t)(P.out_hi - P.out_lo);
+    const bool inplace = in == out;
+    if (P.stage_in.bytes < nin && P.stage_in.alloc(nin) != OK) { err = {E_NOMEM, "staging allocation failed"}; return E_NOMEM; }
+    if (!inplace && P.stage_out.bytes < nout && P.stage_out.alloc(nout) != OK) {
+        err = {E_NOMEM, "staging allocation failed"};
+        return E_NOMEM;
+    }
+    char *din = (char *)P.stage_in.p, *dout = inplace ? din : (char *)P.stage_out.p;
+    const char *hin = (const char *)in + P.in_lo;
+    char *hout = (char *)out + P.out_lo;
+    if (hipMemcpy(din, hin, nin, hipMemcpyHostToDevice) != hipSuccess) goto hip_fail;
+    if (!inplace && !P.out_dense && hipMemcpy(dout, hout, nout, hipMemcpyHostToDevice) != hipSuccess) goto hip_fail;
+    {
+        int rc = plan_execute(P, din - P.in_lo, dout - P.out_lo, fct, nullptr, err);
+        if (rc != OK) return rc;
+    }
+    if (hipDeviceSynchronize() != hipSuccess) goto hip_fail;
+    if (hipMemcpy(hout, dout, nout, hipMemcpyDeviceToHost) != hipSuccess) goto hip_fail;
+    return OK;
+hip_fail:
+    {
+        hipError_t e = hipGetLastError();
+        err = {E_HIP, std::string("HIP error during host execution: ") + hipGetErrorString(e)};
+    }
+    return E_HIP;
+}
+
+}  // namespace ifft

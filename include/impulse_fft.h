@@ -1,0 +1,155 @@
+/* impulse_fft.h -- C ABI of the MI355X (gfx950) FFT engine for impulse.
+ *
+ * Plain C types only (pointers, sizes, ints); no exceptions cross this ABI.
+ * Every entry point returns an impulse_fft_status (0 = success) unless noted.
+ *
+ * Two groups of entry points replace the two FFI surfaces impulse binds today:
+ *
+ *  (1) the C++ path, `pocketfft::c2c / r2c / c2r` as bound by
+ *      impulse/fft/cpp_pocketfft/pocketfft.nim:35-69 and called from `apply`
+ *      (impulse/fft/cpp_pocketfft/pocketfft.nim:235-277), defined in
+ *      impulse/fft/cpp_pocketfft/pocketfft_hdronly.h:3272-3390;
+ *  (2) the C path, `make_/destroy_/_forward/_backward/_length` for cfft and
+ *      rfft plans, impulse/fft/c_pocketfft/pocketfft.h:18-32, bound by
+ *      impulse/fft/c_pocketfft/pocketfft.nim:71-81.
+ *
+ * Semantics follow impulse/fft/cpp_pocketfft/README_pocketfft.md:87-147:
+ * strides are in BYTES (negative allowed), `forward` selects exp(-2 pi i ...),
+ * transforms are unnormalised and `fct` scales the result once, r2c keeps
+ * shape[axis]/2+1 bins along the last entry of `axes`, r2c with forward=0
+ * conjugates its output, c2r with forward=1 conjugates its input, and c2r
+ * ignores the imaginary part of bin 0 (and of bin n/2 for even n).
+ * Results are computed on the GPU in FP64 (IMPULSE_FFT_F64).
+ */
+#ifndef IMPULSE_FFT_H
+#define IMPULSE_FFT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IMPULSE_FFT_ABI_VERSION 1
+
+typedef enum impulse_fft_status {
+    IMPULSE_FFT_OK = 0,
+    IMPULSE_FFT_ERR_INVALID_ARG = 1, /* pocketfft's runtime_error / invalid_argument cases */
+    IMPULSE_FFT_ERR_NOMEM = 2,       /* host or device allocation failed (pocketfft bad_alloc / NULL plan) */
+    IMPULSE_FFT_ERR_HIP = 3,         /* a HIP runtime call failed */
+    IMPULSE_FFT_ERR_UNSUPPORTED = 4, /* valid request this engine does not implement */
+    IMPULSE_FFT_ERR_NO_DEVICE = 5    /* no usable gfx950 device */
+} impulse_fft_status;
+
+typedef enum impulse_fft_kind {
+    IMPULSE_FFT_C2C = 0,
+    IMPULSE_FFT_R2C = 1,
+    IMPULSE_FFT_C2R = 2
+} impulse_fft_kind;
+
+typedef enum impulse_fft_dtype {
+    IMPULSE_FFT_F64 = 0 /* double / interleaved complex<double> (Nim float64 / Complex64) */
+} impulse_fft_dtype;
+
+/* ------------------------------------------------------------------------- */
+/* n-D plans (C++ path)                                                       */
+/* ------------------------------------------------------------------------- */
+
+typedef struct impulse_fft_plan_s *impulse_fft_plan;
+
+/* Plan an n-D transform.  `shape` is the array shape; for R2C it is the real
+ * INPUT shape and for C2R the real OUTPUT shape (README_pocketfft.md:87-92).
+ * The strides describe the input and output arrays this plan will be executed
+ * on.  Argument checks mirror pocketfft_hdronly.h:446-477 (ndim >= 1, equal
+ * stride ranks, axis < ndim, no repeated axis).
+ * Replaces: the plan lookup + checks inside pocketfft::c2c/r2c/c2r
+ * (pocketfft_hdronly.h:3272-3390, get_plan at :2655-2706). */
+int impulse_fft_plan_create(impulse_fft_plan *plan, int kind, int dtype, size_t ndim,
+                            const size_t *shape, const ptrdiff_t *stride_in,
+                            const ptrdiff_t *stride_out, size_t naxes, const size_t *axes,
+                            int forward);
+
+/* Execute on DEVICE pointers, asynchronously on `stream` (a hipStream_t, NULL =
+ * the default stream).  Calls of one plan share its workspace: calls of the
+ * same plan that may overlap must be ordered by the caller (same stream).
+ * in == out is allowed for C2C when both stride sets are equal. */
+int impulse_fft_execute(impulse_fft_plan plan, const void *data_in, void *data_out, double fct,
+                        void *stream);
+
+/* Execute on HOST pointers: copies in, runs, copies out, synchronous.  This is
+ * the drop-in for pocketfft's host-memory calls. */
+int impulse_fft_execute_host(impulse_fft_plan plan, const void *data_in, void *data_out,
+                             double fct);
+
+/* Bytes of device workspace this plan holds (twiddles, chirps, scratch). */
+size_t impulse_fft_plan_device_bytes(impulse_fft_plan plan);
+
+/* Number of kernel launches one execute issues (HBM passes plus helpers). */
+int impulse_fft_plan_num_launches(impulse_fft_plan plan);
+
+int impulse_fft_plan_destroy(impulse_fft_plan plan);
+
+/* One-shot host-pointer calls with the exact argument list of pocketfft::c2c,
+ * r2c and c2r (pocketfft_hdronly.h:3272, :3334, :3366 as bound by
+ * cpp_pocketfft/pocketfft.nim:35-69), plus `dtype`.  Plans are taken from a
+ * thread-safe LRU cache keyed by (kind, dtype, shape, strides, axes,
+ * direction, device).  `nthreads` is accepted and ignored: the GPU grid
+ * replaces PocketFFT's thread pool (pocketfft_hdronly.h:499-705). */
+int impulse_fft_c2c(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in,
+                    const ptrdiff_t *stride_out, size_t naxes, const size_t *axes, int forward,
+                    int dtype, const void *data_in, void *data_out, double fct, size_t nthreads);
+int impulse_fft_r2c(size_t ndim, const size_t *shape_in, const ptrdiff_t *stride_in,
+                    const ptrdiff_t *stride_out, size_t naxes, const size_t *axes, int forward,
+                    int dtype, const void *data_in, void *data_out, double fct, size_t nthreads);
+int impulse_fft_c2r(size_t ndim, const size_t *shape_out, const ptrdiff_t *stride_in,
+                    const ptrdiff_t *stride_out, size_t naxes, const size_t *axes, int forward,
+                    int dtype, const void *data_in, void *data_out, double fct, size_t nthreads);
+
+/* Plan cache control (PocketFFT's is a fixed 16-entry LRU, pocketfft_hdronly.h:50-52). */
+int impulse_fft_cache_set_capacity(size_t entries);
+size_t impulse_fft_cache_size(void);
+int impulse_fft_cache_clear(void);
+
+/* ------------------------------------------------------------------------- */
+/* 1-D in-place plans on host memory (C path)                                 */
+/* ------------------------------------------------------------------------- */
+
+typedef struct impulse_cfft_plan_s *impulse_cfft_plan;
+typedef struct impulse_rfft_plan_s *impulse_rfft_plan;
+
+/* Replaces make_cfft_plan / destroy_cfft_plan / cfft_forward / cfft_backward /
+ * cfft_length (c_pocketfft/pocketfft.h:18-23).  `c` holds `length` interleaved
+ * complex doubles; the transform is done in place and scaled by `fct`.
+ * make_* returns NULL on failure (length 0, no device, out of memory);
+ * *_forward/_backward return 0 on success and -1 on failure, as PocketFFT. */
+impulse_cfft_plan impulse_make_cfft_plan(size_t length);
+void impulse_destroy_cfft_plan(impulse_cfft_plan plan);
+int impulse_cfft_forward(impulse_cfft_plan plan, double c[], double fct);
+int impulse_cfft_backward(impulse_cfft_plan plan, double c[], double fct);
+size_t impulse_cfft_length(impulse_cfft_plan plan);
+
+/* Replaces make_rfft_plan / destroy_rfft_plan / rfft_forward / rfft_backward /
+ * rfft_length (c_pocketfft/pocketfft.h:25-31).  `c` holds `length` doubles in
+ * FFTPACK halfcomplex order [r0, r1, i1, r2, i2, ...(, r_{n/2})], the layout
+ * impulse/fft/c_pocketfft/pocketfft.nim:224-243 documents. */
+impulse_rfft_plan impulse_make_rfft_plan(size_t length);
+void impulse_destroy_rfft_plan(impulse_rfft_plan plan);
+int impulse_rfft_forward(impulse_rfft_plan plan, double c[], double fct);
+int impulse_rfft_backward(impulse_rfft_plan plan, double c[], double fct);
+size_t impulse_rfft_length(impulse_rfft_plan plan);
+
+/* ------------------------------------------------------------------------- */
+/* Diagnostics                                                                */
+/* ------------------------------------------------------------------------- */
+
+/* Static text for a status code. */
+const char *impulse_fft_status_string(int status);
+/* Message of the last failing call on this thread ("" if none). */
+const char *impulse_fft_last_error(void);
+/* 0 if a gfx950 device is usable, else a status code (no kernel is launched). */
+int impulse_fft_device_check(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IMPULSE_FFT_H */

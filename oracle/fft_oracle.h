@@ -1,0 +1,68 @@
+/* CPU oracle for the impulse FFT path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This library restates, in plain C with long-double arithmetic, what the
+ * reference's FFT path computes.  It is the checker for the HIP engine: only
+ * tests, the smoke check and the CPU-baseline leg of the benchmark may load it.
+ * The product library never links, loads or calls it.
+ *
+ * Semantics followed (reference files, relative to the impulse repository):
+ *   - n-D c2c / r2c / c2r with byte strides, axes order, `fct` applied once on
+ *     the first transformed axis:      impulse/fft/cpp_pocketfft/pocketfft_hdronly.h:3011-3050,3272-3390
+ *   - r2c keeps n/2+1 bins; forward=false conjugates the r2c output:  pocketfft_hdronly.h:3125-3179
+ *   - c2r ignores Im of DC (and of Nyquist for even n); forward=true
+ *     conjugates its input:                                    pocketfft_hdronly.h:3180-3250
+ *   - argument checks (ndim, stride ranks, axes):             pocketfft_hdronly.h:446-477
+ *   - C path (`rfft_forward/backward`, FFTPACK halfcomplex order
+ *     [r0, r1, i1, r2, i2, ...]), `cfft_forward/backward`:     impulse/fft/c_pocketfft/pocketfft.h:18-32,
+ *                                                              impulse/fft/c_pocketfft/pocketfft.nim:224-243
+ *   - forward transform = exp(-2 pi i jk/n), unnormalised:     impulse/fft/cpp_pocketfft/README_pocketfft.md:102-116
+ *
+ * The arithmetic is a textbook recursive mixed-radix DFT (smallest prime factor
+ * first, naive prime-size butterflies) with Bluestein's chirp-z algorithm for
+ * prime lengths above 64, all in long double with octant-reduced twiddles, so
+ * it is independent of the GPU engine's radix order and of PocketFFT's own
+ * FFTPACK/Bluestein choice (impulse/fft/cpp_pocketfft/pocketfft_hdronly.h:2349-2417).
+ *
+ * Return codes: 0 ok, 1 invalid argument, 2 out of memory.
+ */
+#ifndef IMPULSE_FFT_ORACLE_H
+#define IMPULSE_FFT_ORACLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* long-double naive O(n^2) DFT of interleaved complex doubles (brute force pin). */
+int oracle_dft_naive(size_t n, const double *in, double *out, int forward);
+
+/* 1-D complex FFT, in place on interleaved (re,im) doubles, result scaled by fct
+ * (the C path leaves length-1 data untouched, fct included: pocketfft.c:874). */
+int oracle_cfft_forward(size_t n, double *c, double fct);
+int oracle_cfft_backward(size_t n, double *c, double fct);
+
+/* 1-D real FFT in place in FFTPACK halfcomplex order, scaled by fct (not for n == 1,
+ * pocketfft.c:1703,1739). */
+int oracle_rfft_forward(size_t n, double *c, double fct);
+int oracle_rfft_backward(size_t n, double *c, double fct);
+
+/* n-D transforms; strides in bytes, `shape` is the real array's shape for r2c/c2r. */
+int oracle_c2c(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in,
+               const ptrdiff_t *stride_out, size_t naxes, const size_t *axes,
+               int forward, const double *data_in, double *data_out, double fct);
+int oracle_r2c(size_t ndim, const size_t *shape_in, const ptrdiff_t *stride_in,
+               const ptrdiff_t *stride_out, size_t naxes, const size_t *axes,
+               int forward, const double *data_in, double *data_out, double fct);
+int oracle_c2r(size_t ndim, const size_t *shape_out, const ptrdiff_t *stride_in,
+               const ptrdiff_t *stride_out, size_t naxes, const size_t *axes,
+               int forward, const double *data_in, double *data_out, double fct);
+
+/* Twiddle exp(sign * 2 pi i k / n) in long double, exact at multiples of pi/4
+ * (returned through re/im as doubles of the long-double value). */
+void oracle_twiddle(size_t k, size_t n, int sign, double *re, double *im);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

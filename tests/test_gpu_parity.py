@@ -1,0 +1,317 @@
+"""GPU parity of the HIP engine against the oracle and the reference's golden
+vectors, through the C ABI (host-pointer and device-pointer entry points).
+
+Bars: rel-L2 <= 1e-12 against the FP64 oracle / PocketFFT golden vectors,
+round trips <= 1e-10, the reference's exact known answers bit-exact.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from impulse_amd import _lib, synth
+from impulse_amd import fft as F
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    torch.cuda.set_device(0)
+    _lib.check(_lib.lib().impulse_fft_device_check())
+    yield
+    torch.cuda.synchronize()
+
+
+def _torch():
+    import torch
+    return torch
+
+
+# ---------------------------------------------------------------------------
+# known answers (tests/test_fft2.nim:7-15, README.md:41-98)
+# ---------------------------------------------------------------------------
+
+def test_kat_fft2_exact():
+    exp = np.array([6, -2 + 2j, -2, -2 - 2j])
+    x = np.arange(4.0)
+    assert np.array_equal(F.fft(x.astype(np.complex128)), exp)          # complex input
+    assert np.array_equal(F.fft(x), exp)                               # real input
+    assert np.array_equal(F.fft(x, forward=True), exp)
+    assert np.array_equal(F.ifft(F.fft(x)).real, x)
+    assert np.array_equal(F.ifft(F.fft(x)).imag, np.zeros(4))
+    assert np.array_equal(F.fft(x, normalize=F.nkForward), exp / 4)
+    assert np.array_equal(F.fft(x, normalize=F.nkOrtho), exp / 2)
+    assert np.array_equal(F.ifft(x), F.fft(x, forward=False))
+
+
+def test_kat_readme():
+    y = F.fft(np.array([1.0, 2.0, 1.0, -1.0, 1.5]))
+    exp = np.array([4.5, 2.081559480312316 - 1.651098762732523j, -1.831559480312316 + 1.608220406444071j,
+                    -1.831559480312316 - 1.608220406444071j, 2.081559480312316 + 1.651098762732523j])
+    assert np.max(np.abs(y - exp)) < 1e-15
+    # C++ path r2c into a full-length complex buffer leaves the tail untouched (README.md:98)
+    d_in = np.array([1.0, 2.0, 1.0, -1.0, 1.5])
+    d_out = np.zeros(5, np.complex128)
+    F.apply(F.FFTDesc([0], True), F.DataDesc(d_out, [5]), F.DataDesc(d_in, [5]))
+    assert np.max(np.abs(d_out[:3] - exp[:3])) < 1e-15
+    assert np.array_equal(d_out[3:], np.zeros(2))
+
+
+# ---------------------------------------------------------------------------
+# golden vectors written by the reference's own PocketFFT
+# ---------------------------------------------------------------------------
+
+def test_golden_1d(golden):
+    lengths = sorted({int(k.split("_")[1]) for k in golden.files if k.startswith("c2c_") and k.endswith("_in")})
+    for n in lengths:
+        x = golden[f"c2c_{n}_in"]
+        assert rel_l2(F.c2c(x, [0], True), golden[f"c2c_{n}_fwd"]) < TOL, n
+        r = golden[f"r2c_{n}_in"]
+        assert rel_l2(F.r2c(r, [0], True), golden[f"r2c_{n}_out"]) < TOL, n
+        assert rel_l2(F.rfft_packed(r, True), golden[f"hc_{n}_out"]) < TOL, n
+        if f"c2c_{n}_bwd" in golden.files:
+            assert rel_l2(F.c2c(x, [0], False, 0.5), golden[f"c2c_{n}_bwd"]) < TOL, n
+            assert rel_l2(F.r2c(r, [0], False, 2.0), golden[f"r2c_{n}_bwd"]) < TOL, n
+            c = golden[f"c2r_{n}_in"]
+            assert rel_l2(F.c2r(c, n, [0], False, 1.0 / n), golden[f"c2r_{n}_out"]) < TOL, n
+            assert rel_l2(F.c2r(c, n, [0], True), golden[f"c2r_{n}_fwdflag"]) < TOL, n
+
+
+def test_golden_nd(golden):
+    a = golden["nd_c2c_in"]
+    assert rel_l2(F.c2c(a[:, ::2], [0, 1], True), golden["nd_c2c_01"]) < TOL
+    assert rel_l2(F.c2c(a[:, ::2], [1, 0], False, 0.25), golden["nd_c2c_10_bwd"]) < TOL
+    b = golden["nd_r2c_in"]
+    assert rel_l2(F.r2c(b, [0, 1, 2]), golden["nd_r2c_012"]) < TOL
+    assert rel_l2(F.r2c(b, [2, 0]), golden["nd_r2c_20"]) < TOL
+    assert rel_l2(F.r2c(b, [1], False), golden["nd_r2c_1_bwd"]) < TOL
+    h = golden["nd_c2r_in"]
+    assert rel_l2(F.c2r(h, 9, [0, 1, 2], False, 1.0 / 540), golden["nd_c2r_012"]) < TOL
+    assert rel_l2(F.c2r(h, 8, [2], False), golden["nd_c2r_2"]) < TOL
+    for n in (4096, 4095):
+        assert rel_l2(F.r2c(golden[f"rows_{n}_in"], [1]), golden[f"rows_{n}_out"]) < TOL
+
+
+# ---------------------------------------------------------------------------
+# every algorithm branch against the oracle
+# ---------------------------------------------------------------------------
+
+# single pass (smooth <= 8192), LDS Bluestein (n2 <= 8192), four-step, four-step Bluestein
+LENGTHS = (list(range(1, 130)) + [143, 169, 243, 360, 1000, 1001, 1024, 2017, 2053, 4095, 4096, 6561, 8192,
+                                  8190, 10000, 4099, 10007, 16384, 30030, 65536, 20014, 131072])
+
+
+@pytest.mark.parametrize("n", LENGTHS)
+def test_lengths_vs_oracle(n):
+    print(f"case n={n}", flush=True)
+    batch = 3
+    x = synth.complex_normal((batch, n), seed=n)
+    for fwd in (True, False):
+        assert rel_l2(F.c2c(x, [1], fwd, 0.5), oracle.c2c(x, [1], fwd, 0.5)) < TOL, ("c2c", n, fwd)
+    r = synth.uniform((batch, n), -1, 1, seed=n + 1)
+    for fwd in (True, False):
+        assert rel_l2(F.r2c(r, [1], fwd), oracle.r2c(r, [1], fwd)) < TOL, ("r2c", n, fwd)
+    h = synth.complex_normal((batch, n // 2 + 1), seed=n + 2)
+    for fwd in (False, True):
+        assert rel_l2(F.c2r(h, n, [1], fwd, 1.0 / n), oracle.c2r(h, n, [1], fwd, 1.0 / n)) < TOL, ("c2r", n, fwd)
+
+
+@pytest.mark.parametrize("n", [6, 64, 97, 4095, 4096, 10007, 65536])
+def test_strided_axis_vs_oracle(n):
+    # transform along axis 0 of [n, 5] (batch contiguous, axis strided) and of a
+    # negative-stride view
+    print(f"case strided n={n}", flush=True)
+    x = synth.complex_normal((n, 5), seed=n)
+    assert rel_l2(F.c2c(x, [0]), oracle.c2c(x, [0])) < TOL
+    v = x[::-1, ::2]
+    assert rel_l2(F.c2c(v, [0], False), oracle.c2c(v, [0], False)) < TOL
+    r = synth.uniform((n, 4), -1, 1, seed=n)
+    assert rel_l2(F.r2c(r, [0]), oracle.r2c(r, [0])) < TOL
+    h = synth.complex_normal((n // 2 + 1, 3), seed=n)
+    assert rel_l2(F.c2r(h, n, [0], False), oracle.c2r(h, n, [0], False)) < TOL
+
+
+def test_nd_mixed():
+    x = synth.complex_normal((6, 35, 16), seed=3)
+    for axes in ([0], [1, 2], [2, 0, 1], [0, 2]):
+        assert rel_l2(F.c2c(x, axes), oracle.c2c(x, axes)) < TOL, axes
+    r = synth.uniform((5, 12, 9), -1, 1, seed=4)
+    for axes in ([2], [0, 2], [2, 1, 0], [1]):
+        assert rel_l2(F.r2c(r, axes, False, 3.0), oracle.r2c(r, axes, False, 3.0)) < TOL, axes
+    h = synth.complex_normal((5, 12, 5), seed=5)
+    for axes in ([2], [0, 2], [1, 0, 2]):
+        assert rel_l2(F.c2r(h, 9, axes, False), oracle.c2r(h, 9, axes, False)) < TOL, axes
+    # in place c2c on host memory
+    y = x.copy()
+    F.c2c(y, [1], True, 1.0, out=y)
+    assert rel_l2(y, oracle.c2c(x, [1])) < TOL
+
+
+# ---------------------------------------------------------------------------
+# the reference's C-path round trip (tests/test_fft.nim:29-49) over every length
+# ---------------------------------------------------------------------------
+
+def test_c_path_round_trip_all_lengths():
+    maxlen = 8192
+    odata = synth.uniform((maxlen,), -0.5, 0.5, seed=0)
+    odata[0] = 0.340188
+    worst, worst_n = 0.0, 0
+    for length in range(1, maxlen):
+        data = odata[:length].copy()
+        plan = F.FFTPlanReal.init(length)
+        plan.call(data, True, 1.0)
+        plan.call(data, False, 1.0 / length)
+        err = rel_l2(data, odata[:length])
+        if err > worst:
+            worst, worst_n = err, length
+    print(f"C-path round trip: worst rel-L2 {worst:.3e} at length {worst_n}")
+    assert worst <= 1e-13, (worst, worst_n)
+
+
+def test_c_path_halfcomplex_vs_oracle():
+    for n in (1, 2, 3, 7, 16, 97, 4095, 4096, 10007):
+        x = synth.uniform((n,), -1, 1, seed=n)
+        for fwd, fct in ((True, 1.0), (False, 0.5)):
+            assert rel_l2(F.rfft_packed(x, fwd, F.nkCustom, fct), oracle.rfft_packed(x, fwd, fct)) < TOL, n
+        z = synth.complex_normal((n,), seed=n)
+        for fwd in (True, False):
+            c = z.copy()
+            F.fft_inplace(c, fwd, F.nkCustom, 0.25)
+            assert rel_l2(c, oracle.cfft(z, fwd, 0.25)) < TOL, n
+
+
+# ---------------------------------------------------------------------------
+# device pointers (C++ path over torch tensors) and the full-size workloads
+# ---------------------------------------------------------------------------
+
+def _dev(a):
+    return _torch().from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def test_device_r2c_rows_4096_4095():
+    torch = _torch()
+    for n in (4096, 4095):
+        print(f"case rows [256,{n}]", flush=True)
+        x = synth.uniform((256, n), -1.0, 1.0, seed=0)
+        xd = _dev(x)
+        yd = torch.empty((256, n // 2 + 1), dtype=torch.complex128, device="cuda")
+        F.apply(F.FFTDesc([1], True), F.DataDesc(yd, yd.shape), F.DataDesc(xd, xd.shape))
+        zd = torch.empty_like(xd)
+        F.apply(F.FFTDesc([1], False, 1.0 / n), F.DataDesc(zd, zd.shape), F.DataDesc(yd, yd.shape))
+        torch.cuda.synchronize()
+        y = yd.cpu().numpy()
+        ref = oracle.r2c(x, [1])
+        errs = [rel_l2(y[i], ref[i]) for i in range(256)]
+        print(f"[256,{n}] r2c rel-L2 max {max(errs):.3e} median {np.median(errs):.3e}")
+        assert max(errs) <= TOL
+        assert rel_l2(zd.cpu().numpy(), x) <= 1e-10
+
+
+def test_device_nondefault_stream():
+    torch = _torch()
+    x = synth.complex_normal((64, 1024), seed=1)
+    xd, yd = _dev(x), torch.empty((64, 1024), dtype=torch.complex128, device="cuda")
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        F.apply(F.FFTDesc([1], True), F.DataDesc(yd, yd.shape), F.DataDesc(xd, xd.shape))
+    s.synchronize()
+    assert rel_l2(yd.cpu().numpy(), oracle.c2c(x, [1])) < TOL
+
+
+def test_c2c_4096x4096_axes01():
+    torch = _torch()
+    n = 4096
+    print("case c2c 4096x4096", flush=True)
+    # separability: the 2-D DFT of an outer product u v^T is (F u)(F v)^T
+    u = synth.complex_normal((n,), seed=1)
+    v = synth.complex_normal((n,), seed=2)
+    xd = torch.outer(_dev(u), _dev(v))
+    yd = torch.empty_like(xd)
+    F.apply(F.FFTDesc([0, 1], True), F.DataDesc(yd, yd.shape), F.DataDesc(xd, xd.shape))
+    torch.cuda.synchronize()
+    Fu, Fv = oracle.c2c(u, [0]), oracle.c2c(v, [0])
+    ref = torch.outer(_dev(Fu), _dev(Fv))
+    err = (torch.linalg.vector_norm(yd - ref) / torch.linalg.vector_norm(ref)).item()
+    print(f"4096^2 separable rel-L2 {err:.3e}")
+    assert err <= TOL
+    # random input: exact DFT sums at sampled bins, then the round trip
+    x = synth.complex_normal((n, n), seed=0)
+    xd = _dev(x)
+    F.apply(F.FFTDesc([0, 1], True), F.DataDesc(yd, yd.shape), F.DataDesc(xd, xd.shape))
+    torch.cuda.synchronize()
+    y = yd.cpu().numpy()
+    k = np.arange(n)
+    rng = np.random.default_rng(0)
+    for _ in range(4):
+        k1, k2 = rng.integers(0, n, 2)
+        w1 = np.exp(-2j * np.pi * ((k1 * k) % n) / n)
+        w2 = np.exp(-2j * np.pi * ((k2 * k) % n) / n)
+        direct = w1 @ x @ w2
+        assert abs(y[k1, k2] - direct) <= 1e-12 * np.linalg.norm(y) / n, (k1, k2)
+    zd = torch.empty_like(xd)
+    F.apply(F.FFTDesc([0, 1], False, 1.0 / (n * n)), F.DataDesc(zd, zd.shape), F.DataDesc(yd, yd.shape))
+    torch.cuda.synchronize()
+    rt = (torch.linalg.vector_norm(zd - xd) / torch.linalg.vector_norm(xd)).item()
+    print(f"4096^2 round trip rel-L2 {rt:.3e}")
+    assert rt <= 1e-12
+
+
+@pytest.mark.parametrize("n,batch", [(1 << 22, 16), (1000003, 4)])
+def test_r2c_long(n, batch):
+    torch = _torch()
+    print(f"case r2c [{batch},{n}]", flush=True)
+    x = synth.uniform((batch, n), -1.0, 1.0, seed=0)
+    xd = _dev(x)
+    yd = torch.empty((batch, n // 2 + 1), dtype=torch.complex128, device="cuda")
+    F.apply(F.FFTDesc([1], True), F.DataDesc(yd, yd.shape), F.DataDesc(xd, xd.shape))
+    torch.cuda.synchronize()
+    for row in (0, batch - 1):
+        ref = oracle.r2c(x[row], [0])
+        err = rel_l2(yd[row].cpu().numpy(), ref)
+        print(f"r2c [{batch},{n}] row {row} rel-L2 {err:.3e}")
+        assert err <= TOL
+    zd = torch.empty_like(xd)
+    F.apply(F.FFTDesc([1], False, 1.0 / n), F.DataDesc(zd, zd.shape), F.DataDesc(yd, yd.shape))
+    torch.cuda.synchronize()
+    rt = (torch.linalg.vector_norm(zd - xd) / torch.linalg.vector_norm(xd)).item()
+    print(f"r2c [{batch},{n}] round trip rel-L2 {rt:.3e}")
+    assert rt <= 1e-10
+
+
+# ---------------------------------------------------------------------------
+# error behaviour (pocketfft_hdronly.h:446-477 cases, forced from the host)
+# ---------------------------------------------------------------------------
+
+def _create(kind, shape, sin, sout, axes, fwd=1):
+    h = ctypes.c_void_p()
+    L = _lib.lib()
+    rc = L.impulse_fft_plan_create(ctypes.byref(h), kind, 0, len(shape), _lib.sizes(shape), _lib.strides(sin),
+                                   _lib.strides(sout), len(axes), _lib.sizes(axes), fwd)
+    if rc == 0:
+        L.impulse_fft_plan_destroy(h)
+    return rc
+
+
+def test_error_codes():
+    L = _lib.lib()
+    assert _create(0, [4, 4], [64, 16], [64, 16], [2]) == 1          # bad axis number
+    assert b"bad axis" in L.impulse_fft_last_error()
+    assert _create(0, [4, 4], [64, 16], [64, 16], [1, 1]) == 1       # axis specified repeatedly
+    assert b"repeatedly" in L.impulse_fft_last_error()
+    assert _create(0, [4], [16], [16], []) == 1                       # no axes
+    assert _create(5, [4], [16], [16], [0]) == 1                      # unknown kind
+    assert _create(0, [4], [8], [16], [0]) == 4                       # misaligned complex stride
+    assert _create(0, [0, 4], [64, 16], [64, 16], [1]) == 0           # empty array: no-op plan
+    x = np.zeros(8)
+    rc = L.impulse_fft_r2c(1, _lib.sizes([8]), _lib.strides([8]), _lib.strides([16]), 1, _lib.sizes([0]), 1, 0,
+                           x.ctypes.data, x.ctypes.data, 1.0, 1)
+    assert rc == 1                                                    # in-place r2c is rejected
+    assert not L.impulse_make_rfft_plan(0)
+    assert not L.impulse_make_cfft_plan(0)

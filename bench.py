@@ -1,0 +1,219 @@
+"""Throughput of the FP64 drop-in FFT on one MI355X.
+
+One step = one pass of the hot path over the plain FP64 GPU drop-in workload
+at scale, inputs resident in HBM:
+  * 2-D complex128 c2c over axes [0, 1] of 4096 x 4096, re/im iid N(0, 1)
+  * 1-D r2c float64, length 2^22, batch 16, uniform[-1, 1]
+  * 1-D r2c float64, length 1,000,003 (prime -> Bluestein), batch 4, uniform[-1, 1]
+value = algorithmic GFLOP (5 N log2 N per complex transform, 2.5 N log2 N per
+real one, N nominal; 2-D N = all points) / time, summed over the three parts.
+
+The "roofline" object describes the dominant kernel: its launches are timed
+with HIP events on the execution stream inside the timed region, and its
+algorithmic bytes per launch are the bytes each launch must read plus write
+(input read once, output written once).  cpu_baseline times the reference
+PocketFFT C++ (built from the reference sources into oracle/_ref) on a bounded
+sample of the same workload on this host's cores.
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "FFT GFLOP/s (5NlogN) + %HBM roofline; split-FP32 vs FP64 at ≤1e-10 rel"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def c_strides(shape, itemsize):
+    st, acc = [0] * len(shape), itemsize
+    for i in range(len(shape) - 1, -1, -1):
+        st[i] = acc
+        acc *= shape[i]
+    return st
+
+
+class Part:
+    def __init__(self, name, kind, shape, axes, in_dtype, out_shape, out_dtype, flops, comp_bytes, make_input):
+        self.name, self.kind, self.shape, self.axes = name, kind, list(shape), list(axes)
+        self.in_dtype, self.out_shape, self.out_dtype = in_dtype, list(out_shape), out_dtype
+        self.flops, self.comp_bytes, self.make_input = flops, comp_bytes, make_input
+
+
+def parts():
+    from impulse_amd import synth
+    n2d = 4096
+    nl, bl = 1 << 22, 16
+    nb, bb = 1000003, 4
+    return [
+        Part("c2c_4096x4096_axes01", 0, (n2d, n2d), (0, 1), "c128", (n2d, n2d), "c128",
+             5.0 * n2d * n2d * math.log2(n2d * n2d), 2 * n2d * n2d * 16,
+             lambda: synth.complex_normal((n2d, n2d), seed=0)),
+        Part("r2c_2^22_x16", 1, (bl, nl), (1,), "f64", (bl, nl // 2 + 1), "c128",
+             bl * 2.5 * nl * math.log2(nl), bl * (nl * 8 + (nl // 2 + 1) * 16),
+             lambda: synth.uniform((bl, nl), -1.0, 1.0, seed=0)),
+        Part("r2c_1000003_x4", 1, (bb, nb), (1,), "f64", (bb, nb // 2 + 1), "c128",
+             bb * 2.5 * nb * math.log2(nb), bb * (nb * 8 + (nb // 2 + 1) * 16),
+             lambda: synth.uniform((bb, nb), -1.0, 1.0, seed=0)),
+    ]
+
+
+def torch_dtype(torch, s):
+    return {"c128": torch.complex128, "f64": torch.float64}[s]
+
+
+def cpu_baseline(plist, budget_s):
+    """Reference PocketFFT C++ (oracle/_ref) on this host, same workload, bounded time."""
+    from oracle import ref
+    if not ref.available():
+        return None
+    threads = max(1, min(16, os.cpu_count() or 1))
+    data = []
+    for p in plist:
+        x = p.make_input()
+        data.append(x)
+    total_flops, total_s, reps = 0.0, 0.0, 0
+    t_start = time.perf_counter()
+    while True:
+        for p, x in zip(plist, data):
+            t0 = time.perf_counter()
+            if p.kind == 0:
+                ref.c2c(x, p.axes, True, 1.0, threads)
+            else:
+                ref.r2c(x, p.axes, True, 1.0, threads)
+            total_s += time.perf_counter() - t0
+            total_flops += p.flops
+        reps += 1
+        if time.perf_counter() - t_start > budget_s or reps >= 50:
+            break
+    return {"value": total_flops / total_s / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "reference",
+            "sample": f"the full 3-part workload repeated {reps}x on the host "
+                      f"(reference PocketFFT C++ c2c/r2c, nthreads={threads}, "
+                      f"built -O3 -march=x86-64-v3 from the reference sources)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from impulse_amd import _lib
+    from impulse_amd import fft as F
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    L = _lib.lib()
+    _lib.check(L.impulse_fft_device_check())
+    plist = parts()
+
+    # device-resident inputs and outputs, plans
+    bufs, plans = [], []
+    for p in plist:
+        x = torch.from_numpy(p.make_input()).cuda()
+        y = torch.empty(p.out_shape, dtype=torch_dtype(torch, p.out_dtype), device="cuda")
+        ib = 16 if p.in_dtype == "c128" else 8
+        pl = F.Plan(p.kind, p.shape, c_strides(p.shape, ib), c_strides(p.out_shape, 16), p.axes, True)
+        bufs.append((x, y))
+        plans.append(pl)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    def step():
+        for pl, (x, y) in zip(plans, bufs):
+            pl.execute(x.data_ptr(), y.data_ptr(), 1.0, sp)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    # timed region: every launch is bracketed by HIP events on `stream` by the library
+    part_ms = [0.0] * len(plist)
+    for pl in plans:
+        _lib.check(L.impulse_fft_plan_profile(pl.handle, args.steps))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    launches = []
+    for pi_, (p, pl) in enumerate(zip(plist, plans)):
+        n = pl.num_launches
+        ms = (ctypes.c_double * n)()
+        cnt = ctypes.c_int()
+        _lib.check(L.impulse_fft_plan_profile_read(pl.handle, ms, n, ctypes.byref(cnt)))
+        for k in range(n):
+            name = ctypes.create_string_buffer(64)
+            bi, bo, gr, bl_, lds = (ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_uint(), ctypes.c_uint(),
+                                    ctypes.c_size_t())
+            _lib.check(L.impulse_fft_plan_pass_info(pl.handle, k, name, 64, ctypes.byref(bi), ctypes.byref(bo),
+                                                    ctypes.byref(gr), ctypes.byref(bl_), ctypes.byref(lds)))
+            avg = ms[k] / max(1, cnt.value)
+            launches.append({"part": p.name, "pass": k, "kernel": name.value.decode(), "ms": avg,
+                             "bytes": bi.value + bo.value, "grid": gr.value, "block": bl_.value,
+                             "lds": lds.value})
+            part_ms[pi_] += avg
+        _lib.check(L.impulse_fft_plan_profile(pl.handle, 0))
+
+    ms_per_step = elapsed * 1e3 / args.steps
+    total_flops = sum(p.flops for p in plist)
+    value = total_flops / (ms_per_step * 1e-3) / 1e9
+
+    # dominant kernel: the kernel name with the largest summed launch time
+    by_kernel = {}
+    for l_ in launches:
+        k = by_kernel.setdefault(l_["kernel"], {"ms": 0.0, "bytes": 0, "n": 0})
+        k["ms"] += l_["ms"]
+        k["bytes"] += l_["bytes"]
+        k["n"] += 1
+    dom = max(by_kernel, key=lambda k: by_kernel[k]["ms"])
+    d = by_kernel[dom]
+    avg_ms = d["ms"] / d["n"]
+    avg_bytes = d["bytes"] / d["n"]
+    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "launches_per_step": d["n"], "avg_launch_ms": round(avg_ms, 4),
+                "alg_bytes_per_launch": int(avg_bytes)}
+
+    part_rows = []
+    for p, pms in zip(plist, part_ms):
+        npass = sum(1 for l_ in launches if l_["part"] == p.name)
+        part_rows.append({"part": p.name, "ms": round(pms, 4), "gflops": round(p.flops / (pms * 1e-3) / 1e9, 1),
+                          "compulsory_GBs": round(p.comp_bytes / (pms * 1e-3) / 1e9, 1),
+                          "hbm_roofline_frac": round(p.comp_bytes / (pms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                          "launches": npass})
+
+    cpu = None if args.no_cpu else cpu_baseline(plist, args.cpu_seconds)
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (SplitMix64 seed 0)",
+        "config": {"workload": "FP64 drop-in at scale: c2c c128 4096x4096 axes [0,1] + r2c f64 2^22 x16 + "
+                               "r2c f64 1000003 x4", "parts": [p.name for p in plist]},
+        "roofline": roofline, "cpu_baseline": cpu, "parts": part_rows, "launches": launches,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

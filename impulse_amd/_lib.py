@@ -13,7 +13,8 @@ LIB_PATH = os.path.join(_HERE, "libimpulse_fft.so")
 # Every function declared in include/impulse_fft.h.
 EXPORTS = (
     "impulse_fft_plan_create", "impulse_fft_execute", "impulse_fft_execute_host",
-    "impulse_fft_plan_device_bytes", "impulse_fft_plan_num_launches", "impulse_fft_plan_destroy",
+    "impulse_fft_plan_device_bytes", "impulse_fft_plan_num_launches", "impulse_fft_plan_pass_info",
+    "impulse_fft_plan_profile", "impulse_fft_plan_profile_read", "impulse_fft_plan_destroy",
     "impulse_fft_c2c", "impulse_fft_r2c", "impulse_fft_c2r",
     "impulse_fft_cache_set_capacity", "impulse_fft_cache_size", "impulse_fft_cache_clear",
     "impulse_make_cfft_plan", "impulse_destroy_cfft_plan", "impulse_cfft_forward",
@@ -40,6 +41,10 @@ def _sig(L):
     L.impulse_fft_plan_device_bytes.argtypes = [_vp]
     L.impulse_fft_plan_device_bytes.restype = sz
     L.impulse_fft_plan_num_launches.argtypes = [_vp]
+    L.impulse_fft_plan_pass_info.argtypes = [_vp, i, ctypes.c_char_p, sz, _szp, _szp,
+                                             ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_uint), _szp]
+    L.impulse_fft_plan_profile.argtypes = [_vp, i]
+    L.impulse_fft_plan_profile_read.argtypes = [_vp, ctypes.POINTER(d), i, ctypes.POINTER(i)]
     L.impulse_fft_plan_destroy.argtypes = [_vp]
     for n in ("impulse_fft_c2c", "impulse_fft_r2c", "impulse_fft_c2r"):
         getattr(L, n).argtypes = [sz, _szp, _pdp, _pdp, sz, _szp, i, i, _vp, _vp, d, sz]

@@ -3,6 +3,7 @@
 // Every entry point converts internal errors to status codes and records a
 // message retrievable with impulse_fft_last_error(); nothing throws across
 // the ABI.
+#include <cstring>
 #include <list>
 #include <sstream>
 #include <unordered_map>
@@ -183,6 +184,41 @@ size_t impulse_fft_plan_device_bytes(impulse_fft_plan plan)
 int impulse_fft_plan_num_launches(impulse_fft_plan plan)
 {
     return (plan && plan->plan) ? (int)plan->plan->passes.size() : -1;
+}
+
+int impulse_fft_plan_pass_info(impulse_fft_plan plan, int pass, char *name, size_t name_len, size_t *bytes_in,
+                               size_t *bytes_out, unsigned *grid, unsigned *block, size_t *lds_bytes)
+{
+    if (!plan || !plan->plan) return fail(ifft::E_INVALID, "plan is NULL");
+    const auto &P = *plan->plan;
+    if (pass < 0 || pass >= (int)P.passes.size()) return fail(ifft::E_INVALID, "pass index out of range");
+    const auto &ps = P.passes[pass];
+    if (name && name_len) {
+        std::strncpy(name, ps.name, name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (bytes_in) *bytes_in = ps.bytes_in;
+    if (bytes_out) *bytes_out = ps.bytes_out;
+    if (grid) *grid = ps.kind != ifft::Pass::REAL_POST ? ps.grid : 0;
+    if (block) *block = ps.kind != ifft::Pass::REAL_POST ? ps.block : 256;
+    if (lds_bytes) *lds_bytes = ps.lds;
+    return ifft::OK;
+}
+
+int impulse_fft_plan_profile(impulse_fft_plan plan, int max_executes)
+{
+    if (!plan || !plan->plan) return fail(ifft::E_INVALID, "plan is NULL");
+    ifft::Error err{0, ""};
+    int rc = ifft::plan_profile(*plan->plan, max_executes, err);
+    return rc == ifft::OK ? rc : fail(err);
+}
+
+int impulse_fft_plan_profile_read(impulse_fft_plan plan, double *pass_ms, int npass, int *executes)
+{
+    if (!plan || !plan->plan || (!pass_ms && npass > 0)) return fail(ifft::E_INVALID, "invalid argument");
+    ifft::Error err{0, ""};
+    int rc = ifft::plan_profile_read(*plan->plan, pass_ms, npass, executes, err);
+    return rc == ifft::OK ? rc : fail(err);
 }
 
 int impulse_fft_plan_destroy(impulse_fft_plan plan)

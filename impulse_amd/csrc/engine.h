@@ -19,6 +19,10 @@ hipError_t launch_fft_lds(const FftParams &P, unsigned grid, unsigned block, siz
 hipError_t launch_real_post(const RealPostParams &P, hipStream_t s);
 hipError_t launch_pack_halfcomplex(const double2 *bins, double *hc, long long n, hipStream_t s);
 hipError_t launch_unpack_halfcomplex(const double *hc, double2 *bins, long long n, hipStream_t s);
+// fft_pow2.hip
+bool pow2_supported(int N);
+int pow2_lds_words(int N);
+hipError_t launch_fft_pow2(const FftParams &P, int map, unsigned grid, unsigned block, size_t lds, hipStream_t s);
 
 // status codes (mirror impulse_fft_status)
 enum Status { OK = 0, E_INVALID = 1, E_NOMEM = 2, E_HIP = 3, E_UNSUPPORTED = 4, E_NODEVICE = 5 };
@@ -43,13 +47,16 @@ struct DevBuf {
 enum BufRef : int { REF_IN = 0, REF_OUT = 1, REF_S0 = 2, REF_S1 = 3, REF_TMP = 4, REF_NONE = 5 };
 
 struct Pass {
-    enum Kind { FFT_LDS, REAL_POST } kind;
+    enum Kind { FFT_LDS, FFT_POW2, REAL_POST } kind;
+    int map = 0;                          // FFT_POW2 lane mapping (0 row, 1 col, 2 row load / col store)
     FftParams fp{};
     RealPostParams rp{};
     unsigned grid = 1, block = 64;
     size_t lds = 0;
     BufRef src = REF_NONE, dst = REF_NONE;
     bool uses_fct = false;
+    size_t bytes_in = 0, bytes_out = 0;   // bytes the launch must move at least once
+    const char *name = "";
 };
 
 struct Plan {
@@ -68,7 +75,11 @@ struct Plan {
     DevBuf stage_in, stage_out;
     ptrdiff_t in_lo = 0, in_hi = 0, out_lo = 0, out_hi = 0;   // byte spans relative to element 0
     bool out_dense = false;
+    // per-launch event timing (impulse_fft_plan_profile)
+    std::vector<hipEvent_t> prof_ev;    // 2 per launch per recorded execute
+    int prof_max = 0, prof_count = 0;
     size_t device_bytes() const;
+    ~Plan();
 };
 
 int plan_create(std::unique_ptr<Plan> &out, int kind, int dtype, size_t ndim, const size_t *shape,
@@ -76,6 +87,8 @@ int plan_create(std::unique_ptr<Plan> &out, int kind, int dtype, size_t ndim, co
                 int forward, Error &err);
 int plan_execute(Plan &plan, const void *in, void *out, double fct, hipStream_t stream, Error &err);
 int plan_execute_host(Plan &plan, const void *in, void *out, double fct, Error &err);
+int plan_profile(Plan &plan, int max_executes, Error &err);
+int plan_profile_read(Plan &plan, double *pass_ms, int npass, int *executes, Error &err);
 
 // host FP64 twiddles (twiddle.hip)
 void twiddle_table(std::vector<double2> &out, uint64_t n, uint64_t count, uint64_t step);

@@ -10,56 +10,9 @@
 //   outputs y[(b - k) R + k + q p] = DFT_R(inputs)[q]
 // which leaves the result in natural order after the last stage.  Each stage
 // reads its inputs into registers, synchronises, and writes back in place.
-#include "butterflies.h"
-#include "fft_params.h"
+#include "fft_common.h"
 
 namespace ifft {
-
-// ---------------------------------------------------------------------------
-// batch geometry helpers
-// ---------------------------------------------------------------------------
-
-__device__ __forceinline__ long long dims_off(const Side &s, long long p)
-{
-    long long off = 0;
-    for (int d = 0; d < s.nbd; ++d) {
-        long long c = s.cnt[d];
-        long long idx = p % c;
-        p /= c;
-        off += idx * s.str[d];
-    }
-    return off;
-}
-
-// base offsets of logical outer index u: row a (and row b for pair sides)
-__device__ __forceinline__ void side_base(const Side &s, long long u, long long &a, long long &b, int &hasb)
-{
-    if (s.kind == SIDE_LINEAR) {
-        a = u * s.lin;
-        b = 0;
-        hasb = 0;
-    } else if (s.kind == SIDE_DIMS_PAIR) {
-        long long p = 2 * u;
-        a = dims_off(s, p);
-        hasb = (p + 1) < s.nphys;
-        b = hasb ? dims_off(s, p + 1) : 0;
-    } else {
-        a = dims_off(s, u);
-        b = 0;
-        hasb = 0;
-    }
-}
-
-__device__ __forceinline__ double2 twolevel(const double2 *lo, const double2 *hi, long long S, long long m)
-{
-    return cmul(lo[m % S], hi[m / S]);
-}
-
-__device__ __forceinline__ double2 real_tw(const FftParams &P, long long k)
-{
-    // W_{2M}^k for the real-half modes
-    return P.rtw ? P.rtw[k] : twolevel(P.rtw_lo, P.rtw_hi, P.rtw_S, k);
-}
 
 // ---------------------------------------------------------------------------
 // Stockham stages in LDS
@@ -116,60 +69,6 @@ __device__ void run_stages(double2 *buf, const FftParams &P, int i, double s)
             default: stage<16, nb_max(16)>(buf, P.N, p, i, P.NT, P.tw, s); break;
         }
         p *= r;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// load / store of one point
-// ---------------------------------------------------------------------------
-
-struct TfInfo {
-    long long ia, ib, oa, ob;   // signal bases (input rows a/b, output rows a/b)
-    long long ip0, op0;         // position of element 0 (input / output side)
-    long long j;                // inner index
-    int flags;                  // 1 valid, 2 input row b exists, 4 output row b exists
-};
-
-__device__ __forceinline__ double2 load_point(const FftParams &P, const TfInfo &T, long long pos)
-{
-    const Side &S = P.in;
-    switch (P.load_mode) {
-        case LD_C2C: {
-            const double2 *x = (const double2 *)P.src;
-            return x[T.ia + pos * S.sax];
-        }
-        case LD_REAL_HALF: {
-            const double *x = (const double *)P.src;
-            return make_double2(x[T.ia + 2 * pos * S.sax], x[T.ia + (2 * pos + 1) * S.sax]);
-        }
-        case LD_R2C_PAIR: {
-            const double *x = (const double *)P.src;
-            double a = x[T.ia + pos * S.sax];
-            double b = (T.flags & 2) ? x[T.ib + pos * S.sax] : 0.0;
-            return make_double2(a, b);
-        }
-        case LD_C2R_HALF: {
-            const double2 *x = (const double2 *)P.src;
-            const long long M = P.M;
-            double2 xp = x[T.ia + pos * S.sax];
-            double2 xq = x[T.ia + (M - pos) * S.sax];
-            if (P.fwd_flag) { xp = cconj(xp); xq = cconj(xq); }
-            if (pos == 0) { xp.y = 0.0; xq.y = 0.0; }
-            double2 e = cadd(xp, cconj(xq));
-            double2 o = cmul(csub(xp, cconj(xq)), cconj(real_tw(P, pos)));
-            return make_double2(e.x - o.y, e.y + o.x);
-        }
-        default: {  // LD_C2R_PAIR
-            const double2 *x = (const double2 *)P.src;
-            const long long L = P.M;
-            const bool mirror = 2 * pos > L;
-            const long long k = mirror ? L - pos : pos;
-            double2 a = x[T.ia + k * S.sax];
-            double2 b = (T.flags & 2) ? x[T.ib + k * S.sax] : make_double2(0.0, 0.0);
-            if (P.fwd_flag != (int)mirror) { a = cconj(a); b = cconj(b); }
-            if (pos == 0) { a.y = 0.0; b.y = 0.0; }
-            return make_double2(a.x - b.y, a.y + b.x);
-        }
     }
 }
 

@@ -45,6 +45,11 @@ int DevBuf::alloc(size_t n)
     return OK;
 }
 
+Plan::~Plan()
+{
+    for (auto e : prof_ev) (void)hipEventDestroy(e);
+}
+
 size_t Plan::device_bytes() const
 {
     size_t b = s0.bytes + s1.bytes + tmp.bytes;
@@ -301,8 +306,55 @@ struct Builder {
         return (int)tb;
     }
 
-    bool add_fft(FftParams P, BufRef src, BufRef dst, bool uses_fct)
+    // the compile-time power-of-two kernel takes per-point load/store modes
+    bool try_pow2(FftParams P, BufRef src, BufRef dst, bool uses_fct, bool colL, bool colS)
     {
+        if (!pow2_supported(P.N) || P.conv || P.nload != P.N || P.nstore != P.N) return false;
+        if (P.store_mode == ST_R2C_HALF || P.store_mode == ST_R2C_PAIR) return false;
+        int map;
+        if (colL && colS) map = 1;
+        else if (!colL && colS) map = 2;
+        else if (!colL && !colS) map = 0;
+        else return false;
+        const int NT = P.N / 16;
+        const long long per = (long long)pow2_lds_words(P.N) * (long long)sizeof(double2);
+        const long long cap = max_lds_bytes() - (long long)sizeof(long long) * 8 * kMaxTB - 1024;
+        long long tb;
+        if (map == 0) {
+            tb = std::max<long long>(1, std::min<long long>(256 / NT, (76 * 1024) / per));
+        } else {
+            tb = std::max<long long>(1, std::min<long long>(kMaxThreads / NT, cap / per));
+            tb = std::min<long long>(tb, kMaxTB);
+        }
+        const long long want = std::max<long long>(1, (P.ntf + 511) / 512);
+        if (map == 0 && tb > want) tb = want;
+        if (tb * per > cap) return false;
+        P.TB = (int)tb;
+        P.NT = NT;
+        Pass ps;
+        ps.kind = Pass::FFT_POW2;
+        ps.map = map;
+        ps.fp = P;
+        ps.block = (unsigned)(tb * NT);
+        const long long grid = (P.ntf + tb - 1) / tb;
+        if (grid > INT_MAX) return false;
+        ps.grid = (unsigned)std::max<long long>(1, grid);
+        ps.lds = (size_t)(per * tb);
+        ps.src = src;
+        ps.dst = dst;
+        ps.uses_fct = uses_fct;
+        ps.name = "k_fft_pow2";
+        ps.bytes_in = side_bytes(P, true);
+        ps.bytes_out = side_bytes(P, false);
+        plan.passes.push_back(ps);
+        return true;
+    }
+
+    // colL / colS: whether consecutive transforms are closer in memory than
+    // consecutive points on the load / store side
+    bool add_fft(FftParams P, BufRef src, BufRef dst, bool uses_fct, bool colL, bool colS)
+    {
+        if (try_pow2(P, src, dst, uses_fct, colL, colS)) return true;
         const long long per = (long long)P.LD * (long long)sizeof(double2) * P.TB;
         const long long cap = max_lds_bytes() - (long long)sizeof(long long) * 8 * kMaxTB - 1024;
         if (per > cap) {
@@ -323,8 +375,34 @@ struct Builder {
         ps.src = src;
         ps.dst = dst;
         ps.uses_fct = uses_fct;
+        ps.name = "k_fft_lds";
+        ps.bytes_in = side_bytes(P, true);
+        ps.bytes_out = side_bytes(P, false);
         plan.passes.push_back(ps);
         return true;
+    }
+
+    // bytes one launch reads (input) or writes (output) if each byte moves once
+    static size_t side_bytes(const FftParams &P, bool input)
+    {
+        long long n;   // points per logical transform on that side
+        if (input) {
+            const long long pts = std::min<long long>(P.nload, P.ilim);
+            switch (P.load_mode) {
+                case LD_REAL_HALF: return (size_t)(P.ntf * pts * 16);
+                case LD_R2C_PAIR: return (size_t)(P.ntf * pts * 16);
+                case LD_C2R_HALF: n = (P.ni > 1) ? pts : (long long)P.M + 1; return (size_t)(P.ntf * n * 16);
+                case LD_C2R_PAIR: n = (P.ni > 1) ? pts : (long long)P.M / 2 + 1; return (size_t)(P.ntf * n * 32);
+                default: return (size_t)(P.ntf * pts * 16);
+            }
+        }
+        const long long pts = std::min<long long>(P.nstore, P.olim);
+        switch (P.store_mode) {
+            case ST_R2C_PAIR: return (size_t)(P.ntf * pts * 32);
+            case ST_C2R_HALF: return (size_t)(P.ntf * pts * 16);
+            case ST_C2R_PAIR: return (size_t)(P.ntf * pts * 16);
+            default: return (size_t)(P.ntf * pts * 16);
+        }
     }
 
     // ---- one LDS pass over the whole job --------------------------------
@@ -374,7 +452,7 @@ struct Builder {
         P.TB = choose_tb(P, P.ntf, colL || colS);
         P.col_load = P.TB > 1 && colL;
         P.col_store = P.TB > 1 && colS;
-        return add_fft(P, J.src, J.dst, J.fct);
+        return add_fft(P, J.src, J.dst, J.fct, colL, colS);
     }
 
     // ---- Bluestein tables ------------------------------------------------
@@ -465,7 +543,7 @@ struct Builder {
         P.TB = choose_tb(P, P.ntf, colL || colS);
         P.col_load = P.TB > 1 && colL;
         P.col_store = P.TB > 1 && colS;
-        return add_fft(P, J.src, J.dst, J.fct);
+        return add_fft(P, J.src, J.dst, J.fct, colL, colS);
     }
 
     // ---- four-step (and four-step Bluestein) -----------------------------
@@ -571,19 +649,19 @@ struct Builder {
             FftParams A = passA(P0);
             A.load_mode = lmode;
             A.sign = dir_sign;
-            if (!add_fft(A, J.src, REF_S0, false)) return false;
+            if (!add_fft(A, J.src, REF_S0, false, true, true)) return false;
             FftParams B = passB(P0);
             B.sign = dir_sign;
             if (J.kind == J_R2C) {
                 side_lin(B.out, n2, 1, L2);
                 B.store_mode = ST_C2C;
-                if (!add_fft(B, REF_S0, REF_S1, false)) return false;
+                if (!add_fft(B, REF_S0, REF_S1, false, false, true)) return false;
                 return add_post(J, pair, Ls, n2, rlo, rhi, rS);
             }
             B.out = outS;
             B.store_mode = (J.kind == J_C2C) ? ST_C2C : (pair ? ST_C2R_PAIR : ST_C2R_HALF);
             B.olim = Ls;
-            return add_fft(B, REF_S0, J.dst, J.fct);
+            return add_fft(B, REF_S0, J.dst, J.fct, false, true);
         }
 
         // Bluestein: A1 (chirp in) -> B1 (x spectrum) -> A2 (inverse) -> B2 (chirp out)
@@ -595,19 +673,19 @@ struct Builder {
         A1.chirp_in = 1;
         A1.conj_in = conj;
         A1.ilim = Ls;
-        if (!add_fft(A1, J.src, REF_S0, false)) return false;
+        if (!add_fft(A1, J.src, REF_S0, false, true, true)) return false;
         FftParams B1 = passB(P0);
         B1.sign = -1;
         B1.store_mode = ST_MUL_TABLE;
         B1.mtab = spec;
         side_lin(B1.out, n2, 1, L2);
-        if (!add_fft(B1, REF_S0, REF_S1, false)) return false;
+        if (!add_fft(B1, REF_S0, REF_S1, false, false, true)) return false;
         FftParams P1 = P0;
         side_lin(P1.in, n2, 1, L1);
         FftParams A2 = passA(P1);
         A2.load_mode = LD_C2C;
         A2.sign = 1;
-        if (!add_fft(A2, REF_S1, REF_S0, false)) return false;
+        if (!add_fft(A2, REF_S1, REF_S0, false, true, true)) return false;
         FftParams B2 = passB(P0);
         B2.sign = 1;
         B2.chirp = chirp;
@@ -617,12 +695,12 @@ struct Builder {
         if (J.kind == J_R2C) {
             side_lin(B2.out, n2, 1, L2);
             B2.store_mode = ST_C2C;
-            if (!add_fft(B2, REF_S0, REF_S1, false)) return false;
+            if (!add_fft(B2, REF_S0, REF_S1, false, false, true)) return false;
             return add_post(J, pair, Ls, n2, rlo, rhi, rS);
         }
         B2.out = outS;
         B2.store_mode = (J.kind == J_C2C) ? ST_C2C : (pair ? ST_C2R_PAIR : ST_C2R_HALF);
-        return add_fft(B2, REF_S0, J.dst, J.fct);
+        return add_fft(B2, REF_S0, J.dst, J.fct, false, true);
     }
 
     bool add_post(const Job &J, bool pair, long long Ls, long long zlen, const double2 *lo, const double2 *hi,
@@ -646,6 +724,10 @@ struct Builder {
         ps.src = REF_S1;
         ps.dst = J.dst;
         ps.uses_fct = J.fct;
+        ps.name = "k_real_post";
+        const long long per = pair ? Ls / 2 + 1 : Ls + 1;
+        ps.bytes_in = (size_t)(R.ntf * Ls * 16);
+        ps.bytes_out = (size_t)(R.ntf * per * (pair ? 32 : 16));
         plan.passes.push_back(ps);
         return true;
     }
@@ -863,14 +945,19 @@ int plan_execute(Plan &P, const void *in, void *out, double fct, hipStream_t str
             default: return nullptr;
         }
     };
-    for (auto &ps : P.passes) {
+    const bool prof = P.prof_count < P.prof_max;
+    hipEvent_t *ev = prof ? &P.prof_ev[(size_t)P.prof_count * P.passes.size() * 2] : nullptr;
+    for (size_t pi = 0; pi < P.passes.size(); ++pi) {
+        auto &ps = P.passes[pi];
         hipError_t e;
-        if (ps.kind == Pass::FFT_LDS) {
+        if (prof) (void)hipEventRecord(ev[2 * pi], stream);
+        if (ps.kind == Pass::FFT_LDS || ps.kind == Pass::FFT_POW2) {
             FftParams fp = ps.fp;
             fp.src = resolve(ps.src);
             fp.dst = resolve(ps.dst);
             fp.fct = ps.uses_fct ? fct : 1.0;
-            e = launch_fft_lds(fp, ps.grid, ps.block, ps.lds, stream);
+            e = ps.kind == Pass::FFT_POW2 ? launch_fft_pow2(fp, ps.map, ps.grid, ps.block, ps.lds, stream)
+                                          : launch_fft_lds(fp, ps.grid, ps.block, ps.lds, stream);
         } else {
             RealPostParams rp = ps.rp;
             rp.Z = (const double2 *)resolve(ps.src);
@@ -878,11 +965,52 @@ int plan_execute(Plan &P, const void *in, void *out, double fct, hipStream_t str
             rp.fct = ps.uses_fct ? fct : 1.0;
             e = launch_real_post(rp, stream);
         }
+        if (prof) (void)hipEventRecord(ev[2 * pi + 1], stream);
         if (e != hipSuccess) {
             err = {E_HIP, std::string("kernel launch failed: ") + hipGetErrorString(e)};
             return E_HIP;
         }
     }
+    if (prof) ++P.prof_count;
+    return OK;
+}
+
+int plan_profile(Plan &P, int max_executes, Error &err)
+{
+    for (auto e : P.prof_ev) (void)hipEventDestroy(e);
+    P.prof_ev.clear();
+    P.prof_max = P.prof_count = 0;
+    if (max_executes <= 0) return OK;
+    const size_t n = (size_t)max_executes * P.passes.size() * 2;
+    P.prof_ev.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        if (hipEventCreate(&P.prof_ev[i]) != hipSuccess) {
+            (void)hipGetLastError();
+            P.prof_ev.resize(i);
+            err = {E_HIP, "hipEventCreate failed"};
+            return E_HIP;
+        }
+    }
+    P.prof_max = max_executes;
+    return OK;
+}
+
+int plan_profile_read(Plan &P, double *pass_ms, int npass, int *executes, Error &err)
+{
+    const size_t np = P.passes.size();
+    for (int i = 0; i < npass; ++i) pass_ms[i] = 0.0;
+    for (int x = 0; x < P.prof_count; ++x)
+        for (size_t pi = 0; pi < np; ++pi) {
+            hipEvent_t a = P.prof_ev[((size_t)x * np + pi) * 2], b = P.prof_ev[((size_t)x * np + pi) * 2 + 1];
+            float ms = 0.f;
+            if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+                (void)hipGetLastError();
+                err = {E_HIP, "reading profile events failed"};
+                return E_HIP;
+            }
+            if ((int)pi < npass) pass_ms[pi] += ms;
+        }
+    if (executes) *executes = P.prof_count;
     return OK;
 }
 

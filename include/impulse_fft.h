@@ -87,6 +87,21 @@ size_t impulse_fft_plan_device_bytes(impulse_fft_plan plan);
 /* Number of kernel launches one execute issues (HBM passes plus helpers). */
 int impulse_fft_plan_num_launches(impulse_fft_plan plan);
 
+/* Describe kernel launch `pass` (0 <= pass < num_launches) of one execute:
+ * kernel name, grid and block size, LDS bytes, and the bytes the launch must
+ * read (`bytes_in`) and write (`bytes_out`) if every byte moves once. */
+int impulse_fft_plan_pass_info(impulse_fft_plan plan, int pass, char *name, size_t name_len,
+                               size_t *bytes_in, size_t *bytes_out, unsigned *grid, unsigned *block,
+                               size_t *lds_bytes);
+
+/* Per-launch timing: while enabled, the next `max_executes` executes record a
+ * HIP event pair around every launch on the execute's stream (0 disables and
+ * frees the events).  `profile_read` waits for the recorded events and writes
+ * the summed milliseconds of each launch into pass_ms[0..npass) and the
+ * number of executes recorded into *executes. */
+int impulse_fft_plan_profile(impulse_fft_plan plan, int max_executes);
+int impulse_fft_plan_profile_read(impulse_fft_plan plan, double *pass_ms, int npass, int *executes);
+
 int impulse_fft_plan_destroy(impulse_fft_plan plan);
 
 /* One-shot host-pointer calls with the exact argument list of pocketfft::c2c,

@@ -8,7 +8,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libimpulse_fft.so")
+# IMPULSE_FFT_LIB may point at another build of the same library (A/B timing)
+LIB_PATH = os.environ.get("IMPULSE_FFT_LIB") or os.path.join(_HERE, "libimpulse_fft.so")
 
 # Every function declared in include/impulse_fft.h.
 EXPORTS = (

@@ -21,7 +21,8 @@ hipError_t launch_pack_halfcomplex(const double2 *bins, double *hc, long long n,
 hipError_t launch_unpack_halfcomplex(const double *hc, double2 *bins, long long n, hipStream_t s);
 // fft_pow2.hip
 bool pow2_supported(int N);
-int pow2_lds_words(int N);
+int pow2_lds_bytes(int N, int map);
+int pow2_tb_host(int N, int map);
 hipError_t launch_fft_pow2(const FftParams &P, int map, unsigned grid, unsigned block, size_t lds, hipStream_t s);
 
 // status codes (mirror impulse_fft_status)
@@ -57,6 +58,10 @@ struct Pass {
     bool uses_fct = false;
     size_t bytes_in = 0, bytes_out = 0;   // bytes the launch must move at least once
     const char *name = "";
+    // even-length real input at unit stride read as complex pairs (used when the
+    // source pointer is 16-byte aligned at execute time)
+    bool has_alt = false;
+    FftParams fp_alt{};
 };
 
 struct Plan {

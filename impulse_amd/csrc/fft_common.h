@@ -11,46 +11,46 @@ namespace ifft {
 // batch geometry helpers
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ long long dims_off(const Side &s, long long p)
+__device__ __forceinline__ long long dims_off(const Side &s, unsigned p)
 {
     long long off = 0;
     for (int d = 0; d < s.nbd; ++d) {
-        long long c = s.cnt[d];
-        long long idx = p % c;
-        p /= c;
-        off += idx * s.str[d];
+        const unsigned c = (unsigned)s.cnt[d];
+        const unsigned q = p / c;
+        off += (long long)(p - q * c) * s.str[d];
+        p = q;
     }
     return off;
 }
 
 // base offsets of logical outer index u: row a (and row b for pair sides)
-__device__ __forceinline__ void side_base(const Side &s, long long u, long long &a, long long &b, int &hasb)
+__device__ __forceinline__ void side_base(const Side &s, unsigned u, long long &a, long long &b, int &hasb)
 {
     if (s.kind == SIDE_LINEAR) {
-        a = u * s.lin;
+        a = (long long)u * s.lin;
         b = 0;
         hasb = 0;
     } else if (s.kind == SIDE_DIMS_PAIR) {
-        long long p = 2 * u;
+        const unsigned p = 2 * (u + (unsigned)s.u0);
         a = dims_off(s, p);
-        hasb = (p + 1) < s.nphys;
+        hasb = (long long)(p + 1) < s.nphys;
         b = hasb ? dims_off(s, p + 1) : 0;
     } else {
-        a = dims_off(s, u);
+        a = dims_off(s, u + (unsigned)s.u0);
         b = 0;
         hasb = 0;
     }
 }
 
-__device__ __forceinline__ double2 twolevel(const double2 *lo, const double2 *hi, long long S, long long m)
+__device__ __forceinline__ double2 twolevel(const double2 *lo, const double2 *hi, int logS, long long m)
 {
-    return cmul(lo[m % S], hi[m / S]);
+    return cmul(lo[m & ((1LL << logS) - 1)], hi[m >> logS]);
 }
 
 __device__ __forceinline__ double2 real_tw(const FftParams &P, long long k)
 {
     // W_{2M}^k for the real-half modes
-    return P.rtw ? P.rtw[k] : twolevel(P.rtw_lo, P.rtw_hi, P.rtw_S, k);
+    return P.rtw ? P.rtw[k] : twolevel(P.rtw_lo, P.rtw_hi, P.rtw_logS, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -107,13 +107,48 @@ __device__ __forceinline__ double2 load_point(const FftParams &P, const TfInfo &
     }
 }
 
+// raw buffer access: a scalar 48-bit base and 32-bit byte offsets (loads past
+// the window return zero, stores past it are dropped)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, long long span)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0,
+                                             (int)(span < 0x7fffffffLL ? span : 0x7fffffffLL), 0x00020000);
+}
+
+__device__ __forceinline__ double2 bload2(__amdgpu_buffer_rsrc_t r, int off)
+{
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+
+__device__ __forceinline__ void bstore2(__amdgpu_buffer_rsrc_t r, int off, double2 v)
+{
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned int, v),
+                                           r, off, 0, 0);
+}
+
+__device__ __forceinline__ double2 twolevel_b(__amdgpu_buffer_rsrc_t lo, __amdgpu_buffer_rsrc_t hi, int logS,
+                                              long long m)
+{
+    return cmul(bload2(lo, (int)(m & ((1LL << logS) - 1)) * 16), bload2(hi, (int)(m >> logS) * 16));
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Renumber them so that each XCD works on a contiguous range of
+// transforms: neighbouring column blocks then share L2 lines.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned grid)
+{
+    const unsigned g8 = grid & ~7u;
+    return b < g8 ? (b & 7u) * (g8 >> 3) + (b >> 3) : b;
+}
+
 __device__ __forceinline__ TfInfo make_tfinfo(const FftParams &P, long long g)
 {
     TfInfo T;
     T.flags = 0;
     T.ia = T.ib = T.oa = T.ob = T.ip0 = T.op0 = T.j = 0;
     if (g < P.ntf) {
-        const long long j = g % P.ni, u = g / P.ni;
+        const unsigned gg = (unsigned)g, ni = (unsigned)P.ni;
+        const unsigned u = gg / ni, j = gg - u * ni;
         int hb_in, hb_out;
         side_base(P.in, u, T.ia, T.ib, hb_in);
         side_base(P.out, u, T.oa, T.ob, hb_out);
@@ -121,8 +156,8 @@ __device__ __forceinline__ TfInfo make_tfinfo(const FftParams &P, long long g)
         T.ib += j * P.in.isig;
         T.oa += j * P.out.isig;
         T.ob += j * P.out.isig;
-        T.ip0 = j * P.in.ipos;
-        T.op0 = j * P.out.ipos;
+        T.ip0 = (long long)j * P.in.ipos;
+        T.op0 = (long long)j * P.out.ipos;
         T.j = j;
         T.flags = 1 | (hb_in ? 2 : 0) | (hb_out ? 4 : 0);
     }
@@ -152,7 +187,7 @@ __device__ __forceinline__ void store_value(const FftParams &P, const TfInfo &T,
     if (P.conj_out) z = cconj(z);
     switch (P.store_mode) {
         case ST_TWIDDLE: {
-            double2 w = tw_sign(twolevel(P.tw_lo, P.tw_hi, P.tw_S, (long long)P.tw_mul * T.j * n), (double)P.sign);
+            double2 w = tw_sign(twolevel(P.tw_lo, P.tw_hi, P.tw_logS, (long long)P.tw_mul * T.j * n), (double)P.sign);
             ((double2 *)P.dst)[T.oa + pos * S.sax] = cmul(z, w);
             return;
         }

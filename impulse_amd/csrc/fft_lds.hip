@@ -89,7 +89,7 @@ __device__ __forceinline__ void store_point(const FftParams &P, const TfInfo &T,
             return;
         }
         case ST_TWIDDLE: {
-            double2 w = tw_sign(twolevel(P.tw_lo, P.tw_hi, P.tw_S, (long long)P.tw_mul * T.j * (long long)n), (double)P.sign);
+            double2 w = tw_sign(twolevel(P.tw_lo, P.tw_hi, P.tw_logS, (long long)P.tw_mul * T.j * (long long)n), (double)P.sign);
             put_c(P, T.oa + pos * S.sax, cmul(buf[n], w));
             return;
         }
@@ -151,27 +151,10 @@ __global__ void __launch_bounds__(kMaxThreads) k_fft_lds(const FftParams P)
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
     const int TB = P.TB, N = P.N, LD = P.LD;
-    const long long g0 = (long long)blockIdx.x * TB;
+    const long long g0 = (long long)xcd_block(blockIdx.x, gridDim.x) * TB;
 
     if (tid < TB) {
-        TfInfo T;
-        const long long g = g0 + tid;
-        T.flags = 0;
-        T.ia = T.ib = T.oa = T.ob = T.ip0 = T.op0 = T.j = 0;
-        if (g < P.ntf) {
-            const long long j = g % P.ni, u = g / P.ni;
-            int hb_in, hb_out;
-            side_base(P.in, u, T.ia, T.ib, hb_in);
-            side_base(P.out, u, T.oa, T.ob, hb_out);
-            T.ia += j * P.in.isig;
-            T.ib += j * P.in.isig;
-            T.oa += j * P.out.isig;
-            T.ob += j * P.out.isig;
-            T.ip0 = j * P.in.ipos;
-            T.op0 = j * P.out.ipos;
-            T.j = j;
-            T.flags = 1 | (hb_in ? 2 : 0) | (hb_out ? 4 : 0);
-        }
+        const TfInfo T = make_tfinfo(P, g0 + tid);
         info[tid] = T;
     }
     __syncthreads();
@@ -232,33 +215,41 @@ __global__ void __launch_bounds__(kMaxThreads) k_fft_lds(const FftParams P)
 // four-step real post-process: Z -> half spectra
 // ---------------------------------------------------------------------------
 
+// Each thread takes the bin pair (k, M - k) (half) or (k, L - k) (pair), so
+// every Z point is read once and both outputs of the pair are written.
 __global__ void __launch_bounds__(256) k_real_post(const RealPostParams P)
 {
-    const long long per = P.half ? (long long)P.M + 1 : (long long)P.M / 2 + 1;
+    const long long L = P.M;
+    const long long per = L / 2 + 1;   // pairs k = 0 .. L/2 cover all outputs
     const long long total = per * P.ntf;
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (long long)gridDim.x * blockDim.x) {
         const long long u = e / per, k = e % per;
+        const long long km = (L - k) % L;   // mirror index of Z
         const double2 *Z = P.Z + u * P.zlen;
         long long oa, ob;
         int hb;
-        side_base(P.out, u, oa, ob, hb);
+        side_base(P.out, (unsigned)u, oa, ob, hb);
         double2 *y = (double2 *)P.dst;
+        const double2 zk = Z[k], zm = Z[km];
         if (P.half) {
-            const long long M = P.M;
-            double2 zk = Z[k % M], zm = cconj(Z[(M - k) % M]);
-            double2 ev = cscale(cadd(zk, zm), 0.5);
-            double2 d = csub(zk, zm);
-            double2 o = make_double2(0.5 * d.y, -0.5 * d.x);
-            double2 w = P.two_level ? twolevel(P.tw_lo, P.tw_hi, P.tw_S, k) : P.rtw[k];
-            double2 X = cscale(cadd(ev, cmul(o, w)), P.fct);
-            if (!P.fwd_flag) X = cconj(X);
-            y[oa + k * P.out.sax] = X;
+            // X[k] = (zk + zm*)/2 - i/2 W^k (zk - zm*),  W = W_{2M}; X[M-k] from the
+            // same pair with the roles swapped and W^{M-k} = -conj(W^k)
+            const double2 w = P.two_level ? twolevel(P.tw_lo, P.tw_hi, P.tw_logS, k) : P.rtw[k];
+            const double2 e1 = cscale(cadd(zk, cconj(zm)), 0.5), d1 = csub(zk, cconj(zm));
+            const double2 o1 = make_double2(0.5 * d1.y, -0.5 * d1.x);
+            double2 Xk = cscale(cadd(e1, cmul(o1, w)), P.fct);
+            const double2 e2 = cscale(cadd(zm, cconj(zk)), 0.5), d2 = csub(zm, cconj(zk));
+            const double2 o2 = make_double2(0.5 * d2.y, -0.5 * d2.x);
+            double2 Xm = cscale(cadd(e2, cmul(o2, make_double2(-w.x, w.y))), P.fct);
+            if (!P.fwd_flag) { Xk = cconj(Xk); Xm = cconj(Xm); }
+            y[oa + k * P.out.sax] = Xk;
+            if (L - k != k) y[oa + (L - k) * P.out.sax] = Xm;
         } else {
-            const long long L = P.M;
-            double2 zk = Z[k], zm = cconj(Z[(L - k) % L]);
-            double2 xa = cscale(cadd(zk, zm), 0.5 * P.fct);
-            double2 d = csub(zk, zm);
+            // odd pair: Xa = (zk + zm*)/2, Xb = -i/2 (zk - zm*) for k = 0 .. L/2
+            const double2 za = cconj(zm);
+            double2 xa = cscale(cadd(zk, za), 0.5 * P.fct);
+            const double2 d = csub(zk, za);
             double2 xb = make_double2(0.5 * P.fct * d.y, -0.5 * P.fct * d.x);
             if (!P.fwd_flag) { xa = cconj(xa); xb = cconj(xb); }
             y[oa + k * P.out.sax] = xa;
@@ -319,7 +310,7 @@ hipError_t launch_fft_lds(const FftParams &P, unsigned grid, unsigned block, siz
 
 hipError_t launch_real_post(const RealPostParams &P, hipStream_t s)
 {
-    const long long per = P.half ? (long long)P.M + 1 : (long long)P.M / 2 + 1;
+    const long long per = (long long)P.M / 2 + 1;
     long long total = per * P.ntf;
     long long blocks = (total + 255) / 256;
     if (blocks > 65536) blocks = 65536;

@@ -1,6 +1,9 @@
 // Kernel-side parameter blocks of the impulse FFT engine (host and device).
 //
-// Geometry model.  A launch runs a batch of "logical transforms" of length N
+// Geometry model.  (Transform counts, batch counts and positions fit 31
+// bits; the planner rejects larger launches, so device code indexes batches
+// with 32-bit integer arithmetic and only offsets are 64-bit.)
+//  A launch runs a batch of "logical transforms" of length N
 // (complex points) through LDS.  Logical transform g is split into an inner
 // index j = g % ni and an outer index u = g / ni.  Element n of transform g
 // sits at signal position   pos = j*ipos + n*pst   of a signal whose element 0
@@ -56,6 +59,7 @@ struct Side {
     long long ipos;                 // added to the position per inner index j
     long long pst;                  // position step per element n
     long long sax;                  // scalar stride per position
+    long long u0;                   // SIDE_DIMS*: first logical outer index of this launch (batch chunks)
 };
 
 struct FftParams {
@@ -86,17 +90,21 @@ struct FftParams {
     // arrays
     const void *src;
     void *dst;
+    long long src_lo, src_span;   // byte window of the source array around src (lo <= 0)
+    long long dst_lo, dst_span;   // byte window of the destination array around dst
     const double2 *tw;         // W_N^j (forward sign), j < N
     const double2 *chirp;      // w[n] = exp(-i pi n^2 / nsig), n < nsig
     const double2 *spec;       // Bluestein kernel spectrum / N, N entries
     const double2 *rtw;        // W_{2M}^k, k <= M (real-half modes)
-    const double2 *tw_lo;      // two-level W_L^m = lo[m % S] * hi[m / S]
+    const double2 *tw_lo;      // two-level W_L^m = lo[m & (S-1)] * hi[m >> log2 S], S a power of two
     const double2 *tw_hi;
     long long tw_S, tw_L;
+    int tw_logS;
     int tw_mul;                // ST_TWIDDLE uses W_L^{tw_mul j n}
     const double2 *rtw_lo;     // two-level W_{2M} used by the real-half modes when rtw is null
     const double2 *rtw_hi;
     long long rtw_S;
+    int rtw_logS;
     const double2 *mtab;       // ST_MUL_TABLE
 };
 
@@ -115,6 +123,7 @@ struct RealPostParams {
     const double2 *rtw;        // W_{2M}^k, k <= M (half)
     const double2 *tw_lo, *tw_hi;
     long long tw_S;
+    int tw_logS;
     int two_level;             // rtw replaced by the two-level table
 };
 

@@ -3,23 +3,31 @@
 // Radix-16 Stockham stages (one radix-2/4/8 stage last when log2 N is not a
 // multiple of 4).  Each of the N/16 threads of a transform holds 16 points in
 // registers for every stage: the first stage reads them straight from global
-// memory (x[i + q N/16], coalesced), the middle stages exchange through a
-// padded LDS row (index + index/16, which keeps the radix-16 scatter free of
-// bank conflicts), and the last stage writes straight to global memory
-// (y[b + q N/R], coalesced).  Per-point load modes (complex, real-half,
-// real-pair, c2r pre-processing, Bluestein chirp) and store modes (plain,
-// four-step twiddle, Bluestein spectrum product or chirp, c2r outputs) are
-// fused into those global accesses.
+// memory (x[i + q N/16], coalesced), the middle stages exchange through LDS,
+// and the last stage writes straight to global memory (y[b + q N/R]).
+//
+// LDS holds ONE component per exchange: real parts are written, read back
+// into the next stage's positions, then the imaginary parts follow.  A
+// 4096-point FP64 transform thus needs 34 KiB of LDS instead of 68 KiB, which
+// doubles the workgroups a CU can keep in flight; rows are padded by one word
+// every 16 (index + index/16) so the radix-16 scatter is free of bank
+// conflicts.  Twiddles W^{qm} are built from four table loads (W^m, W^2m,
+// W^4m, W^8m) and at most three products, which keeps them within a few ulp.
+//
+// Per-point load modes (complex, real-half, real-pair, c2r pre-processing,
+// Bluestein chirp) and store modes (plain, four-step twiddle, Bluestein
+// spectrum product or chirp, c2r outputs) are fused into the global accesses;
+// FAST instantiations cover plain complex loads and plain / twiddled complex
+// stores with direct pointer arithmetic.
 //
 // MAP selects the lane mapping of the global accesses:
-//   MAP_ROW    : lanes walk along a transform (contiguous axis), loads and stores
+//   MAP_ROW    : lanes walk along a transform (contiguous axis)
 //   MAP_COL    : lanes walk across the TB transforms of a workgroup first
-//                (transforms adjacent in memory, strided axis); the LDS rows
-//                are interleaved so the exchange stays conflict-free
+//                (transforms adjacent in memory, strided axis); LDS rows are
+//                interleaved so the exchange stays conflict-free
 //   MAP_ROWCOL : loads along the transform, stores across transforms (the
-//                second pass of a four-step transform, which writes its output
-//                with the stride of the first factor): the last stage goes
-//                through LDS once more to transpose the store.
+//                second pass of a four-step transform, whose output stride is
+//                the first factor): the last stage is transposed through LDS.
 #include "fft_common.h"
 
 namespace ifft {
@@ -36,13 +44,59 @@ template <> struct Pow2Radices<8192> { static constexpr int n = 4; static conste
 
 __device__ __forceinline__ int padx(int idx) { return idx + (idx >> 4); }
 
-// LDS words per transform: padded row plus one word so that rows of
-// neighbouring transforms start on different banks
-template <int N> constexpr int pow2_ld() { return N + N / 16 + 1; }
-
 enum Pow2Map { MAP_ROW = 0, MAP_COL = 1, MAP_ROWCOL = 2 };
 
-// radix-R butterflies held by one thread: R-point groups v[c*R .. c*R+R)
+// transforms per workgroup, fixed per length and mapping: 256 threads along
+// rows, 512 threads across transforms (at most 64 transforms)
+__host__ __device__ constexpr int pow2_tb(int N, int map)
+{
+    return map == MAP_ROW ? (N >= 4096 ? 1 : 4096 / N) : (N >= 8192 ? 1 : (8192 / N > 64 ? 64 : 8192 / N));
+}
+
+// doubles of LDS per transform: one padded component row plus one word so
+// that rows of neighbouring transforms start on different banks; the
+// row-load / column-store variant reuses the row as N/2 complex words with
+// an odd stride for its transposed store, hence two words
+template <int N, int MAP> constexpr int pow2_ld() { return N + N / 16 + (MAP == MAP_ROWCOL ? 2 : 1); }
+
+// v[q] *= W^{q m} (forward table, conjugated for the backward sign), q < R.
+// W^m, W^2m, W^4m, W^8m come from the table, every other power is one to three
+// products of those, and each twiddle is used as soon as it is formed so that
+// at most five are live at a time.
+enum Pow2Io { IO_GENERIC = 0, IO_PLAIN = 1, IO_TWIDDLE = 2 };
+
+template <int R>
+__device__ __forceinline__ void apply_twiddles(double2 *v, __amdgpu_buffer_rsrc_t tw, int m, double s)
+{
+    const double2 w1 = bload2(tw, 16 * m);
+    v[1] = cmul(v[1], tw_sign(w1, s));
+    if constexpr (R >= 4) {
+        const double2 w2 = bload2(tw, 32 * m);
+        v[2] = cmul(v[2], tw_sign(w2, s));
+        const double2 w3 = cmul(w1, w2);
+        v[3] = cmul(v[3], tw_sign(w3, s));
+        if constexpr (R >= 8) {
+            const double2 w4 = bload2(tw, 64 * m);
+            v[4] = cmul(v[4], tw_sign(w4, s));
+            v[5] = cmul(v[5], tw_sign(cmul(w4, w1), s));
+            v[6] = cmul(v[6], tw_sign(cmul(w4, w2), s));
+            v[7] = cmul(v[7], tw_sign(cmul(w4, w3), s));
+            if constexpr (R >= 16) {
+                const double2 w8 = bload2(tw, 128 * m);
+                v[8] = cmul(v[8], tw_sign(w8, s));
+                v[9] = cmul(v[9], tw_sign(cmul(w8, w1), s));
+                v[10] = cmul(v[10], tw_sign(cmul(w8, w2), s));
+                v[11] = cmul(v[11], tw_sign(cmul(w8, w3), s));
+                const double2 w12 = cmul(w8, w4);
+                v[12] = cmul(v[12], tw_sign(w12, s));
+                v[13] = cmul(v[13], tw_sign(cmul(w12, w1), s));
+                v[14] = cmul(v[14], tw_sign(cmul(w12, w2), s));
+                v[15] = cmul(v[15], tw_sign(cmul(w12, w3), s));
+            }
+        }
+    }
+}
+
 template <int R>
 __device__ __forceinline__ void dft_groups(double2 *v, double s)
 {
@@ -50,80 +104,159 @@ __device__ __forceinline__ void dft_groups(double2 *v, double s)
     for (int c = 0; c < 16 / R; ++c) dft<R>(v + c * R, s);
 }
 
-template <int N, int MAP>
-__global__ void __launch_bounds__(512) k_fft_pow2(const FftParams P)
+// waves per SIMD the register allocation must allow: along rows three
+// (no spills, three 256-thread workgroups per CU); across transforms four, so
+// that two 512-thread workgroups share a CU
+#ifndef IFFT_ROWCOL_WPE
+#define IFFT_ROWCOL_WPE 3
+#endif
+#ifndef IFFT_COL_WPE
+#define IFFT_COL_WPE 3
+#endif
+template <int MAP> constexpr int pow2_wpe()
 {
-    extern __shared__ double2 lds[];
+    return MAP == MAP_ROW ? 3 : (MAP == MAP_COL ? IFFT_COL_WPE : IFFT_ROWCOL_WPE);
+}
+
+template <int N, int MAP, int IO>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_wpe<MAP>()))) k_fft_pow2(const FftParams P)
+{
+    extern __shared__ double lds[];
     __shared__ TfInfo info[MAP == MAP_ROWCOL ? kMaxTB : 1];
     constexpr bool COL = MAP == MAP_COL;
     constexpr int NT = N / 16;
-    constexpr int LDP = pow2_ld<N>();
+    constexpr int LDP = pow2_ld<N, MAP>();
     using RS = Pow2Radices<N>;
-    const int TB = P.TB;
+    constexpr int NST = RS::n;
+    constexpr int TB = pow2_tb(N, MAP);
     const int tid = threadIdx.x;
     const int t = COL ? tid % TB : tid / NT;
     const int i = COL ? tid / TB : tid % NT;
     const double s = (double)P.sign;
-    const TfInfo T = make_tfinfo(P, (long long)blockIdx.x * TB + t);
+    const TfInfo T = make_tfinfo(P, (long long)xcd_block(blockIdx.x, gridDim.x) * TB + t);
     const bool valid = T.flags & 1;
-    const double2 *__restrict__ tw = P.tw;
+    constexpr bool FAST = IO != IO_GENERIC;
+    // twiddle table through a buffer resource: 32-bit offsets, scalar base
+    const __amdgpu_buffer_rsrc_t rtw = make_rsrc(P.tw, (long long)N * 16);
 
-    auto at = [&](int idx) -> double2 & {
-        return COL ? lds[padx(idx) * TB + t] : lds[t * LDP + padx(idx)];
-    };
+    auto at = [&](int idx) -> double & { return COL ? lds[padx(idx) * TB + t] : lds[t * LDP + padx(idx)]; };
 
     double2 v[16];
     // ---- stage 0 (radix 16, p = 1): inputs x[i + q N/16] from global memory
+    if constexpr (FAST) {
+        // byte offsets inside the source window fit 31 bits (checked at plan time)
+        const __amdgpu_buffer_rsrc_t rin = make_rsrc((const char *)P.src + P.src_lo, P.src_span);
+        const int step = (int)(P.in.pst * P.in.sax * 16);
+        const int base = (int)((T.ia + T.ip0 * P.in.sax) * 16 - P.src_lo) + i * step;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int n = i + q * (N / 16);
-        v[q] = valid ? load_full(P, T, T.ip0 + (long long)n * P.in.pst) : make_double2(0.0, 0.0);
+        for (int q = 0; q < 16; ++q) {
+            const double2 x = bload2(rin, base + q * (N / 16) * step);
+            v[q] = valid ? x : make_double2(0.0, 0.0);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int n = i + q * (N / 16);
+            v[q] = valid ? load_full(P, T, T.ip0 + (long long)n * P.in.pst) : make_double2(0.0, 0.0);
+        }
     }
     dft<16>(v, s);
-    // outputs y[16 i + q]
-#pragma unroll
-    for (int q = 0; q < 16; ++q) at(16 * i + q) = v[q];
-    __syncthreads();
 
-    int p = 16;
-    // ---- middle stages (all radix 16)
+    // ---- exchanges and the remaining stages
+    int p = 16;   // product of the radices applied so far
 #pragma unroll
-    for (int st = 1; st + 1 < RS::n; ++st) {
-        constexpr int R = 16;
+    for (int st = 1; st < NST; ++st) {
+        const int R = RS::r[st];
         const int nb = N / R;
-        const int k = i % p;
+        // positions the previous stage wrote (radix 16 for all but the last stage)
+        const int kp = (st == 1) ? 0 : i % (p / 16);
+        const int jp = (st == 1) ? 16 * i : (i - kp) * 16 + kp;
+        const int pp = p / 16;
+        // real parts, then imaginary parts
 #pragma unroll
-        for (int q = 0; q < 16; ++q) v[q] = at(i + q * nb);
-        __syncthreads();
-        const int twstep = N / (p * R);
+        for (int half = 0; half < 2; ++half) {
 #pragma unroll
-        for (int q = 1; q < 16; ++q) v[q] = cmul(v[q], tw_sign(tw[q * k * twstep], s));
-        dft<16>(v, s);
-        const int j = (i - k) * R + k;
+            for (int q = 0; q < 16; ++q) at(jp + q * pp) = half ? v[q].y : v[q].x;
+            __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 16; ++q) at(j + q * p) = v[q];
-        __syncthreads();
+            for (int c = 0; c < 16 / R; ++c)
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const double x = at(i + c * NT + q * nb);
+                    if (half) v[c * R + q].y = x;
+                    else v[c * R + q].x = x;
+                }
+            __syncthreads();
+        }
+        // twiddles and butterflies
+#pragma unroll
+        for (int c = 0; c < 16 / R; ++c) {
+            const int b = i + c * NT;
+            const int k = b % p;
+            switch (R) {
+                case 16: apply_twiddles<16>(v + c * R, rtw, k * (N / (p * 16)), s); break;
+                case 8: apply_twiddles<8>(v + c * R, rtw, k * (N / (p * 8)), s); break;
+                case 4: apply_twiddles<4>(v + c * R, rtw, k * (N / (p * 4)), s); break;
+                default: apply_twiddles<2>(v + c * R, rtw, k * (N / (p * 2)), s); break;
+            }
+        }
+        switch (R) {
+            case 16: dft_groups<16>(v, s); break;
+            case 8: dft_groups<8>(v, s); break;
+            case 4: dft_groups<4>(v, s); break;
+            default: dft_groups<2>(v, s); break;
+        }
         p *= R;
     }
-
-    // ---- last stage (radix RL, p = N / RL): 16 / RL butterflies b = i + c NT
-    constexpr int RL = RS::r[RS::n - 1];
+    // the last stage had p_last = N / R_L, so butterfly b = i + c NT (k = b)
+    // holds outputs y[b + q N / R_L]
+    constexpr int RL = RS::r[NST - 1];
     constexpr int NB = 16 / RL;
-#pragma unroll
-    for (int c = 0; c < NB; ++c) {
-        const int b = i + c * NT;
-#pragma unroll
-        for (int q = 0; q < RL; ++q) v[c * RL + q] = at(b + q * (N / RL));
-    }
-#pragma unroll
-    for (int c = 0; c < NB; ++c) {
-        const int b = i + c * NT;   // k = b since p = N / RL > b
-#pragma unroll
-        for (int q = 1; q < RL; ++q) v[c * RL + q] = cmul(v[c * RL + q], tw_sign(tw[q * b], s));
-    }
-    dft_groups<RL>(v, s);
+
     if constexpr (MAP != MAP_ROWCOL) {
-        if (valid) {
+        if (!valid) return;
+        if constexpr (FAST) {
+            const __amdgpu_buffer_rsrc_t rout = make_rsrc((char *)P.dst + P.dst_lo, P.dst_span);
+            const int ostep = (int)(P.out.pst * P.out.sax * 16);
+            const int obase = (int)((T.oa + T.op0 * P.out.sax) * 16 - P.dst_lo);
+            constexpr bool twid = IO == IO_TWIDDLE;
+            if constexpr (twid) {
+                // W_L^{j n}, n = b + q N/RL: W^{j b} times powers of W^{j N/RL}
+                // (binary powers, at most four products per twiddle)
+                const long long tS = 1LL << P.tw_logS;
+                const __amdgpu_buffer_rsrc_t rlo = make_rsrc(P.tw_lo, tS * 16);
+                const __amdgpu_buffer_rsrc_t rhi = make_rsrc(P.tw_hi, (P.tw_L / tS + 2) * 16);
+                const double2 step1 = twolevel_b(rlo, rhi, P.tw_logS, T.j * (long long)(N / RL));
+                double2 pw[4];
+                pw[0] = step1;
+#pragma unroll
+                for (int e = 1; e < 4; ++e) pw[e] = cmul(pw[e - 1], pw[e - 1]);
+#pragma unroll
+                for (int c = 0; c < NB; ++c) {
+                    const int b = i + c * NT;
+                    const double2 base = twolevel_b(rlo, rhi, P.tw_logS, T.j * (long long)b);
+#pragma unroll
+                    for (int q = 0; q < RL; ++q) {
+                        const int n = b + q * (N / RL);
+                        double2 w = base;
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            if (q & (1 << e)) w = cmul(w, pw[e]);
+                        bstore2(rout, obase + n * ostep, cmul(v[c * RL + q], tw_sign(w, s)));
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < NB; ++c) {
+                    const int b = i + c * NT;
+#pragma unroll
+                    for (int q = 0; q < RL; ++q) {
+                        const int n = b + q * (N / RL);
+                        bstore2(rout, obase + n * ostep, cscale(v[c * RL + q], P.fct));
+                    }
+                }
+            }
+        } else {
 #pragma unroll
             for (int c = 0; c < NB; ++c) {
                 const int b = i + c * NT;
@@ -132,58 +265,103 @@ __global__ void __launch_bounds__(512) k_fft_pow2(const FftParams P)
             }
         }
     } else {
-        // transpose through LDS: natural order rows, then lanes across transforms
+        // transpose through LDS in two halves of the point range (one component
+        // row holds half the points as complex values)
         if (i == 0) info[t] = T;
-        __syncthreads();
+        double2 *ldc = (double2 *)lds;
+        constexpr int H = N / 2;
+        constexpr int SH = LDP / 2;   // complex words per transform: H + H/16 + 1, odd
+        static_assert(SH >= H + H / 16, "half row must fit the component row");
+        constexpr int nthr = TB * NT;
 #pragma unroll
-        for (int c = 0; c < NB; ++c) {
-            const int b = i + c * NT;
+        for (int half = 0; half < 2; ++half) {
+            __syncthreads();
 #pragma unroll
-            for (int q = 0; q < RL; ++q) at(b + q * (N / RL)) = v[c * RL + q];
-        }
-        __syncthreads();
-        const int nthr = blockDim.x;
-        for (int e = tid; e < TB * N; e += nthr) {
-            const int tt = e % TB, n = e / TB;
-            const TfInfo &U = info[tt];
-            if (U.flags & 1) store_value(P, U, n, lds[tt * LDP + padx(n)]);
+            for (int c = 0; c < NB; ++c) {
+                const int b = i + c * NT;
+#pragma unroll
+                for (int q = 0; q < RL; ++q) {
+                    const int n = b + q * (N / RL) - half * H;
+                    if (n >= 0 && n < H) ldc[t * SH + padx(n)] = v[c * RL + q];
+                }
+            }
+            __syncthreads();
+            for (int e = tid; e < TB * H; e += nthr) {
+                const int tt = e % TB, n = e / TB;
+                const TfInfo &U = info[tt];
+                if (!(U.flags & 1)) continue;
+                if constexpr (IO == IO_PLAIN) {
+                    const __amdgpu_buffer_rsrc_t rout = make_rsrc((char *)P.dst + P.dst_lo, P.dst_span);
+                    const long long pos = U.op0 + (long long)(n + half * H) * P.out.pst;
+                    bstore2(rout, (int)((U.oa + pos * P.out.sax) * 16 - P.dst_lo), cscale(ldc[tt * SH + padx(n)], P.fct));
+                } else {
+                    store_value(P, U, n + half * H, ldc[tt * SH + padx(n)]);
+                }
+            }
         }
     }
 }
 
-template <int N>
-static hipError_t launch_n(const FftParams &P, int map, unsigned grid, unsigned block, size_t lds, hipStream_t s)
+template <int N, int IO>
+static hipError_t launch_io(const FftParams &P, int map, unsigned grid, unsigned block, size_t lds, hipStream_t s)
 {
     static bool attr_done = false;
     if (!attr_done) {
-        for (const void *f : {(const void *)k_fft_pow2<N, MAP_ROW>, (const void *)k_fft_pow2<N, MAP_COL>,
-                              (const void *)k_fft_pow2<N, MAP_ROWCOL>})
+        for (const void *f : {(const void *)k_fft_pow2<N, MAP_ROW, IO>, (const void *)k_fft_pow2<N, MAP_COL, IO>,
+                              (const void *)k_fft_pow2<N, MAP_ROWCOL, IO>})
             if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 8192) != hipSuccess)
                 (void)hipGetLastError();
         attr_done = true;
     }
     switch (map) {
-        case MAP_COL: hipLaunchKernelGGL((k_fft_pow2<N, MAP_COL>), dim3(grid), dim3(block), lds, s, P); break;
-        case MAP_ROWCOL: hipLaunchKernelGGL((k_fft_pow2<N, MAP_ROWCOL>), dim3(grid), dim3(block), lds, s, P); break;
-        default: hipLaunchKernelGGL((k_fft_pow2<N, MAP_ROW>), dim3(grid), dim3(block), lds, s, P); break;
+        case MAP_COL: hipLaunchKernelGGL((k_fft_pow2<N, MAP_COL, IO>), dim3(grid), dim3(block), lds, s, P); break;
+        case MAP_ROWCOL: hipLaunchKernelGGL((k_fft_pow2<N, MAP_ROWCOL, IO>), dim3(grid), dim3(block), lds, s, P); break;
+        default: hipLaunchKernelGGL((k_fft_pow2<N, MAP_ROW, IO>), dim3(grid), dim3(block), lds, s, P); break;
     }
     return hipGetLastError();
 }
 
+template <int N>
+static hipError_t launch_n(const FftParams &P, int map, int io, unsigned grid, unsigned block, size_t lds,
+                           hipStream_t s)
+{
+    switch (io) {
+        case IO_PLAIN: return launch_io<N, IO_PLAIN>(P, map, grid, block, lds, s);
+        case IO_TWIDDLE: return launch_io<N, IO_TWIDDLE>(P, map, grid, block, lds, s);
+        default: return launch_io<N, IO_GENERIC>(P, map, grid, block, lds, s);
+    }
+}
+
 bool pow2_supported(int N) { return N >= 64 && N <= 8192 && (N & (N - 1)) == 0; }
-int pow2_lds_words(int N) { return N + N / 16 + 1; }
+int pow2_tb_host(int N, int map) { return pow2_tb(N, map); }
+// LDS bytes per transform (one padded component row of doubles)
+int pow2_lds_bytes(int N, int map) { return (N + N / 16 + (map == MAP_ROWCOL ? 2 : 1)) * 8; }
+
+// plain complex loads and plain / twiddled complex stores take the FAST path
+// when no position can reach the zero-padding or store limits
+bool pow2_fast(const FftParams &P)
+{
+    const long long imax = (P.ni - 1) * P.in.ipos + (long long)(P.N - 1) * P.in.pst;
+    const long long omax = (P.ni - 1) * P.out.ipos + (long long)(P.N - 1) * P.out.pst;
+    const long long win = 0x7fffffffLL;   // buffer windows are addressed with 31-bit byte offsets
+    return P.load_mode == LD_C2C && !P.conj_in && !P.chirp_in && imax < P.ilim &&
+           (P.store_mode == ST_C2C || (P.store_mode == ST_TWIDDLE && P.tw_mul == 1)) && !P.chirp_out &&
+           !P.conj_out && omax < P.olim && P.src_span > 0 && P.src_span <= win && P.dst_span > 0 &&
+           P.dst_span <= win;
+}
 
 hipError_t launch_fft_pow2(const FftParams &P, int map, unsigned grid, unsigned block, size_t lds, hipStream_t s)
 {
+    const int fast = pow2_fast(P) ? (P.store_mode == ST_TWIDDLE ? IO_TWIDDLE : IO_PLAIN) : IO_GENERIC;
     switch (P.N) {
-        case 64: return launch_n<64>(P, map, grid, block, lds, s);
-        case 128: return launch_n<128>(P, map, grid, block, lds, s);
-        case 256: return launch_n<256>(P, map, grid, block, lds, s);
-        case 512: return launch_n<512>(P, map, grid, block, lds, s);
-        case 1024: return launch_n<1024>(P, map, grid, block, lds, s);
-        case 2048: return launch_n<2048>(P, map, grid, block, lds, s);
-        case 4096: return launch_n<4096>(P, map, grid, block, lds, s);
-        case 8192: return launch_n<8192>(P, map, grid, block, lds, s);
+        case 64: return launch_n<64>(P, map, fast, grid, block, lds, s);
+        case 128: return launch_n<128>(P, map, fast, grid, block, lds, s);
+        case 256: return launch_n<256>(P, map, fast, grid, block, lds, s);
+        case 512: return launch_n<512>(P, map, fast, grid, block, lds, s);
+        case 1024: return launch_n<1024>(P, map, fast, grid, block, lds, s);
+        case 2048: return launch_n<2048>(P, map, fast, grid, block, lds, s);
+        case 4096: return launch_n<4096>(P, map, fast, grid, block, lds, s);
+        case 8192: return launch_n<8192>(P, map, fast, grid, block, lds, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "butterflies.h"
@@ -98,6 +99,22 @@ int device_check(Error &err)
 // ---------------------------------------------------------------------------
 // arithmetic helpers
 // ---------------------------------------------------------------------------
+
+static long long env_ll(const char *name, long long def)
+{
+    const char *v = std::getenv(name);
+    if (!v || !*v) return def;
+    char *end = nullptr;
+    const long long x = std::strtoll(v, &end, 10);
+    return (end && *end == 0 && x > 0) ? x : def;
+}
+
+static int ilog2(long long v)
+{
+    int l = 0;
+    while ((1LL << l) < v) ++l;
+    return l;
+}
 
 static bool smooth13(long long n)
 {
@@ -235,7 +252,7 @@ struct Builder {
     bool table_two_level(long long L, const double2 *&lo, const double2 *&hi, long long &S)
     {
         S = 1;
-        while (S * S < L) S *= 2;
+        while (S * S < L) S *= 2;   // a power of two: the kernels split m with a shift and a mask
         std::vector<double2> a, b;
         twiddle_table(a, (uint64_t)L, (uint64_t)S, 1);
         twiddle_table(b, (uint64_t)L, (uint64_t)((L + S - 1) / S + 1), (uint64_t)S);
@@ -317,17 +334,12 @@ struct Builder {
         else if (!colL && !colS) map = 0;
         else return false;
         const int NT = P.N / 16;
-        const long long per = (long long)pow2_lds_words(P.N) * (long long)sizeof(double2);
+        const long long per = (long long)pow2_lds_bytes(P.N, map);
         const long long cap = max_lds_bytes() - (long long)sizeof(long long) * 8 * kMaxTB - 1024;
         long long tb;
-        if (map == 0) {
-            tb = std::max<long long>(1, std::min<long long>(256 / NT, (76 * 1024) / per));
-        } else {
-            tb = std::max<long long>(1, std::min<long long>(kMaxThreads / NT, cap / per));
-            tb = std::min<long long>(tb, kMaxTB);
-        }
-        const long long want = std::max<long long>(1, (P.ntf + 511) / 512);
-        if (map == 0 && tb > want) tb = want;
+        // the kernel fixes its transforms per workgroup: 256 threads along rows,
+        // 512 across transforms
+        tb = pow2_tb_host(P.N, map);
         if (tb * per > cap) return false;
         P.TB = (int)tb;
         P.NT = NT;
@@ -346,8 +358,25 @@ struct Builder {
         ps.name = "k_fft_pow2";
         ps.bytes_in = side_bytes(P, true);
         ps.bytes_out = side_bytes(P, false);
+        make_alt(ps);
         plan.passes.push_back(ps);
         return true;
+    }
+
+    // LD_REAL_HALF with unit position stride reads x[2 pos], x[2 pos + 1]: the
+    // same bytes as one complex word, so the pass can run as a plain complex
+    // load when all batch strides are even and the pointer is 16-byte aligned
+    static void make_alt(Pass &ps)
+    {
+        const FftParams &P = ps.fp;
+        if (P.load_mode != LD_REAL_HALF || P.in.sax != 1 || P.in.kind != SIDE_DIMS || P.in.isig % 2) return;
+        for (int d = 0; d < P.in.nbd; ++d)
+            if (P.in.str[d] % 2) return;
+        ps.fp_alt = P;
+        ps.fp_alt.load_mode = LD_C2C;
+        for (int d = 0; d < P.in.nbd; ++d) ps.fp_alt.in.str[d] /= 2;
+        ps.fp_alt.in.isig /= 2;
+        ps.has_alt = true;
     }
 
     // colL / colS: whether consecutive transforms are closer in memory than
@@ -580,7 +609,15 @@ struct Builder {
         const double2 *chirp = nullptr, *spec = nullptr;
         if (blue && !bluestein_tables(Ls, n2, chirp, spec)) return false;
 
-        const size_t ws = (size_t)(nlog * n2) * sizeof(double2);
+        // Optional batch chunks (IMPULSE_FFT_MALL_MB): the workspace of one chunk
+        // stays within the budget so that pass A's output could still sit in the
+        // MALL (Infinity Cache) when pass B reads it.  Measured on MI355X the
+        // smaller launches lose more than the cache saves, so it is off unless set.
+        static const long long mall = env_ll("IMPULSE_FFT_MALL_MB", 1LL << 30) * (1LL << 20);
+        const long long per_sig = n2 * (long long)sizeof(double2);
+        const long long chunk = std::max<long long>(1, std::min<long long>(nlog, mall / per_sig));
+        long long cn = chunk;   // signals in the current chunk (captured by the pass builders)
+        const size_t ws = (size_t)(chunk * n2) * sizeof(double2);
         ensure_ws(plan.s0_bytes, ws);
         ensure_ws(plan.s1_bytes, ws);
 
@@ -593,11 +630,11 @@ struct Builder {
             P.LD = (int)L2;
             P.tw = twA;
             P.ni = L1;
-            P.ntf = L1 * nlog;
+            P.ntf = L1 * cn;
             P.nload = P.nstore = (int)L2;
             P.store_mode = ST_TWIDDLE;
-            P.tw_lo = lo; P.tw_hi = hi; P.tw_S = S; P.tw_L = n2; P.tw_mul = 1;
-            P.rtw_lo = rlo; P.rtw_hi = rhi; P.rtw_S = rS;
+            P.tw_lo = lo; P.tw_hi = hi; P.tw_S = S; P.tw_L = n2; P.tw_mul = 1; P.tw_logS = ilog2(S);
+            P.rtw_lo = rlo; P.rtw_hi = rhi; P.rtw_S = rS; P.rtw_logS = rS ? ilog2(rS) : 0;
             side_lin(P.out, n2, 1, L1);
             P.TB = choose_tb(P, P.ntf, true);
             P.col_load = P.col_store = P.TB > 1;
@@ -612,7 +649,7 @@ struct Builder {
             P.LD = (int)L1;
             P.tw = twB;
             P.ni = L2;
-            P.ntf = L2 * nlog;
+            P.ntf = L2 * cn;
             P.nload = P.nstore = (int)L1;
             P.load_mode = LD_C2C;
             side_lin(P.in, n2, L1, 1);
@@ -645,66 +682,75 @@ struct Builder {
         // real-half load twiddle W_{2 Ls}^{-p} is read from the two-level table
         P0.rtw = nullptr;
 
-        if (!blue) {
-            FftParams A = passA(P0);
-            A.load_mode = lmode;
-            A.sign = dir_sign;
-            if (!add_fft(A, J.src, REF_S0, false, true, true)) return false;
-            FftParams B = passB(P0);
-            B.sign = dir_sign;
-            if (J.kind == J_R2C) {
-                side_lin(B.out, n2, 1, L2);
-                B.store_mode = ST_C2C;
-                if (!add_fft(B, REF_S0, REF_S1, false, false, true)) return false;
-                return add_post(J, pair, Ls, n2, rlo, rhi, rS);
-            }
-            B.out = outS;
-            B.store_mode = (J.kind == J_C2C) ? ST_C2C : (pair ? ST_C2R_PAIR : ST_C2R_HALF);
-            B.olim = Ls;
-            return add_fft(B, REF_S0, J.dst, J.fct, false, true);
-        }
-
-        // Bluestein: A1 (chirp in) -> B1 (x spectrum) -> A2 (inverse) -> B2 (chirp out)
         const int conj = (J.kind == J_C2R) || (J.kind == J_C2C && !J.forward);
-        FftParams A1 = passA(P0);
-        A1.load_mode = lmode;
-        A1.sign = -1;
-        A1.chirp = chirp;
-        A1.chirp_in = 1;
-        A1.conj_in = conj;
-        A1.ilim = Ls;
-        if (!add_fft(A1, J.src, REF_S0, false, true, true)) return false;
-        FftParams B1 = passB(P0);
-        B1.sign = -1;
-        B1.store_mode = ST_MUL_TABLE;
-        B1.mtab = spec;
-        side_lin(B1.out, n2, 1, L2);
-        if (!add_fft(B1, REF_S0, REF_S1, false, false, true)) return false;
-        FftParams P1 = P0;
-        side_lin(P1.in, n2, 1, L1);
-        FftParams A2 = passA(P1);
-        A2.load_mode = LD_C2C;
-        A2.sign = 1;
-        if (!add_fft(A2, REF_S1, REF_S0, false, true, true)) return false;
-        FftParams B2 = passB(P0);
-        B2.sign = 1;
-        B2.chirp = chirp;
-        B2.chirp_out = 1;
-        B2.conj_out = conj;
-        B2.olim = Ls;
-        if (J.kind == J_R2C) {
-            side_lin(B2.out, n2, 1, L2);
-            B2.store_mode = ST_C2C;
-            if (!add_fft(B2, REF_S0, REF_S1, false, false, true)) return false;
-            return add_post(J, pair, Ls, n2, rlo, rhi, rS);
+        auto emit = [&](long long u0) -> bool {
+            P0.in.u0 = u0;
+            outS.u0 = u0;
+            if (!blue) {
+                FftParams A = passA(P0);
+                A.load_mode = lmode;
+                A.sign = dir_sign;
+                if (!add_fft(A, J.src, REF_S0, false, true, true)) return false;
+                FftParams B = passB(P0);
+                B.sign = dir_sign;
+                if (J.kind == J_R2C) {
+                    side_lin(B.out, n2, 1, L2);
+                    B.store_mode = ST_C2C;
+                    if (!add_fft(B, REF_S0, REF_S1, false, false, true)) return false;
+                    return add_post(J, pair, Ls, n2, rlo, rhi, rS, u0, cn);
+                }
+                B.out = outS;
+                B.store_mode = (J.kind == J_C2C) ? ST_C2C : (pair ? ST_C2R_PAIR : ST_C2R_HALF);
+                B.olim = Ls;
+                return add_fft(B, REF_S0, J.dst, J.fct, false, true);
+            }
+
+            // Bluestein: A1 (chirp in) -> B1 (x spectrum) -> A2 (inverse) -> B2 (chirp out)
+            FftParams A1 = passA(P0);
+            A1.load_mode = lmode;
+            A1.sign = -1;
+            A1.chirp = chirp;
+            A1.chirp_in = 1;
+            A1.conj_in = conj;
+            A1.ilim = Ls;
+            if (!add_fft(A1, J.src, REF_S0, false, true, true)) return false;
+            FftParams B1 = passB(P0);
+            B1.sign = -1;
+            B1.store_mode = ST_MUL_TABLE;
+            B1.mtab = spec;
+            side_lin(B1.out, n2, 1, L2);
+            if (!add_fft(B1, REF_S0, REF_S1, false, false, true)) return false;
+            FftParams P1 = P0;
+            side_lin(P1.in, n2, 1, L1);
+            FftParams A2 = passA(P1);
+            A2.load_mode = LD_C2C;
+            A2.sign = 1;
+            if (!add_fft(A2, REF_S1, REF_S0, false, true, true)) return false;
+            FftParams B2 = passB(P0);
+            B2.sign = 1;
+            B2.chirp = chirp;
+            B2.chirp_out = 1;
+            B2.conj_out = conj;
+            B2.olim = Ls;
+            if (J.kind == J_R2C) {
+                side_lin(B2.out, n2, 1, L2);
+                B2.store_mode = ST_C2C;
+                if (!add_fft(B2, REF_S0, REF_S1, false, false, true)) return false;
+                return add_post(J, pair, Ls, n2, rlo, rhi, rS, u0, cn);
+            }
+            B2.out = outS;
+            B2.store_mode = (J.kind == J_C2C) ? ST_C2C : (pair ? ST_C2R_PAIR : ST_C2R_HALF);
+            return add_fft(B2, REF_S0, J.dst, J.fct, false, true);
+        };
+        for (long long u0 = 0; u0 < nlog; u0 += chunk) {
+            cn = std::min(chunk, nlog - u0);
+            if (!emit(u0)) return false;
         }
-        B2.out = outS;
-        B2.store_mode = (J.kind == J_C2C) ? ST_C2C : (pair ? ST_C2R_PAIR : ST_C2R_HALF);
-        return add_fft(B2, REF_S0, J.dst, J.fct, false, true);
+        return true;
     }
 
     bool add_post(const Job &J, bool pair, long long Ls, long long zlen, const double2 *lo, const double2 *hi,
-                  long long S)
+                  long long S, long long u0, long long cn)
     {
         Pass ps;
         ps.kind = Pass::REAL_POST;
@@ -714,13 +760,15 @@ struct Builder {
         R.M = (int)Ls;
         R.fwd_flag = J.forward;
         R.fct = 1.0;
-        R.ntf = pair ? (J.nbatch() + 1) / 2 : J.nbatch();
+        R.ntf = cn;
         R.zlen = zlen;
         side_dims(R.out, J, false, pair);
+        R.out.u0 = u0;
         R.two_level = 1;
         R.tw_lo = lo;
         R.tw_hi = hi;
         R.tw_S = S;
+        R.tw_logS = ilog2(S);
         ps.src = REF_S1;
         ps.dst = J.dst;
         ps.uses_fct = J.fct;
@@ -947,14 +995,28 @@ int plan_execute(Plan &P, const void *in, void *out, double fct, hipStream_t str
     };
     const bool prof = P.prof_count < P.prof_max;
     hipEvent_t *ev = prof ? &P.prof_ev[(size_t)P.prof_count * P.passes.size() * 2] : nullptr;
+    // byte windows of the arrays around their element-0 pointers
+    auto window = [&](BufRef r, long long &lo, long long &span) {
+        switch (r) {
+            case REF_IN: lo = P.in_lo; span = P.in_hi - P.in_lo; break;
+            case REF_OUT: lo = P.out_lo; span = P.out_hi - P.out_lo; break;
+            case REF_S0: lo = 0; span = (long long)P.s0.bytes; break;
+            case REF_S1: lo = 0; span = (long long)P.s1.bytes; break;
+            case REF_TMP: lo = 0; span = (long long)P.tmp.bytes; break;
+            default: lo = 0; span = 0; break;
+        }
+    };
     for (size_t pi = 0; pi < P.passes.size(); ++pi) {
         auto &ps = P.passes[pi];
         hipError_t e;
         if (prof) (void)hipEventRecord(ev[2 * pi], stream);
         if (ps.kind == Pass::FFT_LDS || ps.kind == Pass::FFT_POW2) {
-            FftParams fp = ps.fp;
-            fp.src = resolve(ps.src);
+            const void *srcp = resolve(ps.src);
+            FftParams fp = (ps.has_alt && ((uintptr_t)srcp % 16) == 0) ? ps.fp_alt : ps.fp;
+            fp.src = srcp;
             fp.dst = resolve(ps.dst);
+            window(ps.src, fp.src_lo, fp.src_span);
+            window(ps.dst, fp.dst_lo, fp.dst_span);
             fp.fct = ps.uses_fct ? fct : 1.0;
             e = ps.kind == Pass::FFT_POW2 ? launch_fft_pow2(fp, ps.map, ps.grid, ps.block, ps.lds, stream)
                                           : launch_fft_lds(fp, ps.grid, ps.block, ps.lds, stream);

@@ -39,7 +39,21 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy three times (the reference "
+                         "reading for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    import torch
+    n = 1 << 27   # 2^27 doubles = 1 GiB
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 3}))
 
 
 def c_strides(shape, itemsize):
@@ -73,6 +87,20 @@ def parts():
              bb * 2.5 * nb * math.log2(nb), bb * (nb * 8 + (nb // 2 + 1) * 16),
              lambda: synth.uniform((bb, nb), -1.0, 1.0, seed=0)),
     ]
+
+
+def measured_traffic(kernel, threads):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by a plain copy's reading), or None."""
+    path = os.path.join(HERE, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    vals = [e["hbm_bytes_per_launch"] for e in rec.get("launches", [])
+            if e.get("kernel") == kernel and e.get("grid_threads") in threads]
+    return int(sum(vals) / len(vals)) if vals and len(vals) == len(threads) else None
 
 
 def torch_dtype(torch, s):
@@ -111,6 +139,9 @@ def cpu_baseline(plist, budget_s):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU runs are not part of this build", file=sys.stderr)
@@ -178,21 +209,23 @@ def main():
     total_flops = sum(p.flops for p in plist)
     value = total_flops / (ms_per_step * 1e-3) / 1e9
 
-    # dominant kernel: the kernel name with the largest summed launch time
+    # dominant kernel: the kernel instantiation (as rocprof names it) with the
+    # largest summed launch time per step
     by_kernel = {}
     for l_ in launches:
-        k = by_kernel.setdefault(l_["kernel"], {"ms": 0.0, "bytes": 0, "n": 0})
+        k = by_kernel.setdefault(l_["kernel"], {"ms": 0.0, "bytes": 0, "n": 0, "threads": set()})
         k["ms"] += l_["ms"]
         k["bytes"] += l_["bytes"]
         k["n"] += 1
+        k["threads"].add(l_["grid"] * l_["block"])
     dom = max(by_kernel, key=lambda k: by_kernel[k]["ms"])
     d = by_kernel[dom]
     avg_ms = d["ms"] / d["n"]
     avg_bytes = d["bytes"] / d["n"]
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                "launches_per_step": d["n"], "avg_launch_ms": round(avg_ms, 4),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, d["threads"]),
+                "kernel": dom, "launches_per_step": d["n"], "avg_launch_ms": round(avg_ms, 4),
                 "alg_bytes_per_launch": int(avg_bytes)}
 
     part_rows = []

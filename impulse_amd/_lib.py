@@ -23,10 +23,12 @@ EXPORTS = (
     "impulse_make_rfft_plan", "impulse_destroy_rfft_plan", "impulse_rfft_forward",
     "impulse_rfft_backward", "impulse_rfft_length",
     "impulse_fft_status_string", "impulse_fft_last_error", "impulse_fft_device_check",
+    "impulse_fft_conv2d_create", "impulse_fft_conv2d_forward", "impulse_fft_conv2d_backward",
+    "impulse_fft_conv2d_grid", "impulse_fft_conv2d_destroy",
 )
 
-C2C, R2C, C2R = 0, 1, 2
-F64 = 0
+C2C, R2C, C2R, R2C_ADJOINT, C2R_ADJOINT = 0, 1, 2, 3, 4
+F64, F32 = 0, 1
 
 _szp = ctypes.POINTER(ctypes.c_size_t)
 _pdp = ctypes.POINTER(ctypes.c_ssize_t)
@@ -60,6 +62,11 @@ def _sig(L):
             getattr(L, f"impulse_{kind}_{d_}").argtypes = [_vp, _vp, d]
         getattr(L, f"impulse_{kind}_length").argtypes = [_vp]
         getattr(L, f"impulse_{kind}_length").restype = sz
+    L.impulse_fft_conv2d_create.argtypes = [ctypes.POINTER(_vp), i, i, i, i, i, i]
+    L.impulse_fft_conv2d_forward.argtypes = [_vp, _vp, _vp, _vp, _vp]
+    L.impulse_fft_conv2d_backward.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.impulse_fft_conv2d_grid.argtypes = [_vp, _szp, _szp]
+    L.impulse_fft_conv2d_destroy.argtypes = [_vp]
     L.impulse_fft_status_string.argtypes = [i]
     L.impulse_fft_status_string.restype = ctypes.c_char_p
     L.impulse_fft_last_error.restype = ctypes.c_char_p

@@ -15,6 +15,11 @@ struct impulse_fft_plan_s {
     std::shared_ptr<ifft::Plan> plan;
 };
 
+struct impulse_fft_conv2d_s {
+    ifft::Conv2d *c = nullptr;
+    ~impulse_fft_conv2d_s() { ifft::conv2d_destroy(c); }
+};
+
 struct impulse_cfft_plan_s {
     size_t length = 0;
     std::unique_ptr<ifft::Plan> fwd, bwd;
@@ -194,7 +199,7 @@ int impulse_fft_plan_pass_info(impulse_fft_plan plan, int pass, char *name, size
     if (pass < 0 || pass >= (int)P.passes.size()) return fail(ifft::E_INVALID, "pass index out of range");
     const auto &ps = P.passes[pass];
     if (name && name_len) {
-        std::strncpy(name, ps.name, name_len - 1);
+        std::strncpy(name, ps.kname.empty() ? ps.name : ps.kname.c_str(), name_len - 1);
         name[name_len - 1] = 0;
     }
     if (bytes_in) *bytes_in = ps.bytes_in;
@@ -276,6 +281,61 @@ int impulse_fft_cache_clear(void)
     std::lock_guard<std::mutex> lk(c.mtx);
     c.lru.clear();
     c.map.clear();
+    return ifft::OK;
+}
+
+// ---- FFT convolution ----------------------------------------------------------
+
+int impulse_fft_conv2d_create(impulse_fft_conv2d *conv, int batch, int height, int width, int kh, int kw, int dtype)
+{
+    if (!conv) return fail(ifft::E_INVALID, "conv pointer is NULL");
+    *conv = nullptr;
+    ifft::Error err{0, ""};
+    int rc = ifft::device_check(err);
+    if (rc != ifft::OK) return fail(err);
+    ifft::Conv2d *c = nullptr;
+    rc = ifft::conv2d_create(&c, batch, height, width, kh, kw, dtype == IMPULSE_FFT_F32 ? 1 : (dtype == 0 ? 0 : -1), err);
+    if (rc != ifft::OK) return fail(err);
+    auto *h = new (std::nothrow) impulse_fft_conv2d_s;
+    if (!h) {
+        ifft::conv2d_destroy(c);
+        return fail(ifft::E_NOMEM, "out of host memory");
+    }
+    h->c = c;
+    *conv = h;
+    return ifft::OK;
+}
+
+int impulse_fft_conv2d_forward(impulse_fft_conv2d conv, const void *x, const void *k, void *y, void *stream)
+{
+    if (!conv || !conv->c || !x || !k || !y) return fail(ifft::E_INVALID, "NULL argument");
+    ifft::Error err{0, ""};
+    int rc = ifft::conv2d_forward(*conv->c, x, k, y, (hipStream_t)stream, err);
+    return rc == ifft::OK ? rc : fail(err);
+}
+
+int impulse_fft_conv2d_backward(impulse_fft_conv2d conv, const void *x, const void *k, const void *gy, void *gx,
+                                void *gk, void *stream)
+{
+    if (!conv || !conv->c || !x || !k || !gy) return fail(ifft::E_INVALID, "NULL argument");
+    ifft::Error err{0, ""};
+    int rc = ifft::conv2d_backward(*conv->c, x, k, gy, gx, gk, (hipStream_t)stream, err);
+    return rc == ifft::OK ? rc : fail(err);
+}
+
+int impulse_fft_conv2d_grid(impulse_fft_conv2d conv, size_t *P, size_t *Q)
+{
+    if (!conv || !conv->c) return fail(ifft::E_INVALID, "conv is NULL");
+    long long p, q;
+    ifft::conv2d_grid(*conv->c, p, q);
+    if (P) *P = (size_t)p;
+    if (Q) *Q = (size_t)q;
+    return ifft::OK;
+}
+
+int impulse_fft_conv2d_destroy(impulse_fft_conv2d conv)
+{
+    delete conv;
     return ifft::OK;
 }
 

@@ -23,6 +23,7 @@ hipError_t launch_unpack_halfcomplex(const double *hc, double2 *bins, long long 
 bool pow2_supported(int N);
 int pow2_lds_bytes(int N, int map);
 int pow2_tb_host(int N, int map);
+int pow2_io(const FftParams &P);
 hipError_t launch_fft_pow2(const FftParams &P, int map, unsigned grid, unsigned block, size_t lds, hipStream_t s);
 
 // status codes (mirror impulse_fft_status)
@@ -58,6 +59,7 @@ struct Pass {
     bool uses_fct = false;
     size_t bytes_in = 0, bytes_out = 0;   // bytes the launch must move at least once
     const char *name = "";
+    std::string kname;                    // kernel instantiation of the last execute, as rocprof names it
     // even-length real input at unit stride read as complex pairs (used when the
     // source pointer is 16-byte aligned at execute time)
     bool has_alt = false;
@@ -94,6 +96,15 @@ int plan_execute(Plan &plan, const void *in, void *out, double fct, hipStream_t 
 int plan_execute_host(Plan &plan, const void *in, void *out, double fct, Error &err);
 int plan_profile(Plan &plan, int max_executes, Error &err);
 int plan_profile_read(Plan &plan, double *pass_ms, int npass, int *executes, Error &err);
+
+// FFT convolution (conv.hip)
+struct Conv2d;
+int conv2d_create(Conv2d **out, int B, int H, int W, int KH, int KW, int dtype, Error &err);
+int conv2d_forward(Conv2d &c, const void *x, const void *k, void *y, hipStream_t s, Error &err);
+int conv2d_backward(Conv2d &c, const void *x, const void *k, const void *gy, void *gx, void *gk, hipStream_t s,
+                    Error &err);
+void conv2d_grid(const Conv2d &c, long long &P, long long &Q);
+void conv2d_destroy(Conv2d *c);
 
 // host FP64 twiddles (twiddle.hip)
 void twiddle_table(std::vector<double2> &out, uint64_t n, uint64_t count, uint64_t step);

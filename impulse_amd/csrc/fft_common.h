@@ -89,6 +89,7 @@ __device__ __forceinline__ double2 load_point(const FftParams &P, const TfInfo &
             double2 xq = x[T.ia + (M - pos) * S.sax];
             if (P.fwd_flag) { xp = cconj(xp); xq = cconj(xq); }
             if (pos == 0) { xp.y = 0.0; xq.y = 0.0; }
+            else if (P.adj_w) { xp = cscale(xp, 0.5); xq = cscale(xq, 0.5); }
             double2 e = cadd(xp, cconj(xq));
             double2 o = cmul(csub(xp, cconj(xq)), cconj(real_tw(P, pos)));
             return make_double2(e.x - o.y, e.y + o.x);
@@ -102,6 +103,7 @@ __device__ __forceinline__ double2 load_point(const FftParams &P, const TfInfo &
             double2 b = (T.flags & 2) ? x[T.ib + k * S.sax] : make_double2(0.0, 0.0);
             if (P.fwd_flag != (int)mirror) { a = cconj(a); b = cconj(b); }
             if (pos == 0) { a.y = 0.0; b.y = 0.0; }
+            else if (P.adj_w) { a = cscale(a, 0.5); b = cscale(b, 0.5); }
             return make_double2(a.x - b.y, a.y + b.x);
         }
     }

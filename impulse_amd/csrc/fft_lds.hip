@@ -104,7 +104,7 @@ __device__ __forceinline__ void store_point(const FftParams &P, const TfInfo &T,
             double2 d = csub(zk, zm);
             double2 o = make_double2(0.5 * d.y, -0.5 * d.x);  // -i/2 (zk - zm*)
             double2 X = cadd(e, cmul(o, real_tw(P, n)));
-            X = cscale(X, P.fct);
+            X = cscale(X, (P.adj_w && n != 0 && n != M) ? 2.0 * P.fct : P.fct);
             if (!P.fwd_flag) X = cconj(X);
             put_c(P, T.oa + pos * S.sax, X);
             return;
@@ -112,9 +112,10 @@ __device__ __forceinline__ void store_point(const FftParams &P, const TfInfo &T,
         case ST_R2C_PAIR: {
             const int L = P.M;
             double2 zk = buf[n], zm = cconj(buf[(L - n) % L]);
-            double2 xa = cscale(cadd(zk, zm), 0.5 * P.fct);
+            const double f = (P.adj_w && n != 0) ? 2.0 * P.fct : P.fct;
+            double2 xa = cscale(cadd(zk, zm), 0.5 * f);
             double2 d = csub(zk, zm);
-            double2 xb = make_double2(0.5 * P.fct * d.y, -0.5 * P.fct * d.x);
+            double2 xb = make_double2(0.5 * f * d.y, -0.5 * f * d.x);
             if (!P.fwd_flag) { xa = cconj(xa); xb = cconj(xb); }
             put_c(P, T.oa + pos * S.sax, xa);
             if (T.flags & 4) put_c(P, T.ob + pos * S.sax, xb);
@@ -238,19 +239,22 @@ __global__ void __launch_bounds__(256) k_real_post(const RealPostParams P)
             const double2 w = P.two_level ? twolevel(P.tw_lo, P.tw_hi, P.tw_logS, k) : P.rtw[k];
             const double2 e1 = cscale(cadd(zk, cconj(zm)), 0.5), d1 = csub(zk, cconj(zm));
             const double2 o1 = make_double2(0.5 * d1.y, -0.5 * d1.x);
-            double2 Xk = cscale(cadd(e1, cmul(o1, w)), P.fct);
+            const double fk = (P.adj_w && k != 0) ? 2.0 * P.fct : P.fct;
+            const double fm = (P.adj_w && L - k != L && L - k != 0) ? 2.0 * P.fct : P.fct;
+            double2 Xk = cscale(cadd(e1, cmul(o1, w)), fk);
             const double2 e2 = cscale(cadd(zm, cconj(zk)), 0.5), d2 = csub(zm, cconj(zk));
             const double2 o2 = make_double2(0.5 * d2.y, -0.5 * d2.x);
-            double2 Xm = cscale(cadd(e2, cmul(o2, make_double2(-w.x, w.y))), P.fct);
+            double2 Xm = cscale(cadd(e2, cmul(o2, make_double2(-w.x, w.y))), fm);
             if (!P.fwd_flag) { Xk = cconj(Xk); Xm = cconj(Xm); }
             y[oa + k * P.out.sax] = Xk;
             if (L - k != k) y[oa + (L - k) * P.out.sax] = Xm;
         } else {
             // odd pair: Xa = (zk + zm*)/2, Xb = -i/2 (zk - zm*) for k = 0 .. L/2
             const double2 za = cconj(zm);
-            double2 xa = cscale(cadd(zk, za), 0.5 * P.fct);
+            const double f = (P.adj_w && k != 0) ? 2.0 * P.fct : P.fct;
+            double2 xa = cscale(cadd(zk, za), 0.5 * f);
             const double2 d = csub(zk, za);
-            double2 xb = make_double2(0.5 * P.fct * d.y, -0.5 * P.fct * d.x);
+            double2 xb = make_double2(0.5 * f * d.y, -0.5 * f * d.x);
             if (!P.fwd_flag) { xa = cconj(xa); xb = cconj(xb); }
             y[oa + k * P.out.sax] = xa;
             if (hb) y[ob + k * P.out.sax] = xb;

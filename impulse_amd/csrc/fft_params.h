@@ -77,6 +77,8 @@ struct FftParams {
     int conj_in, conj_out;     // backward Bluestein runs the forward tables on conjugates
     int sign;                  // -1: forward stages, +1: backward stages
     int fwd_flag;              // r2c: 0 => conjugate output; c2r: 1 => conjugate input
+    int adj_w;                 // adjoint weights: c2r loads take interior bins x1/2, r2c stores
+                               // give interior bins x2 (bins 0 and n/2 keep weight 1)
     long long ilim;            // input positions >= ilim load as zero (Bluestein zero padding)
     long long olim;            // output positions >= olim are not stored
     int M;                     // half length (real-half modes) or odd length (pair modes)
@@ -112,6 +114,7 @@ struct FftParams {
 // signal) -> half spectra in the output layout.
 struct RealPostParams {
     int half;                  // 1: even length, Z has M points; 0: odd pair, Z has L points
+    int adj_w;                 // interior bins x2 (adjoint of c2r)
     int M;                     // M (half) or L (pair)
     int fwd_flag;
     double fct;

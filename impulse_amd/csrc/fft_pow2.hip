@@ -350,9 +350,14 @@ bool pow2_fast(const FftParams &P)
            P.dst_span <= win;
 }
 
+int pow2_io(const FftParams &P)
+{
+    return pow2_fast(P) ? (P.store_mode == ST_TWIDDLE ? IO_TWIDDLE : IO_PLAIN) : IO_GENERIC;
+}
+
 hipError_t launch_fft_pow2(const FftParams &P, int map, unsigned grid, unsigned block, size_t lds, hipStream_t s)
 {
-    const int fast = pow2_fast(P) ? (P.store_mode == ST_TWIDDLE ? IO_TWIDDLE : IO_PLAIN) : IO_GENERIC;
+    const int fast = pow2_io(P);
     switch (P.N) {
         case 64: return launch_n<64>(P, map, fast, grid, block, lds, s);
         case 128: return launch_n<128>(P, map, fast, grid, block, lds, s);

@@ -201,6 +201,7 @@ struct Job {
     int kind;
     long long L;         // transform length (real length for r2c / c2r)
     int forward;
+    int adj = 0;         // adjoint weights on the real pre/post-processing
     bool fct;
     BufRef src, dst;
     long long is_ax, os_ax;                       // scalar-unit strides along the axis
@@ -446,6 +447,7 @@ struct Builder {
         P.tw = M > 1 ? table_direct(M) : nullptr;
         if (M > 1 && !P.tw) return false;
         P.fwd_flag = J.forward;
+        P.adj_w = J.adj;
         P.M = (int)M;
         side_dims(P.in, J, true, pair);
         side_dims(P.out, J, false, pair);
@@ -535,6 +537,7 @@ struct Builder {
         P.chirp_in = P.conv = P.chirp_out = 1;
         P.sign = -1;
         P.fwd_flag = J.forward;
+        P.adj_w = J.adj;
         P.M = (int)Ls;
         P.ilim = Ls;
         P.nload = (int)Ls;
@@ -661,6 +664,7 @@ struct Builder {
 
         FftParams P0 = base(1);
         P0.fwd_flag = J.forward;
+        P0.adj_w = J.adj;
         P0.M = (int)Ls;
         // input side of pass A: the user's layout, inner index = n1 (position step L1)
         side_dims(P0.in, J, true, pair);
@@ -759,6 +763,7 @@ struct Builder {
         R.half = pair ? 0 : 1;
         R.M = (int)Ls;
         R.fwd_flag = J.forward;
+        R.adj_w = J.adj;
         R.fct = 1.0;
         R.ntf = cn;
         R.zlen = zlen;
@@ -860,7 +865,13 @@ int plan_create(std::unique_ptr<Plan> &out, int kind, int dtype, size_t ndim, co
                 const ptrdiff_t *stride_in, const ptrdiff_t *stride_out, size_t naxes, const size_t *axes,
                 int forward, Error &err)
 {
-    if (kind < 0 || kind > 2) { err = {E_INVALID, "unknown transform kind"}; return E_INVALID; }
+    if (kind < 0 || kind > 4) { err = {E_INVALID, "unknown transform kind"}; return E_INVALID; }
+    // adjoints of the real transforms (with respect to the real inner product):
+    // r2c* = c2r of the other direction with interior bins weighted 1/2,
+    // c2r* = r2c of the other direction with interior bins weighted 2
+    int adj = 0;
+    if (kind == 3) { kind = J_C2R; forward = !forward; adj = 1; }
+    else if (kind == 4) { kind = J_R2C; forward = !forward; adj = 1; }
     if (dtype != 0) { err = {E_UNSUPPORTED, "only IMPULSE_FFT_F64 is supported"}; return E_UNSUPPORTED; }
     if (ndim < 1 || !shape) { err = {E_INVALID, "ndim must be >= 1"}; return E_INVALID; }
     if (!stride_in || !stride_out) { err = {E_INVALID, "stride dimension mismatch"}; return E_INVALID; }
@@ -925,6 +936,7 @@ int plan_create(std::unique_ptr<Plan> &out, int kind, int dtype, size_t ndim, co
         }
     } else if (kind == J_R2C) {
         jobs.push_back(make_job(J_R2C, (long long)shape[last], P.forward, true, REF_IN, REF_OUT, last, P.shape, is, os));
+        jobs.back().adj = adj;
         for (size_t a = 0; a + 1 < naxes; ++a)
             jobs.push_back(make_job(J_C2C, (long long)shape_c[axes[a]], P.forward, false, REF_OUT, REF_OUT, axes[a],
                                     shape_c, os, os));
@@ -946,6 +958,7 @@ int plan_create(std::unique_ptr<Plan> &out, int kind, int dtype, size_t ndim, co
         const std::vector<long long> &cis = naxes > 1 ? ts : is;
         Job J = make_job(J_C2R, (long long)shape[last], P.forward, true, naxes > 1 ? REF_TMP : REF_IN, REF_OUT, last,
                          P.shape, cis, os);
+        J.adj = adj;
         jobs.push_back(J);
     }
 
@@ -1018,14 +1031,21 @@ int plan_execute(Plan &P, const void *in, void *out, double fct, hipStream_t str
             window(ps.src, fp.src_lo, fp.src_span);
             window(ps.dst, fp.dst_lo, fp.dst_span);
             fp.fct = ps.uses_fct ? fct : 1.0;
-            e = ps.kind == Pass::FFT_POW2 ? launch_fft_pow2(fp, ps.map, ps.grid, ps.block, ps.lds, stream)
-                                          : launch_fft_lds(fp, ps.grid, ps.block, ps.lds, stream);
+            if (ps.kind == Pass::FFT_POW2) {
+                e = launch_fft_pow2(fp, ps.map, ps.grid, ps.block, ps.lds, stream);
+                ps.kname = "k_fft_pow2<" + std::to_string(fp.N) + ", " + std::to_string(ps.map) + ", " +
+                           std::to_string(pow2_io(fp)) + ">";
+            } else {
+                e = launch_fft_lds(fp, ps.grid, ps.block, ps.lds, stream);
+                ps.kname = "k_fft_lds";
+            }
         } else {
             RealPostParams rp = ps.rp;
             rp.Z = (const double2 *)resolve(ps.src);
             rp.dst = resolve(ps.dst);
             rp.fct = ps.uses_fct ? fct : 1.0;
             e = launch_real_post(rp, stream);
+            ps.kname = "k_real_post";
         }
         if (prof) (void)hipEventRecord(ev[2 * pi + 1], stream);
         if (e != hipSuccess) {

@@ -44,11 +44,23 @@ typedef enum impulse_fft_status {
 typedef enum impulse_fft_kind {
     IMPULSE_FFT_C2C = 0,
     IMPULSE_FFT_R2C = 1,
-    IMPULSE_FFT_C2R = 2
+    IMPULSE_FFT_C2R = 2,
+    /* Adjoints for reverse-mode differentiation (no PocketFFT counterpart).
+     * With respect to the real inner products <x,y> = sum x y and
+     * <X,Y> = Re sum conj(X) Y over the stored half spectrum:
+     *   R2C_ADJOINT maps a half spectrum G (shape[last]/2+1 bins) to the real
+     *   array r2c*(G); it equals c2r of the opposite direction with bins
+     *   1..(n-1)/2 weighted 1/2 (`shape` is the REAL shape, as for C2R);
+     *   C2R_ADJOINT maps a real array g to c2r*(g); it equals r2c of the
+     *   opposite direction with those bins weighted 2 (`shape` is the real shape).
+     * The adjoint of C2C is C2C of the opposite direction. */
+    IMPULSE_FFT_R2C_ADJOINT = 3,
+    IMPULSE_FFT_C2R_ADJOINT = 4
 } impulse_fft_kind;
 
 typedef enum impulse_fft_dtype {
-    IMPULSE_FFT_F64 = 0 /* double / interleaved complex<double> (Nim float64 / Complex64) */
+    IMPULSE_FFT_F64 = 0, /* double / interleaved complex<double> (Nim float64 / Complex64) */
+    IMPULSE_FFT_F32 = 1  /* float input/output; arithmetic stays FP64 (FFT convolution) */
 } impulse_fft_dtype;
 
 /* ------------------------------------------------------------------------- */
@@ -88,7 +100,8 @@ size_t impulse_fft_plan_device_bytes(impulse_fft_plan plan);
 int impulse_fft_plan_num_launches(impulse_fft_plan plan);
 
 /* Describe kernel launch `pass` (0 <= pass < num_launches) of one execute:
- * kernel name, grid and block size, LDS bytes, and the bytes the launch must
+ * kernel name (the template instantiation of the last execute once the plan
+ * has run), grid and block size, LDS bytes, and the bytes the launch must
  * read (`bytes_in`) and write (`bytes_out`) if every byte moves once. */
 int impulse_fft_plan_pass_info(impulse_fft_plan plan, int pass, char *name, size_t name_len,
                                size_t *bytes_in, size_t *bytes_out, unsigned *grid, unsigned *block,
@@ -124,6 +137,29 @@ int impulse_fft_c2r(size_t ndim, const size_t *shape_out, const ptrdiff_t *strid
 int impulse_fft_cache_set_capacity(size_t entries);
 size_t impulse_fft_cache_size(void);
 int impulse_fft_cache_clear(void);
+
+/* ------------------------------------------------------------------------- */
+/* FFT convolution with a backward pass (no PocketFFT counterpart)            */
+/* ------------------------------------------------------------------------- */
+
+/* "same" 2-D convolution of a batch of images [batch, height, width] with one
+ * kernel [kh, kw] (origin at its centre ((kh-1)/2, (kw-1)/2), zero outside the
+ * image), through r2c / spectrum product / c2r on a 13-smooth grid of at
+ * least (height+kh-1) x (width+kw-1).  dtype IMPULSE_FFT_F32 or _F64 gives
+ * the element type of x, k, y and the gradients; arithmetic is FP64.  All
+ * pointers are DEVICE pointers, C-contiguous; calls are asynchronous on
+ * `stream` and calls on one conv object must be ordered by the caller. */
+typedef struct impulse_fft_conv2d_s *impulse_fft_conv2d;
+int impulse_fft_conv2d_create(impulse_fft_conv2d *conv, int batch, int height, int width, int kh, int kw,
+                              int dtype);
+int impulse_fft_conv2d_forward(impulse_fft_conv2d conv, const void *x, const void *k, void *y, void *stream);
+/* Gradients of a scalar loss: given gy = dL/dy, writes gx = dL/dx and
+ * gk = dL/dk (either may be NULL).  Built from the adjoint transforms; the
+ * spectra of x and k are recomputed rather than kept from the forward call. */
+int impulse_fft_conv2d_backward(impulse_fft_conv2d conv, const void *x, const void *k, const void *gy, void *gx,
+                                void *gk, void *stream);
+int impulse_fft_conv2d_grid(impulse_fft_conv2d conv, size_t *P, size_t *Q);
+int impulse_fft_conv2d_destroy(impulse_fft_conv2d conv);
 
 /* ------------------------------------------------------------------------- */
 /* 1-D in-place plans on host memory (C path)                                 */

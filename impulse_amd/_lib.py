@@ -25,6 +25,8 @@ EXPORTS = (
     "impulse_fft_status_string", "impulse_fft_last_error", "impulse_fft_device_check",
     "impulse_fft_conv2d_create", "impulse_fft_conv2d_forward", "impulse_fft_conv2d_backward",
     "impulse_fft_conv2d_grid", "impulse_fft_conv2d_destroy",
+    "impulse_fft_ragged_create", "impulse_fft_ragged_execute", "impulse_fft_ragged_tune",
+    "impulse_fft_ragged_configure", "impulse_fft_ragged_info", "impulse_fft_ragged_destroy",
 )
 
 C2C, R2C, C2R, R2C_ADJOINT, C2R_ADJOINT = 0, 1, 2, 3, 4
@@ -67,6 +69,13 @@ def _sig(L):
     L.impulse_fft_conv2d_backward.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.impulse_fft_conv2d_grid.argtypes = [_vp, _szp, _szp]
     L.impulse_fft_conv2d_destroy.argtypes = [_vp]
+    L.impulse_fft_ragged_create.argtypes = [ctypes.POINTER(_vp), sz, _szp, _szp, _szp, i]
+    L.impulse_fft_ragged_execute.argtypes = [_vp, _vp, _vp, d, _vp]
+    L.impulse_fft_ragged_tune.argtypes = [_vp, _vp, _vp, _vp, i, ctypes.POINTER(d)]
+    L.impulse_fft_ragged_configure.argtypes = [_vp, i, i, i]
+    ip = ctypes.POINTER(i)
+    L.impulse_fft_ragged_info.argtypes = [_vp, ip, ip, ip, ip, ctypes.POINTER(d), _szp]
+    L.impulse_fft_ragged_destroy.argtypes = [_vp]
     L.impulse_fft_status_string.argtypes = [i]
     L.impulse_fft_status_string.restype = ctypes.c_char_p
     L.impulse_fft_last_error.restype = ctypes.c_char_p

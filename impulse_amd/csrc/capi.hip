@@ -20,6 +20,11 @@ struct impulse_fft_conv2d_s {
     ~impulse_fft_conv2d_s() { ifft::conv2d_destroy(c); }
 };
 
+struct impulse_fft_ragged_s {
+    ifft::Ragged *r = nullptr;
+    ~impulse_fft_ragged_s() { ifft::ragged_destroy(r); }
+};
+
 struct impulse_cfft_plan_s {
     size_t length = 0;
     std::unique_ptr<ifft::Plan> fwd, bwd;
@@ -336,6 +341,66 @@ int impulse_fft_conv2d_grid(impulse_fft_conv2d conv, size_t *P, size_t *Q)
 int impulse_fft_conv2d_destroy(impulse_fft_conv2d conv)
 {
     delete conv;
+    return ifft::OK;
+}
+
+// ---- ragged batches -----------------------------------------------------------
+
+int impulse_fft_ragged_create(impulse_fft_ragged *batch, size_t count, const size_t *lengths,
+                              const size_t *in_offsets, const size_t *out_offsets, int forward)
+{
+    if (!batch) return fail(ifft::E_INVALID, "batch pointer is NULL");
+    *batch = nullptr;
+    ifft::Error err{0, ""};
+    ifft::Ragged *r = nullptr;
+    int rc = ifft::ragged_create(&r, count, lengths, in_offsets, out_offsets, forward, err);
+    if (rc != ifft::OK) return fail(err);
+    auto *h = new (std::nothrow) impulse_fft_ragged_s;
+    if (!h) {
+        ifft::ragged_destroy(r);
+        return fail(ifft::E_NOMEM, "out of host memory");
+    }
+    h->r = r;
+    *batch = h;
+    return ifft::OK;
+}
+
+int impulse_fft_ragged_execute(impulse_fft_ragged batch, const float *in, void *out, double fct, void *stream)
+{
+    if (!batch || !batch->r) return fail(ifft::E_INVALID, "batch is NULL");
+    ifft::Error err{0, ""};
+    int rc = ifft::ragged_execute(*batch->r, in, (double2 *)out, fct, (hipStream_t)stream, true, err);
+    return rc == ifft::OK ? rc : fail(err);
+}
+
+int impulse_fft_ragged_tune(impulse_fft_ragged batch, const float *in, void *out, void *stream, int reps,
+                            double *best_ms)
+{
+    if (!batch || !batch->r) return fail(ifft::E_INVALID, "batch is NULL");
+    ifft::Error err{0, ""};
+    int rc = ifft::ragged_tune(*batch->r, in, (double2 *)out, (hipStream_t)stream, reps, best_ms, err);
+    return rc == ifft::OK ? rc : fail(err);
+}
+
+int impulse_fft_ragged_configure(impulse_fft_ragged batch, int threads, int lds_points, int radix_order)
+{
+    if (!batch || !batch->r) return fail(ifft::E_INVALID, "batch is NULL");
+    ifft::Error err{0, ""};
+    int rc = ifft::ragged_configure(*batch->r, threads, lds_points, radix_order, err);
+    return rc == ifft::OK ? rc : fail(err);
+}
+
+int impulse_fft_ragged_info(impulse_fft_ragged batch, int *threads, int *lds_points, int *radix_order, int *grid,
+                            double *flops, size_t *scratch_bytes)
+{
+    if (!batch || !batch->r) return fail(ifft::E_INVALID, "batch is NULL");
+    ifft::ragged_info(*batch->r, threads, lds_points, radix_order, grid, flops, scratch_bytes);
+    return ifft::OK;
+}
+
+int impulse_fft_ragged_destroy(impulse_fft_ragged batch)
+{
+    delete batch;
     return ifft::OK;
 }
 

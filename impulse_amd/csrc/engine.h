@@ -106,6 +106,17 @@ int conv2d_backward(Conv2d &c, const void *x, const void *k, const void *gy, voi
 void conv2d_grid(const Conv2d &c, long long &P, long long &Q);
 void conv2d_destroy(Conv2d *c);
 
+// ragged batches of real transforms (ragged.hip)
+struct Ragged;
+int ragged_create(Ragged **out, size_t count, const size_t *lengths, const size_t *in_off, const size_t *out_off,
+                  int forward, Error &err);
+int ragged_execute(Ragged &R, const float *x, double2 *y, double fct, hipStream_t s, bool graph, Error &err);
+int ragged_tune(Ragged &R, const float *x, double2 *y, hipStream_t s, int reps, double *best_ms, Error &err);
+int ragged_configure(Ragged &R, int threads, int lds_max, int order, Error &err);
+void ragged_info(const Ragged &R, int *threads, int *lds_max, int *order, int *grid, double *flops,
+                 size_t *scratch_bytes);
+void ragged_destroy(Ragged *r);
+
 // host FP64 twiddles (twiddle.hip)
 void twiddle_table(std::vector<double2> &out, uint64_t n, uint64_t count, uint64_t step);
 double2 twiddle_at(uint64_t k, uint64_t n);   // exp(-2 pi i k / n), exact at multiples of pi/4

@@ -1,0 +1,725 @@
+// Ragged batches: thousands of real transforms of unequal lengths in one
+// persistent launch.
+//
+// Every workgroup takes whole signals from a queue (one vector atomic per
+// claim; the host orders the queue by decreasing cost, so the tail of the
+// launch is short signals).  For a real signal of length n the workgroup forms
+// a complex signal z of length M (M = n/2, z[m] = x[2m] + i x[2m+1] for even n;
+// M = n with zero imaginary part for odd n), transforms it, and writes the
+// n/2+1 bins.  The host plans each signal (task table):
+//   M 13-smooth, fits the LDS budget : Stockham stages in LDS;
+//   M 13-smooth, longer              : four-step M = L1 L2 through the
+//                                      workgroup's global scratch (batches of
+//                                      columns, then batches of rows, in LDS);
+//   otherwise                        : Bluestein with a 7-smooth n2 >= 2M-1;
+//                                      the same workgroup computes the chirp
+//                                      kernel's spectrum (three transforms of
+//                                      length n2).
+// Twiddles and the chirp exp(-i pi j^2 / M) are generated in FP64 with
+// sincospi (the chirp index j^2 mod 2M in 64-bit integers).  Phases that hand
+// data to other threads of the workgroup through global scratch are separated
+// by a device-scope fence and a barrier.  Input is float32; arithmetic and the
+// output bins (complex128) are FP64.
+//
+// The autotuner chooses the workgroup size, the LDS budget per transform
+// (which decides which lengths stay in LDS and how four-step lengths split)
+// and the radix order of the LDS stages.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "fft_common.h"
+
+namespace ifft {
+
+struct RaggedTask {
+    long long in_off;    // first float of the signal
+    long long out_off;   // first complex bin of the output
+    int n;               // real length
+    int M;               // complex length (n/2 or n)
+    int L;               // transform length: M, or n2 for Bluestein
+    int L1, L2;          // four-step split of L (0: L stays in LDS)
+    int ca, cb;          // columns per LDS batch in pass A (length L2) / rows in pass B (length L1)
+    int blue;            // Bluestein
+};
+
+struct RaggedParams {
+    const float *x;
+    double2 *y;
+    const RaggedTask *tasks;
+    int ntasks;
+    int *counter;
+    double2 *scratch;          // per workgroup: 3 buffers of `slot` complex points
+    long long slot;
+    int order;                 // radix order of LDS stages: 0 powers of two first, 1 odd radices first
+    double fct;                // output scale
+    double osign;              // +1: forward bins; -1: conjugated (backward sign)
+};
+
+// ---------------------------------------------------------------------------
+// shared host/device planning helpers
+// ---------------------------------------------------------------------------
+
+__host__ __device__ inline bool rg_smooth13(long long n)
+{
+    if (n < 1) return false;
+    const int ps[6] = {2, 3, 5, 7, 11, 13};
+    for (int k = 0; k < 6; ++k)
+        while (n % ps[k] == 0) n /= ps[k];
+    return n == 1;
+}
+
+// radix list of a 13-smooth n (powers of two as 16s and one 2/4/8)
+__host__ __device__ inline int rg_radices(long long n, int order, int *r)
+{
+    int k = 0, twos = 0;
+    while (n % 2 == 0) { n /= 2; ++twos; }
+    int odd[kMaxStages], no = 0;
+    const int ps[5] = {3, 5, 7, 11, 13};
+    for (int q = 0; q < 5; ++q)
+        while (n % ps[q] == 0) { odd[no++] = ps[q]; n /= ps[q]; }
+    if (order == 1)
+        for (int q = no - 1; q >= 0; --q) r[k++] = odd[q];
+    while (twos >= 4) { r[k++] = 16; twos -= 4; }
+    if (twos == 3) r[k++] = 8;
+    else if (twos == 2) r[k++] = 4;
+    else if (twos == 1) r[k++] = 2;
+    if (order == 0)
+        for (int q = 0; q < no; ++q) r[k++] = odd[q];
+    return k;
+}
+
+// Every configuration keeps lds_points <= 8 threads, so a thread owns at most
+// 8 points of any LDS batch and ceil(8 / R) butterflies of a radix-R stage.
+constexpr int kPointsPerThread = 8;
+__host__ __device__ constexpr int nb_rg(int R) { return (kPointsPerThread + R - 1) / R; }
+
+// transforms of length L per LDS batch: largest power of two c <= 64 with c L <= lds_max
+__host__ __device__ inline int rg_batch(long long L, int lds_max)
+{
+    int c = 1;
+    while (2 * c <= 64 && 2 * c * L <= lds_max) c *= 2;
+    return c;
+}
+
+static long long rg_good7(long long n)
+{
+    long long best = 1LL << 62;
+    for (long long f7 = 1; f7 < best; f7 *= 7)
+        for (long long f5 = f7; f5 < best; f5 *= 5)
+            for (long long f3 = f5; f3 < best; f3 *= 3) {
+                long long v = f3;
+                while (v < n) v *= 2;
+                if (v < best) best = v;
+                if (f3 >= n) break;
+            }
+    return best;
+}
+
+// ---------------------------------------------------------------------------
+// device code
+// ---------------------------------------------------------------------------
+
+// exp(sign 2 pi i m / L)
+__device__ __forceinline__ double2 twid(long long m, long long L, double sign)
+{
+    double s, c;
+    sincospi(2.0 * (double)(m % L) / (double)L, &s, &c);
+    return make_double2(c, sign * s);
+}
+
+// chirp w[j] = exp(-i pi j^2 / M)
+__device__ __forceinline__ double2 chirp(long long j, long long M)
+{
+    const unsigned long long m2 = 2ull * (unsigned long long)M;
+    const unsigned long long jj = (unsigned long long)j % m2;
+    const long long idx = (long long)((jj * jj) % m2);
+    double s, c;
+    sincospi((double)idx / (double)M, &s, &c);
+    return make_double2(c, -s);
+}
+
+// exp(sign 2 pi i k / L), 0 <= k < L < 2^31
+__device__ __forceinline__ double2 twid32(int k, int L, double sign)
+{
+    double s, c;
+    sincospi(2.0 * (double)k / (double)L, &s, &c);
+    return make_double2(c, sign * s);
+}
+
+// Stockham stage on one transform in LDS: butterflies b = i, i + NT, ...
+// The twiddle roots are generated before the values are loaded, so the
+// sincospi evaluation does not hold the R NB values live.
+template <int R, int NB>
+__device__ __noinline__ void rstage(double2 *buf, int N, int p, int i, int NT, double s)
+{
+    const int nb = N / R;
+    double2 w1[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        const int k = (i + c * NT) % p;
+        w1[c] = k ? twid32(k, p * R, s) : make_double2(1.0, 0.0);
+    }
+    double2 v[NB][R];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        const int b = i + c * NT;
+        if (b < nb)
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[c][q] = buf[b + q * nb];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        const int b = i + c * NT;
+        if (b < nb) {
+            const int k = b % p;
+            if (k) {
+                double2 w = w1[c];
+#pragma unroll
+                for (int q = 1; q < R; ++q) {
+                    v[c][q] = cmul(v[c][q], w);
+                    if (q + 1 < R) w = cmul(w, w1[c]);
+                }
+            }
+            dft<R>(v[c], s);
+            const int j = (b - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) buf[j + q * p] = v[c][q];
+        }
+    }
+    __syncthreads();
+}
+
+// Radix sequence of a 13-smooth L as 4-bit codes (stage 0 in the low bits),
+// in registers: powers of two as 16s and one 2/4/8, odd radices ascending,
+// before (order 1, descending) or after (order 0) the powers of two.
+__device__ __forceinline__ unsigned long long radix_codes(int L, int order, int &ns)
+{
+    // code -> radix: 1:2 2:3 3:4 4:5 5:7 6:8 7:11 8:13 9:16
+    unsigned long long pw = 0, od = 0;
+    int np = 0, no = 0;
+    int n = L;
+    int twos = __builtin_ctz((unsigned)n);
+    n >>= twos;
+    for (; twos >= 4; twos -= 4) pw |= 9ull << (4 * np++);
+    if (twos) pw |= (unsigned long long)(twos == 3 ? 6 : twos == 2 ? 3 : 1) << (4 * np++);
+    const int ps[5] = {3, 5, 7, 11, 13}, cs[5] = {2, 4, 5, 7, 8};
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+        while (n % ps[q] == 0) {
+            n /= ps[q];
+            if (order == 1) od = (od << 4) | (unsigned long long)cs[q];   // descending: latest first
+            else od |= (unsigned long long)cs[q] << (4 * no);
+            ++no;
+        }
+    ns = np + no;
+    return order == 1 ? (od | (pw << (4 * no))) : (pw | (od << (4 * np)));
+}
+
+// `count` transforms of length L at lds + t L, blockDim.x / count threads each
+// (count divides blockDim.x).  Ends with a barrier.
+__device__ __noinline__ void lds_fft(double2 *lds, int L, int count, double s, int order)
+{
+    int ns;
+    unsigned long long codes = radix_codes(L, order, ns);
+    const int NT = blockDim.x / count;
+    const int t = threadIdx.x / NT, i = threadIdx.x % NT;
+    double2 *buf = lds + (long long)t * L;
+    int p = 1;
+    for (int st = 0; st < ns; ++st, codes >>= 4) {
+        switch ((int)(codes & 15)) {
+            case 1: rstage<2, nb_rg(2)>(buf, L, p, i, NT, s); p *= 2; break;
+            case 2: rstage<3, nb_rg(3)>(buf, L, p, i, NT, s); p *= 3; break;
+            case 3: rstage<4, nb_rg(4)>(buf, L, p, i, NT, s); p *= 4; break;
+            case 4: rstage<5, nb_rg(5)>(buf, L, p, i, NT, s); p *= 5; break;
+            case 5: rstage<7, nb_rg(7)>(buf, L, p, i, NT, s); p *= 7; break;
+            case 6: rstage<8, nb_rg(8)>(buf, L, p, i, NT, s); p *= 8; break;
+            case 7: rstage<11, nb_rg(11)>(buf, L, p, i, NT, s); p *= 11; break;
+            case 8: rstage<13, nb_rg(13)>(buf, L, p, i, NT, s); p *= 13; break;
+            default: rstage<16, nb_rg(16)>(buf, L, p, i, NT, s); p *= 16; break;
+        }
+    }
+    if (ns == 0) __syncthreads();
+}
+
+__device__ __forceinline__ void wg_sync_global()
+{
+    __threadfence();
+    __syncthreads();
+}
+
+// Complex transform of length T.L (sign s) by the workgroup; load(m) gives
+// input point m.  Returns true with the result in LDS (natural order), or
+// false with the result in `out` (global).  `tmp` is global scratch of L
+// points for the four-step split.
+template <class LOAD>
+__device__ bool wg_fft(LOAD load, const RaggedTask &T, double s, double2 *lds, double2 *tmp, double2 *out,
+                       int order)
+{
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const long long L = T.L;
+    if (T.L1 == 0) {
+        for (long long m = tid; m < L; m += nthr) lds[m] = load(m);
+        __syncthreads();
+        lds_fft(lds, (int)L, 1, s, order);
+        return true;
+    }
+    const int L1 = T.L1, L2 = T.L2, ca = T.ca, cb = T.cb;
+    // pass A: columns n1 (length L2, stride L1) times W_L^{n1 k2} -> tmp[n1 + L1 k2]
+    for (int c0 = 0; c0 < L1; c0 += ca) {
+        const int cnt = min(ca, L1 - c0);
+        for (int e = tid; e < ca * L2; e += nthr) {
+            const int t = e % ca, n2 = e / ca;
+            lds[t * L2 + n2] = t < cnt ? load((long long)(c0 + t) + (long long)L1 * n2) : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        lds_fft(lds, L2, ca, s, order);
+        {
+            // t = tid % ca is fixed per thread (ca divides nthr); k2 advances by nthr / ca
+            const int t = tid % ca, step = nthr / ca;
+            const long long n1 = c0 + t;
+            double2 w = twid(n1 * (long long)(tid / ca), L, s);
+            const double2 ws = twid(n1 * (long long)step, L, s);
+            for (int k2 = tid / ca; k2 < L2; k2 += step) {
+                if (t < cnt) tmp[n1 + (long long)L1 * k2] = cmul(lds[t * L2 + k2], w);
+                w = cmul(w, ws);
+            }
+        }
+        __syncthreads();
+    }
+    wg_sync_global();
+    // pass B: rows k2 (length L1, contiguous in tmp) -> out[k2 + L2 k1]
+    for (int r0 = 0; r0 < L2; r0 += cb) {
+        const int cnt = min(cb, L2 - r0);
+        for (int e = tid; e < cb * L1; e += nthr) {
+            const int t = e / L1;
+            lds[e] = t < cnt ? tmp[(long long)(r0 + t) * L1 + (e - t * L1)] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        lds_fft(lds, L1, cb, s, order);
+        for (int e = tid; e < cb * L1; e += nthr) {
+            const int t = e % cb, k1 = e / cb;
+            if (t < cnt) out[(long long)(r0 + t) + (long long)L2 * k1] = lds[t * L1 + k1];
+        }
+        __syncthreads();
+    }
+    wg_sync_global();
+    return false;
+}
+
+__global__ void k_ragged_reset(int *counter)
+{
+    if (threadIdx.x == 0) counter[0] = 0;
+}
+
+__global__ void __launch_bounds__(1024) k_ragged(const RaggedParams P)
+{
+    extern __shared__ double2 lds[];
+    __shared__ int s_task;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    double2 *S0 = P.scratch + (long long)blockIdx.x * 3 * P.slot;   // four-step temporary
+    double2 *S1 = S0 + P.slot;                                        // transform results
+    double2 *S2 = S1 + P.slot;                                        // Bluestein kernel spectrum
+    for (;;) {
+        if (tid == 0) s_task = atomicAdd(P.counter, 1);
+        __syncthreads();
+        const int task = __builtin_amdgcn_readfirstlane(s_task);   // uniform: the task record loads as scalars
+        __syncthreads();
+        if (task >= P.ntasks) break;
+        const RaggedTask T = P.tasks[task];
+        const long long n = T.n, M = T.M;
+        const bool half = (n % 2) == 0;
+        const float *x = P.x + T.in_off;
+        double2 *y = P.y + T.out_off;
+        auto zin = [&](long long m) -> double2 {
+            return half ? make_double2((double)x[2 * m], (double)x[2 * m + 1]) : make_double2((double)x[m], 0.0);
+        };
+        bool in_lds;
+        if (!T.blue) {
+            in_lds = wg_fft(zin, T, -1.0, lds, S0, S1, P.order);
+        } else {
+            const long long n2 = T.L;
+            const double inv = 1.0 / (double)n2;
+            // kernel spectrum B = FFT(b) / n2, b[m] = conj(w[|m|]) for |m| < M (indices mod n2)
+            auto bload = [&](long long m) -> double2 {
+                if (m < M) return cconj(chirp(m, M));
+                if (m > n2 - M) return cconj(chirp(n2 - m, M));
+                return make_double2(0.0, 0.0);
+            };
+            if (wg_fft(bload, T, -1.0, lds, S0, S2, P.order)) {
+                for (long long m = tid; m < n2; m += nthr) S2[m] = cscale(lds[m], inv);
+            } else {
+                for (long long m = tid; m < n2; m += nthr) S2[m] = cscale(S2[m], inv);
+            }
+            wg_sync_global();
+            // A = FFT(z w); C = A B; c = IFFT(C) (unnormalized; B carries 1/n2)
+            auto aload = [&](long long m) -> double2 {
+                return m < M ? cmul(zin(m), chirp(m, M)) : make_double2(0.0, 0.0);
+            };
+            if (wg_fft(aload, T, -1.0, lds, S0, S1, P.order)) {
+                for (long long m = tid; m < n2; m += nthr) lds[m] = cmul(lds[m], S2[m]);
+                __syncthreads();
+                lds_fft(lds, (int)n2, 1, 1.0, P.order);
+                in_lds = true;
+            } else {
+                for (long long m = tid; m < n2; m += nthr) S1[m] = cmul(S1[m], S2[m]);
+                wg_sync_global();
+                auto cload = [&](long long m) -> double2 { return S1[m]; };
+                in_lds = wg_fft(cload, T, 1.0, lds, S0, S1, P.order);
+            }
+        }
+        // Z[k] = FFT_M(z)[k] (Bluestein: times the chirp)
+        auto Z = [&](long long k) -> double2 {
+            const double2 v = in_lds ? lds[k] : S1[k];
+            return T.blue ? cmul(v, chirp(k, M)) : v;
+        };
+        const double f = P.fct, fo = P.fct * P.osign;
+        if (half) {
+            // X[k] = E + W_n^k O with E = (Z[k] + conj Z[M-k]) / 2, O = (Z[k] - conj Z[M-k]) / 2i;
+            // X[M-k] from the same pair with W_n^{M-k} = -conj(W_n^k)
+            for (long long k = tid; k <= M / 2; k += nthr) {
+                const long long km = (M - k) % M;
+                const double2 zk = Z(k), zm = Z(km);
+                const double2 w = twid(k, n, -1.0);
+                const double2 e1 = cscale(cadd(zk, cconj(zm)), 0.5), d1 = csub(zk, cconj(zm));
+                const double2 o1 = make_double2(0.5 * d1.y, -0.5 * d1.x);
+                const double2 x1 = cadd(e1, cmul(o1, w));
+                y[k] = make_double2(f * x1.x, fo * x1.y);
+                if (M - k != k) {
+                    const double2 e2 = cscale(cadd(zm, cconj(zk)), 0.5), d2 = csub(zm, cconj(zk));
+                    const double2 o2 = make_double2(0.5 * d2.y, -0.5 * d2.x);
+                    const double2 x2 = cadd(e2, cmul(o2, make_double2(-w.x, w.y)));
+                    y[M - k] = make_double2(f * x2.x, fo * x2.y);
+                }
+            }
+        } else {
+            for (long long k = tid; k <= n / 2; k += nthr) {
+                const double2 v = Z(k);
+                y[k] = make_double2(f * v.x, fo * v.y);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+struct Ragged {
+    int device = 0;
+    std::vector<size_t> n, in_off, out_off;   // as given
+    std::vector<int> order;                   // queue order (decreasing cost)
+    int forward = 1;
+    // configuration
+    int threads = 0, lds_max = 0, rorder = 0;
+    // derived for the configuration
+    std::vector<RaggedTask> tasks;
+    long long slot = 0;
+    int grid = 0;
+    DevBuf dtasks, counter, scratch;
+    double flops = 0.0;
+    size_t in_elems = 0, out_elems = 0;
+    // captured graph for (x, y, fct)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void *gx = nullptr;
+    void *gy = nullptr;
+    double gfct = 0.0;
+    void drop_graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~Ragged() { drop_graph(); }
+};
+
+// plan one signal for (threads, lds_max); false if no split fits
+static bool plan_task(long long n, int lds_max, RaggedTask &t)
+{
+    const long long M = (n % 2 == 0) ? n / 2 : n;
+    t.n = (int)n;
+    t.M = (int)M;
+    t.blue = rg_smooth13(M) ? 0 : 1;
+    const long long L = t.blue ? rg_good7(2 * M - 1) : M;
+    if (L > (1LL << 30)) return false;
+    t.L = (int)L;
+    t.L1 = t.L2 = t.ca = t.cb = 0;
+    if (L <= lds_max) return true;
+    // L = L1 L2 with both factors in LDS; most balanced split
+    long long bestd = 1LL << 62;
+    for (long long a = 2; a <= lds_max && a < L; ++a) {
+        if (L % a) continue;
+        const long long b = L / a;
+        if (b > lds_max) continue;
+        const long long d = a > b ? a - b : b - a;
+        if (d < bestd) { bestd = d; t.L2 = (int)a; t.L1 = (int)b; }
+    }
+    if (!t.L1) return false;
+    t.ca = rg_batch(t.L2, lds_max);
+    t.cb = rg_batch(t.L1, lds_max);
+    return true;
+}
+
+constexpr long long kScratchCap = 16LL << 30;
+
+static int build_tasks(Ragged &R, int threads, int lds_max, Error &err)
+{
+    std::vector<RaggedTask> tasks(R.order.size());
+    long long slot = 1;
+    for (size_t q = 0; q < R.order.size(); ++q) {
+        const size_t i = (size_t)R.order[q];
+        RaggedTask &t = tasks[q];
+        if (!plan_task((long long)R.n[i], lds_max, t)) {
+            err = {E_UNSUPPORTED, "ragged signal of length " + std::to_string(R.n[i]) +
+                                      " has no four-step split for this configuration"};
+            return E_UNSUPPORTED;
+        }
+        t.in_off = (long long)R.in_off[i];
+        t.out_off = (long long)R.out_off[i];
+        if (t.L1 || t.blue) slot = std::max<long long>(slot, t.L);
+    }
+    int ncu = 0, per = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, R.device) != hipSuccess || ncu <= 0)
+        ncu = 256;
+    const size_t lds = (size_t)lds_max * sizeof(double2);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ragged, threads, lds) != hipSuccess || per <= 0) {
+        (void)hipGetLastError();
+        per = 1;
+    }
+    // persistent grid: every resident workgroup, with the per-workgroup scratch
+    // held to kScratchCap in total (fewer resident workgroups past that)
+    const long long per_wg = 3 * slot * (long long)sizeof(double2);
+    long long grid_ll = std::min<long long>((long long)ncu * per, (long long)tasks.size());
+    grid_ll = std::max<long long>(1, std::min<long long>(grid_ll, std::max<long long>(ncu, kScratchCap / per_wg)));
+    const int grid = (int)grid_ll;
+    const size_t sbytes = (size_t)grid * (size_t)per_wg;
+    if (sbytes > R.scratch.bytes && R.scratch.alloc(sbytes)) {
+        err = {E_NOMEM, "ragged scratch allocation failed"};
+        return E_NOMEM;
+    }
+    if (hipMemcpy(R.dtasks.p, tasks.data(), tasks.size() * sizeof(RaggedTask), hipMemcpyHostToDevice) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        err = {E_HIP, "ragged task upload failed"};
+        return E_HIP;
+    }
+    R.tasks.swap(tasks);
+    R.slot = slot;
+    R.grid = grid;
+    R.threads = threads;
+    R.lds_max = lds_max;
+    R.drop_graph();
+    return OK;
+}
+
+int ragged_create(Ragged **out, size_t count, const size_t *lengths, const size_t *in_off, const size_t *out_off,
+                  int forward, Error &err)
+{
+    if (!out) { err = {E_INVALID, "null handle pointer"}; return E_INVALID; }
+    *out = nullptr;
+    if (!count || !lengths || !in_off || !out_off) { err = {E_INVALID, "empty ragged batch"}; return E_INVALID; }
+    if (count > (size_t)(1 << 30)) { err = {E_INVALID, "too many signals"}; return E_INVALID; }
+    int rc = device_check(err);
+    if (rc != OK) return rc;
+    auto R = std::make_unique<Ragged>();
+    (void)hipGetDevice(&R->device);
+    R->forward = forward ? 1 : 0;
+    R->n.assign(lengths, lengths + count);
+    R->in_off.assign(in_off, in_off + count);
+    R->out_off.assign(out_off, out_off + count);
+    std::vector<std::pair<double, int>> cost(count);
+    for (size_t i = 0; i < count; ++i) {
+        const long long n = (long long)lengths[i];
+        if (n < 1 || n > (1LL << 28)) { err = {E_INVALID, "ragged lengths must be in [1, 2^28]"}; return E_INVALID; }
+        const long long M = (n % 2 == 0) ? n / 2 : n;
+        const long long L = rg_smooth13(M) ? M : rg_good7(2 * M - 1);
+        const double c = (rg_smooth13(M) ? 1.0 : 3.0) * (double)L * std::log2((double)L + 1.0);
+        cost[i] = {-c, (int)i};
+        if (n > 1) R->flops += 2.5 * (double)n * std::log2((double)n);
+        R->in_elems = std::max<size_t>(R->in_elems, in_off[i] + (size_t)n);
+        R->out_elems = std::max<size_t>(R->out_elems, out_off[i] + (size_t)n / 2 + 1);
+    }
+    std::stable_sort(cost.begin(), cost.end());
+    R->order.resize(count);
+    for (size_t q = 0; q < count; ++q) R->order[q] = cost[q].second;
+    if (R->dtasks.alloc(count * sizeof(RaggedTask)) || R->counter.alloc(256)) {
+        err = {E_NOMEM, "ragged workspace allocation failed"};
+        return E_NOMEM;
+    }
+    if (hipFuncSetAttribute((const void *)k_ragged, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(8192 * sizeof(double2))) != hipSuccess) {
+        (void)hipGetLastError();
+        err = {E_HIP, "cannot raise the LDS limit of the ragged kernel"};
+        return E_HIP;
+    }
+    rc = build_tasks(*R, 1024, 8192, err);
+    if (rc != OK) return rc;
+    *out = R.release();
+    return OK;
+}
+
+void ragged_destroy(Ragged *r) { delete r; }
+
+int ragged_configure(Ragged &R, int threads, int lds_max, int order, Error &err)
+{
+    const bool pow2 = lds_max > 0 && (lds_max & (lds_max - 1)) == 0;
+    if ((threads != 256 && threads != 512 && threads != 1024) || !pow2 || lds_max < 1024 || lds_max > 8192 ||
+        lds_max > kPointsPerThread * threads || (order != 0 && order != 1)) {
+        err = {E_INVALID, "ragged configuration out of range (threads 256/512/1024; lds_points a power of two in "
+                          "[1024, 8192] and at most 8 per thread; radix order 0/1)"};
+        return E_INVALID;
+    }
+    R.rorder = order;
+    if (threads == R.threads && lds_max == R.lds_max && !R.tasks.empty()) {
+        R.drop_graph();
+        return OK;
+    }
+    return build_tasks(R, threads, lds_max, err);
+}
+
+static hipError_t ragged_launch(Ragged &R, const float *x, double2 *y, double fct, hipStream_t s)
+{
+    RaggedParams P;
+    P.x = x;
+    P.y = y;
+    P.tasks = (const RaggedTask *)R.dtasks.p;
+    P.ntasks = (int)R.tasks.size();
+    P.counter = (int *)R.counter.p;
+    P.scratch = (double2 *)R.scratch.p;
+    P.slot = R.slot;
+    P.order = R.rorder;
+    P.fct = fct;
+    P.osign = R.forward ? 1.0 : -1.0;
+    hipLaunchKernelGGL(k_ragged_reset, dim3(1), dim3(64), 0, s, (int *)R.counter.p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_ragged, dim3(R.grid), dim3(R.threads), (size_t)R.lds_max * sizeof(double2), s, P);
+    return hipGetLastError();
+}
+
+static int check_ptrs(const Ragged &R, const float *x, double2 *y, Error &err)
+{
+    if (!x || !y) { err = {E_INVALID, "null data pointer"}; return E_INVALID; }
+    const char *xa = (const char *)x, *ya = (const char *)y;
+    const char *xe = xa + R.in_elems * sizeof(float), *ye = ya + R.out_elems * sizeof(double2);
+    if (xa < ye && ya < xe) { err = {E_INVALID, "ragged input and output overlap"}; return E_INVALID; }
+    return OK;
+}
+
+// One hipGraph launch (reset kernel + persistent kernel) per execute.  The
+// graph is captured on first use and re-captured when the pointers, the scale
+// or the configuration change.
+int ragged_execute(Ragged &R, const float *x, double2 *y, double fct, hipStream_t s, bool graph, Error &err)
+{
+    int rc = check_ptrs(R, x, y, err);
+    if (rc != OK) return rc;
+    if (!graph) {
+        hipError_t e = ragged_launch(R, x, y, fct, s);
+        if (e != hipSuccess) { err = {E_HIP, std::string("ragged launch: ") + hipGetErrorString(e)}; return E_HIP; }
+        return OK;
+    }
+    if (!R.exec || R.gx != x || R.gy != y || R.gfct != fct) {
+        R.drop_graph();
+        hipStream_t cs;
+        if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+            (void)hipGetLastError();
+            err = {E_HIP, "stream creation failed"};
+            return E_HIP;
+        }
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            hipError_t el = ragged_launch(R, x, y, fct, cs);
+            hipError_t ee = hipStreamEndCapture(cs, &R.graph);
+            e = el != hipSuccess ? el : ee;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&R.exec, R.graph, nullptr, nullptr, 0);
+        (void)hipStreamDestroy(cs);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            R.drop_graph();
+            err = {E_HIP, std::string("ragged graph capture: ") + hipGetErrorString(e)};
+            return E_HIP;
+        }
+        R.gx = x;
+        R.gy = y;
+        R.gfct = fct;
+    }
+    hipError_t e = hipGraphLaunch(R.exec, s);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        err = {E_HIP, std::string("ragged graph launch: ") + hipGetErrorString(e)};
+        return E_HIP;
+    }
+    return OK;
+}
+
+// Autotuner: time every configuration that can plan the whole batch on the
+// caller's buffers and keep the fastest.
+int ragged_tune(Ragged &R, const float *x, double2 *y, hipStream_t s, int reps, double *best_ms, Error &err)
+{
+    int rc = check_ptrs(R, x, y, err);
+    if (rc != OK) return rc;
+    struct Cfg { int t, l, o; };
+    const Cfg cands[] = {{1024, 8192, 0}, {1024, 8192, 1}, {1024, 4096, 0}, {512, 4096, 0}, {512, 4096, 1},
+                         {512, 2048, 0}, {256, 2048, 0}, {256, 2048, 1}, {256, 1024, 0}};
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+        (void)hipGetLastError();
+        if (a) (void)hipEventDestroy(a);
+        err = {E_HIP, "event creation failed"};
+        return E_HIP;
+    }
+    double best = 1e300;
+    Cfg bestc = {R.threads, R.lds_max, R.rorder};
+    reps = std::max(1, reps);
+    rc = OK;
+    for (const Cfg &c : cands) {
+        Error e2{0, ""};
+        if (ragged_configure(R, c.t, c.l, c.o, e2) != OK) {
+            if (e2.code == E_UNSUPPORTED) continue;   // this batch has a length the configuration cannot split
+            err = e2;
+            rc = e2.code;
+            break;
+        }
+        if ((rc = ragged_execute(R, x, y, 1.0, s, false, err)) != OK) break;   // warm-up
+        (void)hipEventRecord(a, s);
+        for (int r = 0; r < reps && rc == OK; ++r) rc = ragged_execute(R, x, y, 1.0, s, false, err);
+        if (rc != OK) break;
+        (void)hipEventRecord(b, s);
+        float ms = 0.f;
+        if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+            (void)hipGetLastError();
+            err = {E_HIP, "autotuner timing failed"};
+            rc = E_HIP;
+            break;
+        }
+        const double per = ms / reps;
+        if (per < best) { best = per; bestc = c; }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc != OK) return rc;
+    if (best_ms) *best_ms = best;
+    return ragged_configure(R, bestc.t, bestc.l, bestc.o, err);
+}
+
+void ragged_info(const Ragged &R, int *threads, int *lds_max, int *order, int *grid, double *flops,
+                 size_t *scratch_bytes)
+{
+    if (threads) *threads = R.threads;
+    if (lds_max) *lds_max = R.lds_max;
+    if (order) *order = R.rorder;
+    if (grid) *grid = R.grid;
+    if (flops) *flops = R.flops;
+    if (scratch_bytes) *scratch_bytes = R.scratch.bytes;
+}
+
+}  // namespace ifft

@@ -162,6 +162,43 @@ int impulse_fft_conv2d_grid(impulse_fft_conv2d conv, size_t *P, size_t *Q);
 int impulse_fft_conv2d_destroy(impulse_fft_conv2d conv);
 
 /* ------------------------------------------------------------------------- */
+/* Ragged batches of real transforms (no PocketFFT counterpart)               */
+/* ------------------------------------------------------------------------- */
+
+/* `count` float32 signals of lengths[i] (any length >= 1) at in_offsets[i]
+ * floats into one input buffer; each gets its lengths[i]/2+1 complex128 bins
+ * at out_offsets[i] elements into one output buffer -- the same bins as
+ * impulse_fft_r2c on the signal widened to float64 (forward = 0 conjugates
+ * them, as PocketFFT's r2c does), scaled by `fct`.  Arithmetic is FP64.
+ * Nearest reference: one r2c call per signal, cpp_pocketfft/pocketfft.nim:235-277
+ * over pocketfft_hdronly.h:3320-3350, each with its own plan lookup
+ * (pocketfft_hdronly.h:2655-2706).
+ *
+ * One execute is one hipGraph launch (a queue reset and one persistent
+ * kernel in which each workgroup claims signals, longest first).  The graph is
+ * captured at the first execute for given (in, out, fct) and replayed after.
+ * Buffers are DEVICE pointers and must not overlap. */
+typedef struct impulse_fft_ragged_s *impulse_fft_ragged;
+int impulse_fft_ragged_create(impulse_fft_ragged *batch, size_t count, const size_t *lengths,
+                              const size_t *in_offsets, const size_t *out_offsets, int forward);
+int impulse_fft_ragged_execute(impulse_fft_ragged batch, const float *in, void *out, double fct, void *stream);
+/* Autotuner: runs every candidate configuration (workgroup size, LDS budget
+ * per transform, radix order) `reps` times on these buffers, keeps the
+ * fastest and reports its time per execute (ms).  Synchronizes `stream`;
+ * `out` is overwritten. */
+int impulse_fft_ragged_tune(impulse_fft_ragged batch, const float *in, void *out, void *stream, int reps,
+                            double *best_ms);
+/* Pick a configuration by hand: threads 256/512/1024, lds_points
+ * 2048/4096/8192 (longest transform kept in LDS), radix_order 0 (powers of
+ * two first) or 1 (odd radices first). */
+int impulse_fft_ragged_configure(impulse_fft_ragged batch, int threads, int lds_points, int radix_order);
+/* Current configuration, persistent grid size, nominal flops of one execute
+ * (2.5 n log2 n per signal) and device scratch bytes.  Any pointer may be NULL. */
+int impulse_fft_ragged_info(impulse_fft_ragged batch, int *threads, int *lds_points, int *radix_order, int *grid,
+                            double *flops, size_t *scratch_bytes);
+int impulse_fft_ragged_destroy(impulse_fft_ragged batch);
+
+/* ------------------------------------------------------------------------- */
 /* 1-D in-place plans on host memory (C path)                                 */
 /* ------------------------------------------------------------------------- */
 

@@ -82,3 +82,43 @@ def test_datadesc_strides():
     assert d.stride == [80, 16]
     with pytest.raises(ValueError):
         F.DataDesc(a, (3, 5), (1,))
+
+
+def test_uniform_device_matches_numpy():
+    torch = pytest.importorskip("torch")
+    a = synth.uniform(5000, -1.0, 1.0, seed=7)
+    b = synth.uniform_device(5000, -1.0, 1.0, seed=7, device="cpu", chunk=777).numpy()
+    assert np.array_equal(a, b)
+    a32 = synth.uniform(999, -1.0, 1.0, seed=0).astype(np.float32)
+    b32 = synth.uniform_device(999, -1.0, 1.0, seed=0, dtype=torch.float32, device="cpu").numpy()
+    assert np.array_equal(a32, b32)
+
+
+def _largest_prime_factor(n):
+    best, p = 1, 2
+    while p * p <= n:
+        while n % p == 0:
+            best, n = p, n // p
+        p += 1
+    return max(best, n) if n > 1 else best
+
+
+def test_ragged_lengths_workload():
+    L = synth.ragged_lengths(4000, seed=0)
+    assert L.min() >= 64 and L.max() <= 262144
+    smooth = np.array([_largest_prime_factor(int(n)) <= 7 for n in L])
+    big = np.array([_largest_prime_factor(int(n)) > 1000 for n in L])
+    assert np.all(smooth | big)
+    assert abs(smooth.mean() - 0.7) < 0.03
+    # log-uniform: the smooth class's log-lengths spread evenly over [log 64, log 262144]
+    lg = np.log(L[smooth])
+    q = np.quantile(lg, [0.25, 0.5, 0.75])
+    span = np.log(262144) - np.log(64)
+    assert np.allclose(q, np.log(64) + span * np.array([0.25, 0.5, 0.75]), atol=0.05 * span)
+    assert np.array_equal(L, synth.ragged_lengths(4000, seed=0))
+
+
+def test_ragged_packed_offsets():
+    from impulse_amd import ragged
+    ins, outs = ragged.packed_offsets([5, 8, 1])
+    assert list(ins) == [0, 5, 13] and list(outs) == [0, 3, 8]

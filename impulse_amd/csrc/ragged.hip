@@ -11,10 +11,14 @@
 //   M 13-smooth, longer              : four-step M = L1 L2 through the
 //                                      workgroup's global scratch (batches of
 //                                      columns, then batches of rows, in LDS);
-//   otherwise                        : Bluestein with a 7-smooth n2 >= 2M-1;
-//                                      the same workgroup computes the chirp
-//                                      kernel's spectrum (three transforms of
-//                                      length n2).
+//   otherwise                        : Bluestein with a 7-smooth n2 >= M+K-1,
+//                                      where K = M outputs are needed for even n
+//                                      and K = (n+1)/2 for odd n; the chirp
+//                                      kernel's spectrum is computed once per
+//                                      distinct length when the batch is
+//                                      created and kept in a device table, so an
+//                                      execute runs two transforms of length n2
+//                                      per Bluestein signal.
 // Twiddles and the chirp exp(-i pi j^2 / M) are generated in FP64 with
 // sincospi (the chirp index j^2 mod 2M in 64-bit integers).  Phases that hand
 // data to other threads of the workgroup through global scratch are separated
@@ -45,6 +49,7 @@ struct RaggedTask {
     int L1, L2;          // four-step split of L (0: L stays in LDS)
     int ca, cb;          // columns per LDS batch in pass A (length L2) / rows in pass B (length L1)
     int blue;            // Bluestein
+    long long spec_off;  // Bluestein: first point of the chirp kernel's spectrum in the table
 };
 
 struct RaggedParams {
@@ -53,8 +58,10 @@ struct RaggedParams {
     const RaggedTask *tasks;
     int ntasks;
     int *counter;
-    double2 *scratch;          // per workgroup: 3 buffers of `slot` complex points
+    double2 *scratch;          // per workgroup: 2 buffers of `slot` complex points
     long long slot;
+    double2 *spec;             // table of chirp-kernel spectra (unnormalized)
+    int mode;                  // 0: transform the signals; 1: fill the spectrum table
     int order;                 // radix order of LDS stages: 0 powers of two first, 1 odd radices first
     double fct;                // output scale
     double osign;              // +1: forward bins; -1: conjugated (backward sign)
@@ -132,14 +139,18 @@ __device__ __forceinline__ double2 twid(long long m, long long L, double sign)
     return make_double2(c, sign * s);
 }
 
-// chirp w[j] = exp(-i pi j^2 / M)
-__device__ __forceinline__ double2 chirp(long long j, long long M)
+// chirp w[j] = exp(-i pi j^2 / M) for 0 <= j < M < 2^28.  j^2 mod 2M from a
+// quotient estimate in FP64 (off by at most one, since j^2 / 2M < 2^27) and
+// exact 64-bit products; inv2m = 1 / (2M).
+__device__ __forceinline__ double2 chirp(long long j, long long M, double inv2m)
 {
-    const unsigned long long m2 = 2ull * (unsigned long long)M;
-    const unsigned long long jj = (unsigned long long)j % m2;
-    const long long idx = (long long)((jj * jj) % m2);
+    const long long m2 = 2 * M;
+    const long long sq = j * j;
+    long long r = sq - (long long)((double)sq * inv2m) * m2;
+    if (r < 0) r += m2;
+    if (r >= m2) r -= m2;
     double s, c;
-    sincospi((double)idx / (double)M, &s, &c);
+    sincospi((double)r / (double)M, &s, &c);
     return make_double2(c, -s);
 }
 
@@ -255,11 +266,11 @@ __device__ __forceinline__ void wg_sync_global()
 
 // Complex transform of length T.L (sign s) by the workgroup; load(m) gives
 // input point m.  Returns true with the result in LDS (natural order), or
-// false with the result in `out` (global).  `tmp` is global scratch of L
-// points for the four-step split.
+// false with the result in `out` (global; times mul[] pointwise when `mul` is
+// given).  `tmp` is global scratch of L points for the four-step split.
 template <class LOAD>
 __device__ bool wg_fft(LOAD load, const RaggedTask &T, double s, double2 *lds, double2 *tmp, double2 *out,
-                       int order)
+                       int order, const double2 *mul = nullptr)
 {
     const int tid = threadIdx.x, nthr = blockDim.x;
     const long long L = T.L;
@@ -304,7 +315,10 @@ __device__ bool wg_fft(LOAD load, const RaggedTask &T, double s, double2 *lds, d
         lds_fft(lds, L1, cb, s, order);
         for (int e = tid; e < cb * L1; e += nthr) {
             const int t = e % cb, k1 = e / cb;
-            if (t < cnt) out[(long long)(r0 + t) + (long long)L2 * k1] = lds[t * L1 + k1];
+            if (t < cnt) {
+                const long long o = (long long)(r0 + t) + (long long)L2 * k1;
+                out[o] = mul ? cmul(lds[t * L1 + k1], mul[o]) : lds[t * L1 + k1];
+            }
         }
         __syncthreads();
     }
@@ -317,14 +331,16 @@ __global__ void k_ragged_reset(int *counter)
     if (threadIdx.x == 0) counter[0] = 0;
 }
 
-__global__ void __launch_bounds__(1024) k_ragged(const RaggedParams P)
+// One instance per workgroup size: a bound of 1024 threads leaves 128 VGPRs
+// per lane, 512 and 256 leave more registers at lower occupancy.
+template <int MAXT>
+__global__ void __launch_bounds__(MAXT) k_ragged(const RaggedParams P)
 {
     extern __shared__ double2 lds[];
     __shared__ int s_task;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    double2 *S0 = P.scratch + (long long)blockIdx.x * 3 * P.slot;   // four-step temporary
+    double2 *S0 = P.scratch + (long long)blockIdx.x * 2 * P.slot;   // four-step temporary
     double2 *S1 = S0 + P.slot;                                        // transform results
-    double2 *S2 = S1 + P.slot;                                        // Bluestein kernel spectrum
     for (;;) {
         if (tid == 0) s_task = atomicAdd(P.counter, 1);
         __syncthreads();
@@ -339,44 +355,46 @@ __global__ void __launch_bounds__(1024) k_ragged(const RaggedParams P)
         auto zin = [&](long long m) -> double2 {
             return half ? make_double2((double)x[2 * m], (double)x[2 * m + 1]) : make_double2((double)x[m], 0.0);
         };
+        const double inv2m = 0.5 / (double)M;
         bool in_lds;
         if (!T.blue) {
             in_lds = wg_fft(zin, T, -1.0, lds, S0, S1, P.order);
-        } else {
-            const long long n2 = T.L;
-            const double inv = 1.0 / (double)n2;
-            // kernel spectrum B = FFT(b) / n2, b[m] = conj(w[|m|]) for |m| < M (indices mod n2)
+        } else if (P.mode == 1) {
+            // chirp kernel b[m] = conj(w[m]) for 0 <= m < K, conj(w[n2 - m]) for n2 - m < M
+            // (indices mod n2, zero between); its unnormalized spectrum goes to the table
+            const long long n2 = T.L, K = half ? M : n / 2 + 1;
+            double2 *B = P.spec + T.spec_off;
             auto bload = [&](long long m) -> double2 {
-                if (m < M) return cconj(chirp(m, M));
-                if (m > n2 - M) return cconj(chirp(n2 - m, M));
+                if (m < K) return cconj(chirp(m, M, inv2m));
+                if (m > n2 - M) return cconj(chirp(n2 - m, M, inv2m));
                 return make_double2(0.0, 0.0);
             };
-            if (wg_fft(bload, T, -1.0, lds, S0, S2, P.order)) {
-                for (long long m = tid; m < n2; m += nthr) S2[m] = cscale(lds[m], inv);
-            } else {
-                for (long long m = tid; m < n2; m += nthr) S2[m] = cscale(S2[m], inv);
-            }
-            wg_sync_global();
-            // A = FFT(z w); C = A B; c = IFFT(C) (unnormalized; B carries 1/n2)
+            if (wg_fft(bload, T, -1.0, lds, S0, B, P.order))
+                for (long long m = tid; m < n2; m += nthr) B[m] = lds[m];
+            __syncthreads();
+            continue;
+        } else {
+            // A = FFT(z w); C = A B; c = IFFT(C) (unnormalized; 1/n2 applied with the output chirp)
+            const long long n2 = T.L;
+            const double2 *B = P.spec + T.spec_off;
             auto aload = [&](long long m) -> double2 {
-                return m < M ? cmul(zin(m), chirp(m, M)) : make_double2(0.0, 0.0);
+                return m < M ? cmul(zin(m), chirp(m, M, inv2m)) : make_double2(0.0, 0.0);
             };
-            if (wg_fft(aload, T, -1.0, lds, S0, S1, P.order)) {
-                for (long long m = tid; m < n2; m += nthr) lds[m] = cmul(lds[m], S2[m]);
+            if (wg_fft(aload, T, -1.0, lds, S0, S1, P.order, B)) {
+                for (long long m = tid; m < n2; m += nthr) lds[m] = cmul(lds[m], B[m]);
                 __syncthreads();
                 lds_fft(lds, (int)n2, 1, 1.0, P.order);
                 in_lds = true;
             } else {
-                for (long long m = tid; m < n2; m += nthr) S1[m] = cmul(S1[m], S2[m]);
-                wg_sync_global();
                 auto cload = [&](long long m) -> double2 { return S1[m]; };
                 in_lds = wg_fft(cload, T, 1.0, lds, S0, S1, P.order);
             }
         }
-        // Z[k] = FFT_M(z)[k] (Bluestein: times the chirp)
+        // Z[k] = FFT_M(z)[k], k < K (Bluestein: c[k] w[k] / n2)
+        const double bscale = T.blue ? 1.0 / (double)T.L : 1.0;
         auto Z = [&](long long k) -> double2 {
             const double2 v = in_lds ? lds[k] : S1[k];
-            return T.blue ? cmul(v, chirp(k, M)) : v;
+            return T.blue ? cscale(cmul(v, chirp(k, M, inv2m)), bscale) : v;
         };
         const double f = P.fct, fo = P.fct * P.osign;
         if (half) {
@@ -385,7 +403,7 @@ __global__ void __launch_bounds__(1024) k_ragged(const RaggedParams P)
             for (long long k = tid; k <= M / 2; k += nthr) {
                 const long long km = (M - k) % M;
                 const double2 zk = Z(k), zm = Z(km);
-                const double2 w = twid(k, n, -1.0);
+                const double2 w = twid32((int)k, (int)n, -1.0);
                 const double2 e1 = cscale(cadd(zk, cconj(zm)), 0.5), d1 = csub(zk, cconj(zm));
                 const double2 o1 = make_double2(0.5 * d1.y, -0.5 * d1.x);
                 const double2 x1 = cadd(e1, cmul(o1, w));
@@ -423,6 +441,9 @@ struct Ragged {
     long long slot = 0;
     int grid = 0;
     DevBuf dtasks, counter, scratch;
+    // chirp-kernel spectra, one per distinct Bluestein length
+    std::vector<std::pair<long long, long long>> spec_index;   // (n, offset), sorted by n
+    DevBuf spec;
     double flops = 0.0;
     size_t in_elems = 0, out_elems = 0;
     // captured graph for (x, y, fct)
@@ -441,14 +462,38 @@ struct Ragged {
     ~Ragged() { drop_graph(); }
 };
 
-// plan one signal for (threads, lds_max); false if no split fits
+static const void *ragged_fn(int threads)
+{
+    return threads == 256 ? (const void *)k_ragged<256>
+                          : threads == 512 ? (const void *)k_ragged<512> : (const void *)k_ragged<1024>;
+}
+
+static hipError_t launch_ragged(const RaggedParams &P, int threads, unsigned grid, size_t lds, hipStream_t s)
+{
+    if (threads == 256) hipLaunchKernelGGL(k_ragged<256>, dim3(grid), dim3(256), lds, s, P);
+    else if (threads == 512) hipLaunchKernelGGL(k_ragged<512>, dim3(grid), dim3(512), lds, s, P);
+    else hipLaunchKernelGGL(k_ragged<1024>, dim3(grid), dim3(1024), lds, s, P);
+    return hipGetLastError();
+}
+
+// transform length of a signal: M, or the Bluestein length n2 >= M + K - 1
+static long long transform_len(long long n)
+{
+    const long long M = (n % 2 == 0) ? n / 2 : n;
+    if (rg_smooth13(M)) return M;
+    const long long K = (n % 2 == 0) ? M : n / 2 + 1;
+    return rg_good7(M + K - 1);
+}
+
+// plan one signal for an LDS budget of lds_max points; false if no split fits
 static bool plan_task(long long n, int lds_max, RaggedTask &t)
 {
     const long long M = (n % 2 == 0) ? n / 2 : n;
     t.n = (int)n;
     t.M = (int)M;
     t.blue = rg_smooth13(M) ? 0 : 1;
-    const long long L = t.blue ? rg_good7(2 * M - 1) : M;
+    t.spec_off = 0;
+    const long long L = transform_len(n);
     if (L > (1LL << 30)) return false;
     t.L = (int)L;
     t.L1 = t.L2 = t.ca = t.cb = 0;
@@ -484,19 +529,23 @@ static int build_tasks(Ragged &R, int threads, int lds_max, Error &err)
         }
         t.in_off = (long long)R.in_off[i];
         t.out_off = (long long)R.out_off[i];
+        if (t.blue)
+            t.spec_off = std::lower_bound(R.spec_index.begin(), R.spec_index.end(),
+                                          std::make_pair((long long)R.n[i], -1LL))->second;
         if (t.L1 || t.blue) slot = std::max<long long>(slot, t.L);
     }
     int ncu = 0, per = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, R.device) != hipSuccess || ncu <= 0)
         ncu = 256;
     const size_t lds = (size_t)lds_max * sizeof(double2);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ragged, threads, lds) != hipSuccess || per <= 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, ragged_fn(threads), threads, lds) != hipSuccess ||
+        per <= 0) {
         (void)hipGetLastError();
         per = 1;
     }
     // persistent grid: every resident workgroup, with the per-workgroup scratch
     // held to kScratchCap in total (fewer resident workgroups past that)
-    const long long per_wg = 3 * slot * (long long)sizeof(double2);
+    const long long per_wg = 2 * slot * (long long)sizeof(double2);
     long long grid_ll = std::min<long long>((long long)ncu * per, (long long)tasks.size());
     grid_ll = std::max<long long>(1, std::min<long long>(grid_ll, std::max<long long>(ncu, kScratchCap / per_wg)));
     const int grid = (int)grid_ll;
@@ -520,6 +569,72 @@ static int build_tasks(Ragged &R, int threads, int lds_max, Error &err)
     return OK;
 }
 
+// Spectrum table: one entry per distinct Bluestein length, filled by the
+// persistent kernel in spectrum mode.
+static int build_spectra(Ragged &R, Error &err)
+{
+    std::vector<long long> ns;
+    for (size_t i = 0; i < R.n.size(); ++i) {
+        const long long n = (long long)R.n[i];
+        const long long M = (n % 2 == 0) ? n / 2 : n;
+        if (!rg_smooth13(M)) ns.push_back(n);
+    }
+    std::sort(ns.begin(), ns.end());
+    ns.erase(std::unique(ns.begin(), ns.end()), ns.end());
+    R.spec_index.clear();
+    if (ns.empty()) return OK;
+    std::vector<RaggedTask> st(ns.size());
+    long long off = 0, slot = 1;
+    for (size_t q = 0; q < ns.size(); ++q) {
+        if (!plan_task(ns[q], 8192, st[q])) {
+            err = {E_UNSUPPORTED, "no four-step split for the Bluestein length of a signal of length " +
+                                      std::to_string(ns[q])};
+            return E_UNSUPPORTED;
+        }
+        st[q].in_off = st[q].out_off = 0;
+        st[q].spec_off = off;
+        R.spec_index.push_back({ns[q], off});
+        off += st[q].L;
+        if (st[q].L1) slot = std::max<long long>(slot, st[q].L);
+    }
+    // longest first
+    std::stable_sort(st.begin(), st.end(), [](const RaggedTask &a, const RaggedTask &b) { return a.L > b.L; });
+    if (R.spec.alloc((size_t)off * sizeof(double2))) { err = {E_NOMEM, "spectrum table allocation failed"}; return E_NOMEM; }
+    DevBuf dt, ws;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, R.device) != hipSuccess || ncu <= 0)
+        ncu = 256;
+    const int grid = (int)std::min<long long>(ncu, (long long)st.size());
+    if (dt.alloc(st.size() * sizeof(RaggedTask)) || ws.alloc((size_t)grid * 2 * (size_t)slot * sizeof(double2))) {
+        err = {E_NOMEM, "spectrum workspace allocation failed"};
+        return E_NOMEM;
+    }
+    RaggedParams P{};
+    P.tasks = (const RaggedTask *)dt.p;
+    P.ntasks = (int)st.size();
+    P.counter = (int *)R.counter.p;
+    P.scratch = (double2 *)ws.p;
+    P.slot = slot;
+    P.spec = (double2 *)R.spec.p;
+    P.mode = 1;
+    P.order = 0;
+    P.fct = 1.0;
+    P.osign = 1.0;
+    hipError_t e = hipMemcpy(dt.p, st.data(), st.size() * sizeof(RaggedTask), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_ragged_reset, dim3(1), dim3(64), 0, 0, (int *)R.counter.p);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = launch_ragged(P, 1024, grid, (size_t)8192 * sizeof(double2), 0);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        err = {E_HIP, std::string("spectrum table: ") + hipGetErrorString(e)};
+        return E_HIP;
+    }
+    return OK;
+}
+
 int ragged_create(Ragged **out, size_t count, const size_t *lengths, const size_t *in_off, const size_t *out_off,
                   int forward, Error &err)
 {
@@ -540,8 +655,8 @@ int ragged_create(Ragged **out, size_t count, const size_t *lengths, const size_
         const long long n = (long long)lengths[i];
         if (n < 1 || n > (1LL << 28)) { err = {E_INVALID, "ragged lengths must be in [1, 2^28]"}; return E_INVALID; }
         const long long M = (n % 2 == 0) ? n / 2 : n;
-        const long long L = rg_smooth13(M) ? M : rg_good7(2 * M - 1);
-        const double c = (rg_smooth13(M) ? 1.0 : 3.0) * (double)L * std::log2((double)L + 1.0);
+        const long long L = transform_len(n);
+        const double c = (rg_smooth13(M) ? 1.0 : 2.0) * (double)L * std::log2((double)L + 1.0);
         cost[i] = {-c, (int)i};
         if (n > 1) R->flops += 2.5 * (double)n * std::log2((double)n);
         R->in_elems = std::max<size_t>(R->in_elems, in_off[i] + (size_t)n);
@@ -554,12 +669,15 @@ int ragged_create(Ragged **out, size_t count, const size_t *lengths, const size_
         err = {E_NOMEM, "ragged workspace allocation failed"};
         return E_NOMEM;
     }
-    if (hipFuncSetAttribute((const void *)k_ragged, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)(8192 * sizeof(double2))) != hipSuccess) {
-        (void)hipGetLastError();
-        err = {E_HIP, "cannot raise the LDS limit of the ragged kernel"};
-        return E_HIP;
-    }
+    for (int t : {256, 512, 1024})
+        if (hipFuncSetAttribute(ragged_fn(t), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(8192 * sizeof(double2))) != hipSuccess) {
+            (void)hipGetLastError();
+            err = {E_HIP, "cannot raise the LDS limit of the ragged kernel"};
+            return E_HIP;
+        }
+    rc = build_spectra(*R, err);
+    if (rc != OK) return rc;
     rc = build_tasks(*R, 1024, 8192, err);
     if (rc != OK) return rc;
     *out = R.release();
@@ -595,14 +713,15 @@ static hipError_t ragged_launch(Ragged &R, const float *x, double2 *y, double fc
     P.counter = (int *)R.counter.p;
     P.scratch = (double2 *)R.scratch.p;
     P.slot = R.slot;
+    P.spec = (double2 *)R.spec.p;
+    P.mode = 0;
     P.order = R.rorder;
     P.fct = fct;
     P.osign = R.forward ? 1.0 : -1.0;
     hipLaunchKernelGGL(k_ragged_reset, dim3(1), dim3(64), 0, s, (int *)R.counter.p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_ragged, dim3(R.grid), dim3(R.threads), (size_t)R.lds_max * sizeof(double2), s, P);
-    return hipGetLastError();
+    return launch_ragged(P, R.threads, (unsigned)R.grid, (size_t)R.lds_max * sizeof(double2), s);
 }
 
 static int check_ptrs(const Ragged &R, const float *x, double2 *y, Error &err)

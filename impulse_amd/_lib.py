@@ -27,10 +27,13 @@ EXPORTS = (
     "impulse_fft_conv2d_grid", "impulse_fft_conv2d_destroy",
     "impulse_fft_ragged_create", "impulse_fft_ragged_execute", "impulse_fft_ragged_tune",
     "impulse_fft_ragged_configure", "impulse_fft_ragged_info", "impulse_fft_ragged_destroy",
+    "impulse_fft_mfma_create", "impulse_fft_mfma_execute", "impulse_fft_mfma_workspace_bytes",
+    "impulse_fft_mfma_destroy",
 )
 
 C2C, R2C, C2R, R2C_ADJOINT, C2R_ADJOINT = 0, 1, 2, 3, 4
 F64, F32 = 0, 1
+MFMA_F32, MFMA_BF16X = 0, 1
 
 _szp = ctypes.POINTER(ctypes.c_size_t)
 _pdp = ctypes.POINTER(ctypes.c_ssize_t)
@@ -76,6 +79,10 @@ def _sig(L):
     ip = ctypes.POINTER(i)
     L.impulse_fft_ragged_info.argtypes = [_vp, ip, ip, ip, ip, ctypes.POINTER(d), _szp]
     L.impulse_fft_ragged_destroy.argtypes = [_vp]
+    L.impulse_fft_mfma_create.argtypes = [ctypes.POINTER(_vp), sz, sz, i]
+    L.impulse_fft_mfma_execute.argtypes = [_vp, _vp, _vp, i, d, _vp]
+    L.impulse_fft_mfma_workspace_bytes.argtypes = [_vp, _szp]
+    L.impulse_fft_mfma_destroy.argtypes = [_vp]
     L.impulse_fft_status_string.argtypes = [i]
     L.impulse_fft_status_string.restype = ctypes.c_char_p
     L.impulse_fft_last_error.restype = ctypes.c_char_p

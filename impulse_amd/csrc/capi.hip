@@ -25,6 +25,11 @@ struct impulse_fft_ragged_s {
     ~impulse_fft_ragged_s() { ifft::ragged_destroy(r); }
 };
 
+struct impulse_fft_mfma_s {
+    ifft::Mfma *m = nullptr;
+    ~impulse_fft_mfma_s() { ifft::mfma_destroy(m); }
+};
+
 struct impulse_cfft_plan_s {
     size_t length = 0;
     std::unique_ptr<ifft::Plan> fwd, bwd;
@@ -401,6 +406,47 @@ int impulse_fft_ragged_info(impulse_fft_ragged batch, int *threads, int *lds_poi
 int impulse_fft_ragged_destroy(impulse_fft_ragged batch)
 {
     delete batch;
+    return ifft::OK;
+}
+
+// ---- matrix-core FFTs ---------------------------------------------------------
+
+int impulse_fft_mfma_create(impulse_fft_mfma *plan, size_t length, size_t batch, int precision)
+{
+    if (!plan) return fail(ifft::E_INVALID, "plan pointer is NULL");
+    *plan = nullptr;
+    ifft::Error err{0, ""};
+    ifft::Mfma *m = nullptr;
+    int rc = ifft::mfma_create(&m, length, batch, precision, err);
+    if (rc != ifft::OK) return fail(err);
+    auto *h = new (std::nothrow) impulse_fft_mfma_s;
+    if (!h) {
+        ifft::mfma_destroy(m);
+        return fail(ifft::E_NOMEM, "out of host memory");
+    }
+    h->m = m;
+    *plan = h;
+    return ifft::OK;
+}
+
+int impulse_fft_mfma_execute(impulse_fft_mfma plan, const void *in, void *out, int forward, double fct, void *stream)
+{
+    if (!plan || !plan->m) return fail(ifft::E_INVALID, "plan is NULL");
+    ifft::Error err{0, ""};
+    int rc = ifft::mfma_execute(*plan->m, in, out, forward, fct, (hipStream_t)stream, err);
+    return rc == ifft::OK ? rc : fail(err);
+}
+
+int impulse_fft_mfma_workspace_bytes(impulse_fft_mfma plan, size_t *bytes)
+{
+    if (!plan || !plan->m || !bytes) return fail(ifft::E_INVALID, "NULL argument");
+    *bytes = ifft::mfma_workspace_bytes(*plan->m);
+    return ifft::OK;
+}
+
+int impulse_fft_mfma_destroy(impulse_fft_mfma plan)
+{
+    delete plan;
     return ifft::OK;
 }
 

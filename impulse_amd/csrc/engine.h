@@ -117,6 +117,13 @@ void ragged_info(const Ragged &R, int *threads, int *lds_max, int *order, int *g
                  size_t *scratch_bytes);
 void ragged_destroy(Ragged *r);
 
+// matrix-core complex FFTs (mfma.hip)
+struct Mfma;
+int mfma_create(Mfma **out, size_t length, size_t batch, int precision, Error &err);
+int mfma_execute(Mfma &m, const void *in, void *out, int forward, double fct, hipStream_t s, Error &err);
+size_t mfma_workspace_bytes(const Mfma &m);
+void mfma_destroy(Mfma *m);
+
 // host FP64 twiddles (twiddle.hip)
 void twiddle_table(std::vector<double2> &out, uint64_t n, uint64_t count, uint64_t step);
 double2 twiddle_at(uint64_t k, uint64_t n);   // exp(-2 pi i k / n), exact at multiples of pi/4

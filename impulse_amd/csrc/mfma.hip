@@ -1,0 +1,469 @@
+// Matrix-core (MFMA) DFT stages for complex FFTs of length 256 and 65536.
+//
+// A 256-point transform is two radix-16 stages, and each stage is one
+// complex 16x16x16 matrix product on a wavefront.  With the input viewed as
+// M[q][b] = x[b + 16 q]:
+//   stage 1   Y1^T = M^T F16                  (DFT over q for each b)
+//   twiddle   T[k1][b] = Y1[k1][b] W256^(b k1)
+//   stage 2   Y2 = T F16                      X[k1 + 16 k2] = Y2[k1][k2]
+// For v_mfma_f32_16x16x* lane l holds row (l mod 16) of the A operand and
+// column (l mod 16) of B, and the result D[4 (l/16) + r][l mod 16] in its
+// accumulator r.  The summation index k may be assigned to (l/16, element)
+// at will as long as A and B agree, so the stage-1 result a lane holds,
+// Y1[l mod 16][4 (l/16) + r], is already the lane's A operand for stage 2:
+// no data moves between lanes.  A complex product is a real one of depth 32
+// over [Re A, Im A] x [Re B; -Im B] (real part) and [Re A, Im A] x [Im B; Re B]
+// (imaginary part).
+//
+// Two precisions:
+//   F32    FP32 data and v_mfma_f32_16x16x4_f32 (FP32 products and
+//          accumulation); twiddles are FP64-generated and applied as split
+//          FP32 (hi + lo) compensated products.  complex64 in and out.
+//   BF16X  FP64 data.  Before each stage the 256 values of a transform are
+//          cut, at one power-of-two scale 2^E >= max |v|, into S = 5 slices of
+//          8-bit integers (bf16-exact); the DFT matrix is cut the same way.
+//          Every slice product is an integer below 2^14 and the 32-term MFMA
+//          sums of all slice pairs with the same weight 2^-8(a+f) stay below
+//          2^22, so v_mfma_f32_16x16x32_bf16 accumulates them exactly in FP32.
+//          Pairs with a + f <= S - 1 are kept (relative truncation ~2^-40 per
+//          stage); the weighted sums are combined in FP64.  complex64 in,
+//          complex128 out.
+//
+// Lengths: 256 (one pass) and 65536 = 256 x 256 (four-step: column FFTs
+// with the W65536^(t k) twiddle, then row FFTs with a transposed store).  A
+// workgroup of four wavefronts owns a tile of 16 sub-transforms in LDS; each
+// wavefront computes four of them.
+#include <cmath>
+#include <vector>
+
+#include "butterflies.h"
+#include "engine.h"
+
+namespace ifft {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+enum { MF_F32 = 0, MF_BF16X = 1 };
+enum { MP_COL = 0, MP_ROW = 1, MP_SINGLE = 2 };
+constexpr int kSlices = 5;
+constexpr int kTile = 16;           // sub-transforms per workgroup
+constexpr int kRow = 273;           // LDS row stride (elements): 256 + 256/16 padding + 1
+
+struct MfmaParams {
+    const void *src;
+    void *dst;
+    long long batch;                // transforms of length L
+    int L;                          // 256 or 65536
+    double sign;                    // -1 forward, +1 backward
+    double fct;                     // applied by the last pass
+    const double2 *tw_l;            // W_L^m (forward sign), m < L, FP64 (L = 65536)
+    const float4 *tw_l32;           // the same as (re_hi, re_lo, im_hi, im_lo)
+    const double2 *tw256;           // W_256^m, m < 256
+    const float4 *tw256_32;
+};
+
+__device__ __forceinline__ int lidx(int n) { return n + (n >> 4); }
+
+// exp(-2 pi i m / 16) for m < 16 (forward sign), exact to FP64 rounding
+__device__ __forceinline__ double2 w16(int m)
+{
+    constexpr double c1 = 0.92387953251128674, s1 = 0.38268343236508978, h = 0.70710678118654752;
+    const double cs[5] = {1.0, c1, h, s1, 0.0};
+    const int o = m & 15;
+    // cos and sin of 2 pi o / 16 from the first quadrant
+    const int q = o >> 2, r = o & 3;
+    double c = cs[r], s = cs[4 - r];
+    double cr, sr;
+    switch (q) {
+        case 0: cr = c; sr = s; break;
+        case 1: cr = -s; sr = c; break;
+        case 2: cr = -c; sr = -s; break;
+        default: cr = s; sr = -c; break;
+    }
+    return make_double2(cr, -sr);
+}
+
+template <int VAR>
+struct Elem;
+template <>
+struct Elem<MF_F32> {
+    using T = float2;
+};
+template <>
+struct Elem<MF_BF16X> {
+    using T = double2;
+};
+
+// ---------------------------------------------------------------------------
+// FP32 MFMA stage: in/out are the lane's four complex values
+// ---------------------------------------------------------------------------
+
+struct F32Consts {
+    float br[8], bi[8];   // B operand for the real / imaginary accumulators, k-chunk kb = 0..7
+};
+
+__device__ __forceinline__ F32Consts f32_consts(double sign)
+{
+    F32Consts C;
+    const int l = threadIdx.x & 63, g = l >> 4, j = l & 15;
+    for (int e = 0; e < 4; ++e) {
+        const double2 w = w16((4 * g + e) * j);
+        const float fr = (float)w.x, fi = (float)(-sign * w.y);   // F = cos + i sign sin
+        C.br[e] = fr;
+        C.bi[e] = fi;
+        C.br[4 + e] = -fi;
+        C.bi[4 + e] = fr;
+    }
+    return C;
+}
+
+__device__ __forceinline__ void f32_stage(float2 (&a)[4], const F32Consts &C)
+{
+    f4 dr = {0.f, 0.f, 0.f, 0.f}, di = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < 8; ++kb) {
+        const float av = kb < 4 ? a[kb].x : a[kb - 4].y;
+        dr = __builtin_amdgcn_mfma_f32_16x16x4f32(av, C.br[kb], dr, 0, 0, 0);
+        di = __builtin_amdgcn_mfma_f32_16x16x4f32(av, C.bi[kb], di, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) a[r] = make_float2(dr[r], di[r]);
+}
+
+// y * (w_hi + w_lo), w given with the forward sign; conjugated for sign = +1
+__device__ __forceinline__ float2 f32_twiddle(float2 y, float4 w, float sgn)
+{
+    const float wr = w.x, wrl = w.y, wi = -sgn * w.z, wil = -sgn * w.w;
+    float re = fmaf(y.x, wr, -y.y * wi);
+    float im = fmaf(y.x, wi, y.y * wr);
+    re += fmaf(y.x, wrl, -y.y * wil);
+    im += fmaf(y.x, wil, y.y * wrl);
+    return make_float2(re, im);
+}
+
+// ---------------------------------------------------------------------------
+// BF16 slice stage (FP64 data)
+// ---------------------------------------------------------------------------
+
+struct BxConsts {
+    bf8 br[kSlices], bi[kSlices];   // slices of [Re F; -Im F] and [Im F; Re F], scaled by 2^7
+};
+
+__device__ __forceinline__ void cut(double v, double (&s)[kSlices])
+{
+    // v in [-128, 128]: s[0] + s[1]/2^8 + ... , each an integer in [-128, 128]
+#pragma unroll
+    for (int a = 0; a < kSlices; ++a) {
+        const double t = rint(v);
+        s[a] = t;
+        v = (v - t) * 256.0;
+    }
+}
+
+__device__ __forceinline__ BxConsts bx_consts(double sign)
+{
+    BxConsts C;
+    const int l = threadIdx.x & 63, g = l >> 4, j = l & 15;
+    for (int e = 0; e < 4; ++e) {
+        const double2 w = w16((4 * g + e) * j);
+        const double fr = w.x * 128.0, fi = -sign * w.y * 128.0;
+        double sr[kSlices], si[kSlices], sn[kSlices];
+        cut(fr, sr);
+        cut(fi, si);
+        cut(-fi, sn);
+        for (int f = 0; f < kSlices; ++f) {
+            C.br[f][e] = (__bf16)(float)sr[f];
+            C.br[f][4 + e] = (__bf16)(float)sn[f];
+            C.bi[f][e] = (__bf16)(float)si[f];
+            C.bi[f][4 + e] = (__bf16)(float)sr[f];
+        }
+    }
+    return C;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ void bx_stage(double2 (&a)[4], const BxConsts &C)
+{
+    double mx = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) mx = fmax(mx, fmax(fabs(a[r].x), fabs(a[r].y)));
+    mx = wave_max(mx);
+    if (mx == 0.0) return;   // an all-zero transform stays zero
+    int E;
+    (void)frexp(mx, &E);     // mx < 2^E
+    const double up = ldexp(1.0, 7 - E);
+    bf8 A[kSlices];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        double sr[kSlices], si[kSlices];
+        cut(a[r].x * up, sr);
+        cut(a[r].y * up, si);
+#pragma unroll
+        for (int s = 0; s < kSlices; ++s) {
+            A[s][r] = (__bf16)(float)sr[s];
+            A[s][4 + r] = (__bf16)(float)si[s];
+        }
+    }
+    f4 accr[kSlices], acci[kSlices];
+#pragma unroll
+    for (int d = 0; d < kSlices; ++d) {
+        accr[d] = f4{0.f, 0.f, 0.f, 0.f};
+        acci[d] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int s = 0; s < kSlices; ++s)
+#pragma unroll
+        for (int f = 0; f + s < kSlices; ++f) {
+            accr[s + f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[s], C.br[f], accr[s + f], 0, 0, 0);
+            acci[s + f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[s], C.bi[f], acci[s + f], 0, 0, 0);
+        }
+    const double down = ldexp(1.0, E - 14);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int d = kSlices - 1; d >= 0; --d) {
+            re = re * (1.0 / 256.0) + (double)accr[d][r];
+            im = im * (1.0 / 256.0) + (double)acci[d][r];
+        }
+        a[r] = make_double2(re * down, im * down);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the kernel
+// ---------------------------------------------------------------------------
+
+template <int VAR, int PASS>
+__global__ void __launch_bounds__(256) k_mfma256(const MfmaParams P)
+{
+    using E = typename Elem<VAR>::T;
+    using EI = float2;                                  // pass-A / single-pass input: complex64
+    using EO = E;                                       // intermediate and output
+    __shared__ E lds[kTile * kRow];
+    const int tid = threadIdx.x;
+    const long long tile = blockIdx.x;
+    const long long L = P.L;
+    // tile geometry
+    long long base = 0;   // first element of the signal (COL/ROW) or of transform 0 of the tile (SINGLE)
+    int t0 = 0, ntr = kTile;
+    if (PASS == MP_SINGLE) {
+        base = tile * kTile * 256;
+        ntr = (int)min((long long)kTile, P.batch - tile * kTile);
+    } else {
+        base = (tile >> 4) * L;
+        t0 = (int)(tile & 15) * kTile;
+    }
+    // ---- load ----
+    if (PASS == MP_COL) {
+        const EI *x = (const EI *)P.src + base;
+        for (int e = tid; e < kTile * 256; e += 256) {
+            const int c = e & 15, n = e >> 4;
+            const EI v = x[(long long)(t0 + c) + 256LL * n];
+            if constexpr (VAR == MF_F32) lds[c * kRow + lidx(n)] = v;
+            else lds[c * kRow + lidx(n)] = make_double2((double)v.x, (double)v.y);
+        }
+    } else if (PASS == MP_ROW) {
+        const EO *x = (const EO *)P.src + base;
+        for (int e = tid; e < kTile * 256; e += 256) {
+            const int c = e >> 8, n = e & 255;
+            lds[c * kRow + lidx(n)] = x[256LL * (t0 + c) + n];
+        }
+    } else {
+        const EI *x = (const EI *)P.src + base;
+        for (int e = tid; e < kTile * 256; e += 256) {
+            const int c = e >> 8, n = e & 255;
+            const EI v = c < ntr ? x[e] : EI{0.f, 0.f};   // a partial last tile computes zeros
+            if constexpr (VAR == MF_F32) lds[c * kRow + lidx(n)] = v;
+            else lds[c * kRow + lidx(n)] = make_double2((double)v.x, (double)v.y);
+        }
+    }
+    __syncthreads();
+    // ---- compute: wave w owns transforms 4w .. 4w+3 ----
+    const int l = tid & 63, w = tid >> 6, g = l >> 4, j = l & 15;
+    const double sign = P.sign;
+    const bool last = PASS != MP_COL;
+    if constexpr (VAR == MF_F32) {
+        const F32Consts C = f32_consts(sign);
+        const float sg = (float)sign;
+        for (int u = 0; u < 4; ++u) {
+            const int c = 4 * w + u;
+            float2 a[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) a[r] = lds[c * kRow + lidx(j + 16 * (4 * g + r))];
+            f32_stage(a, C);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) a[r] = f32_twiddle(a[r], P.tw256_32[((4 * g + r) * j) & 255], sg);
+            f32_stage(a, C);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int k = 4 * g + r + 16 * j;
+                float2 v = a[r];
+                if (PASS == MP_COL) v = f32_twiddle(v, P.tw_l32[((t0 + c) * k) & 65535], sg);
+                if (last && P.fct != 1.0) v = make_float2((float)(v.x * P.fct), (float)(v.y * P.fct));
+                lds[c * kRow + lidx(k)] = v;
+            }
+        }
+    } else {
+        const BxConsts C = bx_consts(sign);
+        for (int u = 0; u < 4; ++u) {
+            const int c = 4 * w + u;
+            double2 a[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) a[r] = lds[c * kRow + lidx(j + 16 * (4 * g + r))];
+            bx_stage(a, C);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) a[r] = cmul(a[r], tw_sign(P.tw256[((4 * g + r) * j) & 255], sign));
+            bx_stage(a, C);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int k = 4 * g + r + 16 * j;
+                double2 v = a[r];
+                if (PASS == MP_COL) v = cmul(v, tw_sign(P.tw_l[((t0 + c) * k) & 65535], sign));
+                if (last) v = cscale(v, P.fct);
+                lds[c * kRow + lidx(k)] = v;
+            }
+        }
+    }
+    __syncthreads();
+    // ---- store ----
+    EO *y = (EO *)P.dst + base;
+    if (PASS == MP_COL) {
+        for (int e = tid; e < kTile * 256; e += 256) {
+            const int c = e & 15, k = e >> 4;
+            y[(long long)(t0 + c) + 256LL * k] = lds[c * kRow + lidx(k)];
+        }
+    } else if (PASS == MP_ROW) {
+        for (int e = tid; e < kTile * 256; e += 256) {
+            const int c = e & 15, k = e >> 4;
+            y[(long long)(t0 + c) + 256LL * k] = lds[c * kRow + lidx(k)];
+        }
+    } else {
+        for (int e = tid; e < ntr * 256; e += 256) {
+            const int c = e >> 8, k = e & 255;
+            y[e] = lds[c * kRow + lidx(k)];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+struct Mfma {
+    int precision = 0;
+    long long L = 0, batch = 0;
+    DevBuf tw_l, tw_l32, tw256, tw256_32, tmp;
+};
+
+static void split32(const std::vector<double2> &w, std::vector<float4> &o)
+{
+    o.resize(w.size());
+    for (size_t i = 0; i < w.size(); ++i) {
+        const float rh = (float)w[i].x, ih = (float)w[i].y;
+        o[i] = make_float4(rh, (float)(w[i].x - (double)rh), ih, (float)(w[i].y - (double)ih));
+    }
+}
+
+static int upload(DevBuf &b, const void *p, size_t n)
+{
+    if (b.alloc(n)) return E_NOMEM;
+    if (hipMemcpy(b.p, p, n, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        return E_HIP;
+    }
+    return OK;
+}
+
+int mfma_create(Mfma **out, size_t length, size_t batch, int precision, Error &err)
+{
+    if (!out) { err = {E_INVALID, "null handle pointer"}; return E_INVALID; }
+    *out = nullptr;
+    if (length != 256 && length != 65536) {
+        err = {E_UNSUPPORTED, "matrix-core FFT lengths are 256 and 65536"};
+        return E_UNSUPPORTED;
+    }
+    if (batch == 0 || batch > (size_t)1 << 40) { err = {E_INVALID, "batch out of range"}; return E_INVALID; }
+    if (precision != MF_F32 && precision != MF_BF16X) { err = {E_INVALID, "unknown precision"}; return E_INVALID; }
+    int rc = device_check(err);
+    if (rc != OK) return rc;
+    auto M = std::make_unique<Mfma>();
+    M->precision = precision;
+    M->L = (long long)length;
+    M->batch = (long long)batch;
+    std::vector<double2> t256, tl;
+    std::vector<float4> s256, sl;
+    twiddle_table(t256, 256, 256, 1);
+    split32(t256, s256);
+    rc = upload(M->tw256, t256.data(), t256.size() * sizeof(double2));
+    if (rc == OK) rc = upload(M->tw256_32, s256.data(), s256.size() * sizeof(float4));
+    if (rc == OK && length == 65536) {
+        twiddle_table(tl, 65536, 65536, 1);
+        split32(tl, sl);
+        rc = upload(M->tw_l, tl.data(), tl.size() * sizeof(double2));
+        if (rc == OK) rc = upload(M->tw_l32, sl.data(), sl.size() * sizeof(float4));
+        const size_t es = precision == MF_F32 ? sizeof(float2) : sizeof(double2);
+        if (rc == OK && M->tmp.alloc((size_t)length * batch * es)) rc = E_NOMEM;
+    }
+    if (rc != OK) {
+        err = {rc, "matrix-core FFT table or workspace allocation failed"};
+        return rc;
+    }
+    *out = M.release();
+    return OK;
+}
+
+void mfma_destroy(Mfma *m) { delete m; }
+
+size_t mfma_workspace_bytes(const Mfma &m) { return m.tmp.bytes; }
+
+template <int VAR>
+static hipError_t launch_var(const MfmaParams &P, int pass, unsigned grid, hipStream_t s)
+{
+    if (pass == MP_COL) hipLaunchKernelGGL((k_mfma256<VAR, MP_COL>), dim3(grid), dim3(256), 0, s, P);
+    else if (pass == MP_ROW) hipLaunchKernelGGL((k_mfma256<VAR, MP_ROW>), dim3(grid), dim3(256), 0, s, P);
+    else hipLaunchKernelGGL((k_mfma256<VAR, MP_SINGLE>), dim3(grid), dim3(256), 0, s, P);
+    return hipGetLastError();
+}
+
+int mfma_execute(Mfma &m, const void *in, void *out, int forward, double fct, hipStream_t s, Error &err)
+{
+    if (!in || !out) { err = {E_INVALID, "null data pointer"}; return E_INVALID; }
+    MfmaParams P{};
+    P.batch = m.batch;
+    P.L = (int)m.L;
+    P.sign = forward ? -1.0 : 1.0;
+    P.fct = fct;
+    P.tw_l = (const double2 *)m.tw_l.p;
+    P.tw_l32 = (const float4 *)m.tw_l32.p;
+    P.tw256 = (const double2 *)m.tw256.p;
+    P.tw256_32 = (const float4 *)m.tw256_32.p;
+    auto go = [&](int pass, const void *src, void *dst, unsigned grid) {
+        P.src = src;
+        P.dst = dst;
+        return m.precision == MF_F32 ? launch_var<MF_F32>(P, pass, grid, s) : launch_var<MF_BF16X>(P, pass, grid, s);
+    };
+    hipError_t e;
+    if (m.L == 256) {
+        e = go(MP_SINGLE, in, out, (unsigned)((m.batch + kTile - 1) / kTile));
+    } else {
+        const long long tiles = m.batch * 16;
+        if (tiles > 0x7fffffffLL) { err = {E_UNSUPPORTED, "batch too large for one launch"}; return E_UNSUPPORTED; }
+        e = go(MP_COL, in, m.tmp.p, (unsigned)tiles);
+        if (e == hipSuccess) e = go(MP_ROW, m.tmp.p, out, (unsigned)tiles);
+    }
+    if (e != hipSuccess) {
+        err = {E_HIP, std::string("matrix-core FFT launch: ") + hipGetErrorString(e)};
+        return E_HIP;
+    }
+    return OK;
+}
+
+}  // namespace ifft

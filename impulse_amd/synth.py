@@ -5,6 +5,8 @@ standard SplitMix64 finaliser; u = (z >> 11) * 2**-53 lies in [0, 1).
 Normals use Box-Muller on consecutive draws (u1, u2):
   r = sqrt(-2 ln(1 - u1)),  z0 = r cos(2 pi u2),  z1 = r sin(2 pi u2).
 """
+import math
+
 import numpy as np
 
 _GAMMA = np.uint64(0x9E3779B97F4A7C15)
@@ -49,7 +51,7 @@ def complex_normal(shape, seed=0, dtype=np.complex128):
     return (z[..., 0] + 1j * z[..., 1]).astype(dtype, copy=False)
 
 
-def uniform_device(n, lo=-1.0, hi=1.0, seed=0, dtype=None, device="cuda", chunk=1 << 26):
+def uniform_device(n, lo=-1.0, hi=1.0, seed=0, dtype=None, device="cuda", chunk=1 << 26, start=0):
     """uniform(n, lo, hi, seed) computed on a torch device, bit-identical to the
     numpy version (SplitMix64 in wrapping int64 arithmetic, logical shifts by
     masking), in chunks of `chunk` draws."""
@@ -67,7 +69,7 @@ def uniform_device(n, lo=-1.0, hi=1.0, seed=0, dtype=None, device="cuda", chunk=
 
     for s in range(0, n, chunk):
         m = min(chunk, n - s)
-        i = torch.arange(s + 1, s + m + 1, dtype=torch.int64, device=device)
+        i = torch.arange(start + s + 1, start + s + m + 1, dtype=torch.int64, device=device)
         z = c64(int(seed) % (1 << 64)) + i * gamma
         z = (z ^ lsr(z, 30)) * m1
         z = (z ^ lsr(z, 27)) * m2
@@ -134,4 +136,21 @@ def ragged_lengths(count, lo=64, hi=262144, smooth_frac=0.7, seed=0, prime_min=1
             m = max(1, min(m, int(T // prime_min)))
             k = int(np.searchsorted(primes, int(T // m), side="right")) - 1
             out[i] = int(primes[k]) * m
+    return out
+
+
+def complex_normal_device(n, seed=0, device="cuda", dtype=None, chunk=1 << 25):
+    """complex_normal((n,), seed) computed on a torch device: value i is the
+    Box-Muller pair of draws 2i, 2i+1 (re = r cos, im = r sin), as in
+    complex_normal (equal to it up to the device's rounding of log, sqrt, cos
+    and sin)."""
+    import torch
+    dtype = dtype or torch.complex128
+    out = torch.empty(n, dtype=dtype, device=device)
+    for s in range(0, n, chunk):
+        m = min(chunk, n - s)
+        u = uniform_device(2 * m, 0.0, 1.0, seed=seed, device=device, start=2 * s).view(m, 2)
+        r = torch.sqrt(-2.0 * torch.log1p(-u[:, 0]))
+        th = 2.0 * math.pi * u[:, 1]
+        out[s:s + m] = torch.complex(r * torch.cos(th), r * torch.sin(th)).to(dtype)
     return out

@@ -199,6 +199,37 @@ int impulse_fft_ragged_info(impulse_fft_ragged batch, int *threads, int *lds_poi
 int impulse_fft_ragged_destroy(impulse_fft_ragged batch);
 
 /* ------------------------------------------------------------------------- */
+/* Matrix-core complex FFTs (no PocketFFT counterpart)                        */
+/* ------------------------------------------------------------------------- */
+
+/* `batch` contiguous complex64 transforms of `length` (256 or 65536) whose
+ * radix-16 DFT stages run as 16x16x16 complex matrix products on the MFMA
+ * units.  Nearest reference: pocketfft::c2c<float> over one axis
+ * (pocketfft_hdronly.h:3272-3282, impulse's FFTDesc[Complex32] through
+ * cpp_pocketfft/pocketfft.nim:235-277), here with the precision stated:
+ *   IMPULSE_FFT_MFMA_F32    FP32 MFMA, FP32 accumulation, FP64-generated
+ *                           twiddles applied as split-FP32 products;
+ *                           complex64 output (error ~1e-7 relative).
+ *   IMPULSE_FFT_MFMA_BF16X  data cut into five 8-bit bf16 slices per stage,
+ *                           slice products accumulated exactly in FP32 by
+ *                           bf16 MFMA and combined in FP64; FP64 twiddles;
+ *                           complex128 output (error ~1e-12 relative).
+ * forward = 1 computes exp(-2 pi i jk/n); fct scales the output (in FP64).
+ * Buffers are DEVICE pointers; in and out must not overlap.  65536-point
+ * plans hold a workspace of one intermediate (batch * 65536 elements of the
+ * output type). */
+typedef enum impulse_fft_mfma_precision {
+    IMPULSE_FFT_MFMA_F32 = 0,
+    IMPULSE_FFT_MFMA_BF16X = 1
+} impulse_fft_mfma_precision;
+typedef struct impulse_fft_mfma_s *impulse_fft_mfma;
+int impulse_fft_mfma_create(impulse_fft_mfma *plan, size_t length, size_t batch, int precision);
+int impulse_fft_mfma_execute(impulse_fft_mfma plan, const void *in, void *out, int forward, double fct,
+                             void *stream);
+int impulse_fft_mfma_workspace_bytes(impulse_fft_mfma plan, size_t *bytes);
+int impulse_fft_mfma_destroy(impulse_fft_mfma plan);
+
+/* ------------------------------------------------------------------------- */
 /* 1-D in-place plans on host memory (C path)                                 */
 /* ------------------------------------------------------------------------- */
 

@@ -207,6 +207,101 @@ def ragged_form(torch, reps, loop_sample=600):
     }
 
 
+def mfma_form(torch, reps):
+    """Matrix-core form: 8192 complex64 transforms of 65536 (re/im N(0,1)),
+    radix-16 stages on MFMA in two precisions, against the FP64 drop-in c2c
+    plan on the same values (complex128 in and out).  Errors are relative L2
+    per transform against the FP64 plan over 64 sampled transforms."""
+    from impulse_amd import _lib, synth
+    from impulse_amd import fft as F
+    from impulse_amd.mfma import MfmaFFT
+
+    B, L = 8192, 65536
+    flops = 5.0 * L * math.log2(L) * B
+    x = synth.complex_normal_device(B * L, seed=0, dtype=torch.complex64)
+    rows = np.random.default_rng(4).choice(B, 64, replace=False)
+    sp = torch.cuda.current_stream().cuda_stream
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    # FP64 drop-in on the same values
+    x64 = x.to(torch.complex128)
+    ref = torch.empty_like(x64)
+    p = F.Plan(_lib.C2C, [B, L], [L * 16, 16], [L * 16, 16], [1], True)
+    ms64 = timed(lambda: p.execute(x64.data_ptr(), ref.data_ptr(), 1.0, sp))
+    refs = ref.view(B, L)[rows].clone()
+    del x64, ref, p
+    out = {"form": "mfma", "workload": "c2c complex64 65536 x 8192, re/im N(0,1)", "gflop": round(flops / 1e9, 2),
+           "fp64_dropin": {"ms": round(ms64, 3), "gflops": round(flops / (ms64 * 1e-3) / 1e9, 1),
+                           "io": "complex128 in and out"}}
+    for prec, io_bytes in (("f32", 8 + 8), ("bf16x", 8 + 16)):
+        plan = MfmaFFT(L, B, prec)
+        y = torch.empty(B * L, dtype=plan.out_dtype, device="cuda")
+        ms = timed(lambda: plan(x, y))
+        err = (torch.linalg.vector_norm(y.view(B, L)[rows].to(torch.complex128) - refs, dim=1) /
+               torch.linalg.vector_norm(refs, dim=1)).cpu().numpy()
+        comp = io_bytes * B * L
+        out[prec] = {"ms": round(ms, 3), "gflops": round(flops / (ms * 1e-3) / 1e9, 1),
+                     "compulsory_GBs": round(comp / (ms * 1e-3) / 1e9, 1),
+                     "hbm_roofline_frac": round(comp / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "launches": 2,
+                     "rel_l2_vs_fp64": {"max": float(err.max()), "median": float(np.median(err))},
+                     "speedup_vs_fp64_dropin": round(ms64 / ms, 2)}
+        del plan, y
+        torch.cuda.empty_cache()
+    return out
+
+
+def conv_form(torch, reps):
+    """Differentiation form: 'same' FFT convolution of 16 float32 images 2048x2048
+    (U[0,1]) with a 63x63 kernel (N(0, 1/63^2)), forward and the adjoint
+    backward (image and kernel gradients) of loss = sum((y - t)^2)."""
+    from impulse_amd import autograd as AG
+    from impulse_amd import synth
+
+    B, H, K = 16, 2048, 63
+    x = synth.uniform_device(B * H * H, 0.0, 1.0, seed=11, dtype=torch.float32).view(B, H, H)
+    k = torch.from_numpy(synth.normal((K, K), seed=12, sigma=1.0 / K).astype(np.float32)).cuda()
+    t = synth.uniform_device(B * H * H, -1.0, 1.0, seed=13, dtype=torch.float32).view(B, H, H)
+    c = AG._conv(B, H, H, K, K, torch.float32)
+    P, Q = c.grid
+    y = torch.empty_like(x)
+    gx, gk = torch.empty_like(x), torch.empty_like(k)
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    fwd = timed(lambda: c.forward(x, k, y))
+    gy = 2.0 * (y - t)
+    bwd = timed(lambda: c.backward(x, k, gy, gx, gk))
+    # transforms per call: forward r2c of the images + kernel, c2r of the product;
+    # backward (recomputing spectra) r2c-adjoint, c2r-adjoints for both gradients
+    n = P * Q
+    fl_r = 2.5 * n * math.log2(n)
+    return {"form": "conv2d", "workload": "16 x 2048^2 f32 images * 63x63 kernel, 'same', FP64 arithmetic",
+            "grid": [P, Q], "forward_ms": round(fwd, 3), "backward_ms": round(bwd, 3),
+            "forward_gflops": round((B + 1 + B) * fl_r / (fwd * 1e-3) / 1e9, 1),
+            "note": "backward recomputes the image and kernel spectra instead of storing them"}
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -306,7 +401,8 @@ def main():
                           "hbm_roofline_frac": round(p.comp_bytes / (pms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                           "launches": npass})
 
-    forms = [] if args.no_forms else [ragged_form(torch, args.form_reps)]
+    forms = [] if args.no_forms else [ragged_form(torch, args.form_reps), mfma_form(torch, args.form_reps),
+                                       conv_form(torch, args.form_reps)]
     cpu = None if args.no_cpu else cpu_baseline(plist, args.cpu_seconds)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,

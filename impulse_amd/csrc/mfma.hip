@@ -38,6 +38,7 @@
 
 #include "butterflies.h"
 #include "engine.h"
+#include "fft_common.h"
 
 namespace ifft {
 
@@ -47,8 +48,11 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 enum { MF_F32 = 0, MF_BF16X = 1 };
 enum { MP_COL = 0, MP_ROW = 1, MP_SINGLE = 2 };
 constexpr int kSlices = 5;
-constexpr int kTile = 16;           // sub-transforms per workgroup
 constexpr int kRow = 273;           // LDS row stride (elements): 256 + 256/16 padding + 1
+// sub-transforms per workgroup (four per wavefront): 32 for FP32 data (256-byte
+// segments in the strided passes), 16 for FP64 data (LDS budget)
+template <int VAR>
+constexpr int tile_of() { return VAR == 0 ? 32 : 16; }
 
 struct MfmaParams {
     const void *src;
@@ -57,10 +61,7 @@ struct MfmaParams {
     int L;                          // 256 or 65536
     double sign;                    // -1 forward, +1 backward
     double fct;                     // applied by the last pass
-    const double2 *tw_l;            // W_L^m (forward sign), m < L, FP64 (L = 65536)
-    const float4 *tw_l32;           // the same as (re_hi, re_lo, im_hi, im_lo)
-    const double2 *tw256;           // W_256^m, m < 256
-    const float4 *tw256_32;
+    const double2 *tw;              // 512 FP64 twiddles (forward sign): W_256^m, then W_65536^m, m < 256
 };
 
 __device__ __forceinline__ int lidx(int n) { return n + (n >> 4); }
@@ -131,10 +132,11 @@ __device__ __forceinline__ void f32_stage(float2 (&a)[4], const F32Consts &C)
     for (int r = 0; r < 4; ++r) a[r] = make_float2(dr[r], di[r]);
 }
 
-// y * (w_hi + w_lo), w given with the forward sign; conjugated for sign = +1
-__device__ __forceinline__ float2 f32_twiddle(float2 y, float4 w, float sgn)
+// y * (w_hi + w_lo): w (FP64, sign applied) split into FP32 high and low parts
+__device__ __forceinline__ float2 f32_twiddle(float2 y, double2 w)
 {
-    const float wr = w.x, wrl = w.y, wi = -sgn * w.z, wil = -sgn * w.w;
+    const float wr = (float)w.x, wi = (float)w.y;
+    const float wrl = (float)(w.x - (double)wr), wil = (float)(w.y - (double)wi);
     float re = fmaf(y.x, wr, -y.y * wi);
     float im = fmaf(y.x, wi, y.y * wr);
     re += fmaf(y.x, wrl, -y.y * wil);
@@ -242,47 +244,60 @@ __device__ __forceinline__ void bx_stage(double2 (&a)[4], const BxConsts &C)
 // ---------------------------------------------------------------------------
 
 template <int VAR, int PASS>
-__global__ void __launch_bounds__(256) k_mfma256(const MfmaParams P)
+__global__ void __launch_bounds__(16 * tile_of<VAR>()) k_mfma256(const MfmaParams P)
 {
+    constexpr int TILE = tile_of<VAR>(), NT = 16 * TILE, TPS = 256 / TILE;   // threads, tiles per signal
     using E = typename Elem<VAR>::T;
-    using EI = float2;                                  // pass-A / single-pass input: complex64
-    using EO = E;                                       // intermediate and output
-    __shared__ E lds[kTile * kRow];
+    __shared__ E lds[TILE * kRow];
+    __shared__ double2 tws[512];   // W_256^m and W_65536^m (m < 256), forward sign
     const int tid = threadIdx.x;
-    const long long tile = blockIdx.x;
+    // the tiles of one signal run on one XCD, so its 2 KB rows are fetched into one L2
+    const long long tile = xcd_block(blockIdx.x, gridDim.x);
     const long long L = P.L;
-    // tile geometry
     long long base = 0;   // first element of the signal (COL/ROW) or of transform 0 of the tile (SINGLE)
-    int t0 = 0, ntr = kTile;
+    int t0 = 0, ntr = TILE;
     if (PASS == MP_SINGLE) {
-        base = tile * kTile * 256;
-        ntr = (int)min((long long)kTile, P.batch - tile * kTile);
+        base = tile * TILE * 256;
+        ntr = (int)min((long long)TILE, P.batch - tile * TILE);
     } else {
-        base = (tile >> 4) * L;
-        t0 = (int)(tile & 15) * kTile;
+        base = (tile / TPS) * L;
+        t0 = (int)(tile % TPS) * TILE;
     }
-    // ---- load ----
+    for (int m = tid; m < (PASS == MP_COL ? 512 : 256); m += NT) tws[m] = P.tw[m];
+    auto put = [&](int c, int n, float2 v) {
+        if constexpr (VAR == MF_F32) lds[c * kRow + lidx(n)] = v;
+        else lds[c * kRow + lidx(n)] = make_double2((double)v.x, (double)v.y);
+    };
+    // ---- load (16-byte accesses: complex64 pairs, or one complex128) ----
     if (PASS == MP_COL) {
-        const EI *x = (const EI *)P.src + base;
-        for (int e = tid; e < kTile * 256; e += 256) {
-            const int c = e & 15, n = e >> 4;
-            const EI v = x[(long long)(t0 + c) + 256LL * n];
-            if constexpr (VAR == MF_F32) lds[c * kRow + lidx(n)] = v;
-            else lds[c * kRow + lidx(n)] = make_double2((double)v.x, (double)v.y);
+        // row n of the tile: TILE consecutive complex64 at t0 + 256 n
+        const float4 *x = (const float4 *)((const float2 *)P.src + base + t0);
+        for (int e = tid; e < TILE * 128; e += NT) {
+            const int p = e % (TILE / 2), n = e / (TILE / 2);
+            const float4 v = x[128LL * n + p];
+            put(2 * p, n, make_float2(v.x, v.y));
+            put(2 * p + 1, n, make_float2(v.z, v.w));
         }
     } else if (PASS == MP_ROW) {
-        const EO *x = (const EO *)P.src + base;
-        for (int e = tid; e < kTile * 256; e += 256) {
-            const int c = e >> 8, n = e & 255;
-            lds[c * kRow + lidx(n)] = x[256LL * (t0 + c) + n];
+        if constexpr (VAR == MF_F32) {
+            const float4 *x = (const float4 *)((const float2 *)P.src + base + 256LL * t0);
+            for (int e = tid; e < TILE * 128; e += NT) {
+                const int c = e >> 7, pp = e & 127;
+                const float4 v = x[e];
+                put(c, 2 * pp, make_float2(v.x, v.y));
+                put(c, 2 * pp + 1, make_float2(v.z, v.w));
+            }
+        } else {
+            const double2 *x = (const double2 *)P.src + base + 256LL * t0;
+            for (int e = tid; e < TILE * 256; e += NT) lds[(e >> 8) * kRow + lidx(e & 255)] = x[e];
         }
     } else {
-        const EI *x = (const EI *)P.src + base;
-        for (int e = tid; e < kTile * 256; e += 256) {
-            const int c = e >> 8, n = e & 255;
-            const EI v = c < ntr ? x[e] : EI{0.f, 0.f};   // a partial last tile computes zeros
-            if constexpr (VAR == MF_F32) lds[c * kRow + lidx(n)] = v;
-            else lds[c * kRow + lidx(n)] = make_double2((double)v.x, (double)v.y);
+        const float4 *x = (const float4 *)((const float2 *)P.src + base);
+        for (int e = tid; e < TILE * 128; e += NT) {
+            const int c = e >> 7, pp = e & 127;
+            const float4 v = c < ntr ? x[e] : float4{0.f, 0.f, 0.f, 0.f};   // a partial last tile computes zeros
+            put(c, 2 * pp, make_float2(v.x, v.y));
+            put(c, 2 * pp + 1, make_float2(v.z, v.w));
         }
     }
     __syncthreads();
@@ -290,9 +305,17 @@ __global__ void __launch_bounds__(256) k_mfma256(const MfmaParams P)
     const int l = tid & 63, w = tid >> 6, g = l >> 4, j = l & 15;
     const double sign = P.sign;
     const bool last = PASS != MP_COL;
+    // W_256^(b k1) for this lane's four (b, k1) pairs; the four-step twiddle
+    // W_65536^(t k) = W_65536^(t k mod 256) W_256^(t k / 256), both factors from LDS, product in FP64
+    double2 w1[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) w1[r] = tw_sign(tws[(4 * g + r) * j], sign);
+    auto step_tw = [&](int t, int k) -> double2 {
+        const int m = t * k;   // < 65536
+        return tw_sign(cmul(tws[256 + (m & 255)], tws[m >> 8]), sign);
+    };
     if constexpr (VAR == MF_F32) {
         const F32Consts C = f32_consts(sign);
-        const float sg = (float)sign;
         for (int u = 0; u < 4; ++u) {
             const int c = 4 * w + u;
             float2 a[4];
@@ -300,13 +323,13 @@ __global__ void __launch_bounds__(256) k_mfma256(const MfmaParams P)
             for (int r = 0; r < 4; ++r) a[r] = lds[c * kRow + lidx(j + 16 * (4 * g + r))];
             f32_stage(a, C);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) a[r] = f32_twiddle(a[r], P.tw256_32[((4 * g + r) * j) & 255], sg);
+            for (int r = 0; r < 4; ++r) a[r] = f32_twiddle(a[r], w1[r]);
             f32_stage(a, C);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int k = 4 * g + r + 16 * j;
                 float2 v = a[r];
-                if (PASS == MP_COL) v = f32_twiddle(v, P.tw_l32[((t0 + c) * k) & 65535], sg);
+                if (PASS == MP_COL) v = f32_twiddle(v, step_tw(t0 + c, k));
                 if (last && P.fct != 1.0) v = make_float2((float)(v.x * P.fct), (float)(v.y * P.fct));
                 lds[c * kRow + lidx(k)] = v;
             }
@@ -320,13 +343,13 @@ __global__ void __launch_bounds__(256) k_mfma256(const MfmaParams P)
             for (int r = 0; r < 4; ++r) a[r] = lds[c * kRow + lidx(j + 16 * (4 * g + r))];
             bx_stage(a, C);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) a[r] = cmul(a[r], tw_sign(P.tw256[((4 * g + r) * j) & 255], sign));
+            for (int r = 0; r < 4; ++r) a[r] = cmul(a[r], w1[r]);
             bx_stage(a, C);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int k = 4 * g + r + 16 * j;
                 double2 v = a[r];
-                if (PASS == MP_COL) v = cmul(v, tw_sign(P.tw_l[((t0 + c) * k) & 65535], sign));
+                if (PASS == MP_COL) v = cmul(v, step_tw(t0 + c, k));
                 if (last) v = cscale(v, P.fct);
                 lds[c * kRow + lidx(k)] = v;
             }
@@ -334,21 +357,33 @@ __global__ void __launch_bounds__(256) k_mfma256(const MfmaParams P)
     }
     __syncthreads();
     // ---- store ----
-    EO *y = (EO *)P.dst + base;
-    if (PASS == MP_COL) {
-        for (int e = tid; e < kTile * 256; e += 256) {
-            const int c = e & 15, k = e >> 4;
-            y[(long long)(t0 + c) + 256LL * k] = lds[c * kRow + lidx(k)];
-        }
-    } else if (PASS == MP_ROW) {
-        for (int e = tid; e < kTile * 256; e += 256) {
-            const int c = e & 15, k = e >> 4;
-            y[(long long)(t0 + c) + 256LL * k] = lds[c * kRow + lidx(k)];
+    if constexpr (VAR == MF_F32) {
+        float2 *y = (float2 *)P.dst + base;
+        if (PASS == MP_SINGLE) {
+            float4 *y4 = (float4 *)y;
+            for (int e = tid; e < ntr * 128; e += NT) {
+                const int c = e >> 7, kk = 2 * (e & 127);
+                const float2 a = lds[c * kRow + lidx(kk)], b = lds[c * kRow + lidx(kk + 1)];
+                y4[e] = make_float4(a.x, a.y, b.x, b.y);
+            }
+        } else {
+            // element (c, k) goes to t0 + c + 256 k: TILE consecutive complex64 per k
+            float4 *y4 = (float4 *)(y + t0);
+            for (int e = tid; e < TILE * 128; e += NT) {
+                const int p = e % (TILE / 2), k = e / (TILE / 2);
+                const float2 a = lds[2 * p * kRow + lidx(k)], b = lds[(2 * p + 1) * kRow + lidx(k)];
+                y4[128LL * k + p] = make_float4(a.x, a.y, b.x, b.y);
+            }
         }
     } else {
-        for (int e = tid; e < ntr * 256; e += 256) {
-            const int c = e >> 8, k = e & 255;
-            y[e] = lds[c * kRow + lidx(k)];
+        double2 *y = (double2 *)P.dst + base;
+        if (PASS == MP_SINGLE) {
+            for (int e = tid; e < ntr * 256; e += NT) y[e] = lds[(e >> 8) * kRow + lidx(e & 255)];
+        } else {
+            for (int e = tid; e < TILE * 256; e += NT) {
+                const int c = e % TILE, k = e / TILE;
+                y[(long long)(t0 + c) + 256LL * k] = lds[c * kRow + lidx(k)];
+            }
         }
     }
 }
@@ -360,17 +395,8 @@ __global__ void __launch_bounds__(256) k_mfma256(const MfmaParams P)
 struct Mfma {
     int precision = 0;
     long long L = 0, batch = 0;
-    DevBuf tw_l, tw_l32, tw256, tw256_32, tmp;
+    DevBuf tw, tmp;
 };
-
-static void split32(const std::vector<double2> &w, std::vector<float4> &o)
-{
-    o.resize(w.size());
-    for (size_t i = 0; i < w.size(); ++i) {
-        const float rh = (float)w[i].x, ih = (float)w[i].y;
-        o[i] = make_float4(rh, (float)(w[i].x - (double)rh), ih, (float)(w[i].y - (double)ih));
-    }
-}
 
 static int upload(DevBuf &b, const void *p, size_t n)
 {
@@ -398,19 +424,15 @@ int mfma_create(Mfma **out, size_t length, size_t batch, int precision, Error &e
     M->precision = precision;
     M->L = (long long)length;
     M->batch = (long long)batch;
-    std::vector<double2> t256, tl;
-    std::vector<float4> s256, sl;
+    // W_256^m then W_65536^m for m < 256, from the long-double host generator
+    std::vector<double2> t256, t64k;
     twiddle_table(t256, 256, 256, 1);
-    split32(t256, s256);
-    rc = upload(M->tw256, t256.data(), t256.size() * sizeof(double2));
-    if (rc == OK) rc = upload(M->tw256_32, s256.data(), s256.size() * sizeof(float4));
+    twiddle_table(t64k, 65536, 256, 1);
+    t256.insert(t256.end(), t64k.begin(), t64k.end());
+    rc = upload(M->tw, t256.data(), t256.size() * sizeof(double2));
     if (rc == OK && length == 65536) {
-        twiddle_table(tl, 65536, 65536, 1);
-        split32(tl, sl);
-        rc = upload(M->tw_l, tl.data(), tl.size() * sizeof(double2));
-        if (rc == OK) rc = upload(M->tw_l32, sl.data(), sl.size() * sizeof(float4));
         const size_t es = precision == MF_F32 ? sizeof(float2) : sizeof(double2);
-        if (rc == OK && M->tmp.alloc((size_t)length * batch * es)) rc = E_NOMEM;
+        if (M->tmp.alloc((size_t)length * batch * es)) rc = E_NOMEM;
     }
     if (rc != OK) {
         err = {rc, "matrix-core FFT table or workspace allocation failed"};
@@ -425,39 +447,43 @@ void mfma_destroy(Mfma *m) { delete m; }
 size_t mfma_workspace_bytes(const Mfma &m) { return m.tmp.bytes; }
 
 template <int VAR>
-static hipError_t launch_var(const MfmaParams &P, int pass, unsigned grid, hipStream_t s)
+static hipError_t launch_var(const MfmaParams &P, int pass, long long batch, hipStream_t s)
 {
-    if (pass == MP_COL) hipLaunchKernelGGL((k_mfma256<VAR, MP_COL>), dim3(grid), dim3(256), 0, s, P);
-    else if (pass == MP_ROW) hipLaunchKernelGGL((k_mfma256<VAR, MP_ROW>), dim3(grid), dim3(256), 0, s, P);
-    else hipLaunchKernelGGL((k_mfma256<VAR, MP_SINGLE>), dim3(grid), dim3(256), 0, s, P);
+    constexpr int TILE = tile_of<VAR>();
+    const long long grid = pass == MP_SINGLE ? (batch + TILE - 1) / TILE : batch * (256 / TILE);
+    if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
+    const dim3 g((unsigned)grid), b(16 * TILE);
+    if (pass == MP_COL) hipLaunchKernelGGL((k_mfma256<VAR, MP_COL>), g, b, 0, s, P);
+    else if (pass == MP_ROW) hipLaunchKernelGGL((k_mfma256<VAR, MP_ROW>), g, b, 0, s, P);
+    else hipLaunchKernelGGL((k_mfma256<VAR, MP_SINGLE>), g, b, 0, s, P);
     return hipGetLastError();
 }
 
 int mfma_execute(Mfma &m, const void *in, void *out, int forward, double fct, hipStream_t s, Error &err)
 {
     if (!in || !out) { err = {E_INVALID, "null data pointer"}; return E_INVALID; }
+    if (((uintptr_t)in | (uintptr_t)out) & 15) {
+        err = {E_INVALID, "matrix-core FFT buffers must be 16-byte aligned"};
+        return E_INVALID;
+    }
     MfmaParams P{};
     P.batch = m.batch;
     P.L = (int)m.L;
     P.sign = forward ? -1.0 : 1.0;
     P.fct = fct;
-    P.tw_l = (const double2 *)m.tw_l.p;
-    P.tw_l32 = (const float4 *)m.tw_l32.p;
-    P.tw256 = (const double2 *)m.tw256.p;
-    P.tw256_32 = (const float4 *)m.tw256_32.p;
-    auto go = [&](int pass, const void *src, void *dst, unsigned grid) {
+    P.tw = (const double2 *)m.tw.p;
+    auto go = [&](int pass, const void *src, void *dst) {
         P.src = src;
         P.dst = dst;
-        return m.precision == MF_F32 ? launch_var<MF_F32>(P, pass, grid, s) : launch_var<MF_BF16X>(P, pass, grid, s);
+        return m.precision == MF_F32 ? launch_var<MF_F32>(P, pass, m.batch, s)
+                                     : launch_var<MF_BF16X>(P, pass, m.batch, s);
     };
     hipError_t e;
     if (m.L == 256) {
-        e = go(MP_SINGLE, in, out, (unsigned)((m.batch + kTile - 1) / kTile));
+        e = go(MP_SINGLE, in, out);
     } else {
-        const long long tiles = m.batch * 16;
-        if (tiles > 0x7fffffffLL) { err = {E_UNSUPPORTED, "batch too large for one launch"}; return E_UNSUPPORTED; }
-        e = go(MP_COL, in, m.tmp.p, (unsigned)tiles);
-        if (e == hipSuccess) e = go(MP_ROW, m.tmp.p, out, (unsigned)tiles);
+        e = go(MP_COL, in, m.tmp.p);
+        if (e == hipSuccess) e = go(MP_ROW, m.tmp.p, out);
     }
     if (e != hipSuccess) {
         err = {E_HIP, std::string("matrix-core FFT launch: ") + hipGetErrorString(e)};

@@ -44,6 +44,7 @@ namespace ifft {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned u4 __attribute__((ext_vector_type(4)));
 
 enum { MF_F32 = 0, MF_BF16X = 1 };
 enum { MP_COL = 0, MP_ROW = 1, MP_SINGLE = 2 };
@@ -152,14 +153,31 @@ struct BxConsts {
     bf8 br[kSlices], bi[kSlices];   // slices of [Re F; -Im F] and [Im F; Re F], scaled by 2^7
 };
 
-__device__ __forceinline__ void cut(double v, double (&s)[kSlices])
+// bf16 of two floats that are integers of magnitude <= 256 (exact: their low
+// 16 bits are zero), packed lo | hi << 16
+__device__ __forceinline__ unsigned pk_bf16(float lo, float hi)
 {
-    // v in [-128, 128]: s[0] + s[1]/2^8 + ... , each an integer in [-128, 128]
+    return __builtin_amdgcn_perm(__float_as_uint(hi), __float_as_uint(lo), 0x07060302u);
+}
+
+// Next slice of v in [-128, 128]: t = rint(v) (an integer in [-128, 128]) and
+// v <- 256 (v - t), exact in FP64.
+__device__ __forceinline__ float take(double &v)
+{
+    const double t = rint(v);
+    v = fma(v, 256.0, -256.0 * t);
+    return (float)t;
+}
+
+// the slices of four values as bf16 in elements e0 .. e0+3 of each A[s] / B[s]
+__device__ __forceinline__ void cut4(double v0, double v1, double v2, double v3, unsigned (&lo)[kSlices],
+                                     unsigned (&hi)[kSlices])
+{
 #pragma unroll
     for (int a = 0; a < kSlices; ++a) {
-        const double t = rint(v);
-        s[a] = t;
-        v = (v - t) * 256.0;
+        const float f0 = take(v0), f1 = take(v1), f2 = take(v2), f3 = take(v3);
+        lo[a] = pk_bf16(f0, f1);
+        hi[a] = pk_bf16(f2, f3);
     }
 }
 
@@ -167,19 +185,19 @@ __device__ __forceinline__ BxConsts bx_consts(double sign)
 {
     BxConsts C;
     const int l = threadIdx.x & 63, g = l >> 4, j = l & 15;
+    double fr[4], fi[4];
     for (int e = 0; e < 4; ++e) {
         const double2 w = w16((4 * g + e) * j);
-        const double fr = w.x * 128.0, fi = -sign * w.y * 128.0;
-        double sr[kSlices], si[kSlices], sn[kSlices];
-        cut(fr, sr);
-        cut(fi, si);
-        cut(-fi, sn);
-        for (int f = 0; f < kSlices; ++f) {
-            C.br[f][e] = (__bf16)(float)sr[f];
-            C.br[f][4 + e] = (__bf16)(float)sn[f];
-            C.bi[f][e] = (__bf16)(float)si[f];
-            C.bi[f][4 + e] = (__bf16)(float)sr[f];
-        }
+        fr[e] = w.x * 128.0;
+        fi[e] = -sign * w.y * 128.0;
+    }
+    unsigned rl[kSlices], rh[kSlices], il[kSlices], ih[kSlices], nl[kSlices], nh[kSlices];
+    cut4(fr[0], fr[1], fr[2], fr[3], rl, rh);
+    cut4(fi[0], fi[1], fi[2], fi[3], il, ih);
+    cut4(-fi[0], -fi[1], -fi[2], -fi[3], nl, nh);
+    for (int f = 0; f < kSlices; ++f) {
+        C.br[f] = __builtin_bit_cast(bf8, (u4){rl[f], rh[f], nl[f], nh[f]});
+        C.bi[f] = __builtin_bit_cast(bf8, (u4){il[f], ih[f], rl[f], rh[f]});
     }
     return C;
 }
@@ -202,16 +220,12 @@ __device__ __forceinline__ void bx_stage(double2 (&a)[4], const BxConsts &C)
     (void)frexp(mx, &E);     // mx < 2^E
     const double up = ldexp(1.0, 7 - E);
     bf8 A[kSlices];
+    {
+        unsigned rl[kSlices], rh[kSlices], il[kSlices], ih[kSlices];
+        cut4(a[0].x * up, a[1].x * up, a[2].x * up, a[3].x * up, rl, rh);
+        cut4(a[0].y * up, a[1].y * up, a[2].y * up, a[3].y * up, il, ih);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        double sr[kSlices], si[kSlices];
-        cut(a[r].x * up, sr);
-        cut(a[r].y * up, si);
-#pragma unroll
-        for (int s = 0; s < kSlices; ++s) {
-            A[s][r] = (__bf16)(float)sr[s];
-            A[s][4 + r] = (__bf16)(float)si[s];
-        }
+        for (int s = 0; s < kSlices; ++s) A[s] = __builtin_bit_cast(bf8, (u4){rl[s], rh[s], il[s], ih[s]});
     }
     f4 accr[kSlices], acci[kSlices];
 #pragma unroll

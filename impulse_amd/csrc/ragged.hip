@@ -22,7 +22,7 @@
 // Twiddles and the chirp exp(-i pi j^2 / M) are generated in FP64 with
 // sincospi (the chirp index j^2 mod 2M in 64-bit integers).  Phases that hand
 // data to other threads of the workgroup through global scratch are separated
-// by a device-scope fence and a barrier.  Input is float32; arithmetic and the
+// by a workgroup barrier.  Input is float32; arithmetic and the
 // output bins (complex128) are FP64.
 //
 // The autotuner chooses the workgroup size, the LDS budget per transform
@@ -258,11 +258,11 @@ __device__ __noinline__ void lds_fft(double2 *lds, int L, int count, double s, i
     if (ns == 0) __syncthreads();
 }
 
-__device__ __forceinline__ void wg_sync_global()
-{
-    __threadfence();
-    __syncthreads();
-}
+// Hand global scratch to the other threads of the workgroup.  A barrier's
+// workgroup-scope release/acquire covers global memory too: the waves of a
+// workgroup share one CU and its write-through L1 (no threadgroup split).  A
+// device-scope fence here would write back and invalidate the whole L2.
+__device__ __forceinline__ void wg_sync_global() { __syncthreads(); }
 
 // Complex transform of length T.L (sign s) by the workgroup; load(m) gives
 // input point m.  Returns true with the result in LDS (natural order), or

@@ -266,22 +266,23 @@ __device__ __forceinline__ void wg_sync_global() { __syncthreads(); }
 
 // Complex transform of length T.L (sign s) by the workgroup; load(m) gives
 // input point m.  Returns true with the result in LDS (natural order), or
-// false with the result in `out` (global; times mul[] pointwise when `mul` is
-// given).  `tmp` is global scratch of L points for the four-step split.
+// false with the result in `out` in row layout: out[k2 L1 + k1] = X[k2 + L2 k1]
+// (four-step, used for the chirp-kernel spectrum table).  `tmp` is global
+// scratch of L points.
 template <class LOAD>
 __device__ bool wg_fft(LOAD load, const RaggedTask &T, double s, double2 *lds, double2 *tmp, double2 *out,
-                       int order, const double2 *mul = nullptr)
+                       int order);
+
+// Column pass of a four-step transform L = L1 L2: columns n1 (length L2,
+// stride L1, point m = n1 + L1 n2 from load(m)) are transformed with sign s in
+// LDS batches of ca and written times W_L^(s n1 k2) to tmp[n1 + L1 k2].  The
+// batches make the strided accesses ca points wide.
+template <class LOAD>
+__device__ void col_pass(LOAD load, const RaggedTask &T, double s, double2 *lds, double2 *tmp, int order)
 {
     const int tid = threadIdx.x, nthr = blockDim.x;
     const long long L = T.L;
-    if (T.L1 == 0) {
-        for (long long m = tid; m < L; m += nthr) lds[m] = load(m);
-        __syncthreads();
-        lds_fft(lds, (int)L, 1, s, order);
-        return true;
-    }
-    const int L1 = T.L1, L2 = T.L2, ca = T.ca, cb = T.cb;
-    // pass A: columns n1 (length L2, stride L1) times W_L^{n1 k2} -> tmp[n1 + L1 k2]
+    const int L1 = T.L1, L2 = T.L2, ca = T.ca;
     for (int c0 = 0; c0 < L1; c0 += ca) {
         const int cnt = min(ca, L1 - c0);
         for (int e = tid; e < ca * L2; e += nthr) {
@@ -304,24 +305,125 @@ __device__ bool wg_fft(LOAD load, const RaggedTask &T, double s, double2 *lds, d
         __syncthreads();
     }
     wg_sync_global();
-    // pass B: rows k2 (length L1, contiguous in tmp) -> out[k2 + L2 k1]
+}
+
+// Row pass: every row k2 of tmp (L1 contiguous points) is transformed in
+// place with sign s, in LDS batches of cb rows.  With `spec` (Bluestein):
+// forward transform, times spec[k2 L1 + k1], inverse transform, times
+// W_L^(+a k2) -- the middle of the chirp convolution in one pass.
+__device__ void row_pass(const RaggedTask &T, double s, double2 *lds, double2 *tmp, int order, const double2 *spec)
+{
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const long long L = T.L;
+    const int L1 = T.L1, L2 = T.L2, cb = T.cb;
     for (int r0 = 0; r0 < L2; r0 += cb) {
         const int cnt = min(cb, L2 - r0);
-        for (int e = tid; e < cb * L1; e += nthr) {
-            const int t = e / L1;
-            lds[e] = t < cnt ? tmp[(long long)(r0 + t) * L1 + (e - t * L1)] : make_double2(0.0, 0.0);
-        }
+        double2 *row = tmp + (long long)r0 * L1;
+        for (int e = tid; e < cb * L1; e += nthr) lds[e] = e < cnt * L1 ? row[e] : make_double2(0.0, 0.0);
         __syncthreads();
         lds_fft(lds, L1, cb, s, order);
-        for (int e = tid; e < cb * L1; e += nthr) {
-            const int t = e % cb, k1 = e / cb;
-            if (t < cnt) {
-                const long long o = (long long)(r0 + t) + (long long)L2 * k1;
-                out[o] = mul ? cmul(lds[t * L1 + k1], mul[o]) : lds[t * L1 + k1];
+        if (spec) {
+            const double2 *sp = spec + (long long)r0 * L1;
+            for (int e = tid; e < cnt * L1; e += nthr) lds[e] = cmul(lds[e], sp[e]);
+            __syncthreads();
+            lds_fft(lds, L1, cb, -s, order);
+            for (int e = tid; e < cnt * L1; e += nthr) {
+                const int t = e / L1, a = e - t * L1;
+                row[e] = cmul(lds[e], twid((long long)a * (r0 + t), L, -s));
             }
+        } else {
+            for (int e = tid; e < cnt * L1; e += nthr) row[e] = lds[e];
         }
         __syncthreads();
     }
+    wg_sync_global();
+}
+
+// Column pass producing final values: columns a of tmp (length L2, stride
+// L1) transformed with sign s; emit(k, v) receives c[k], k = a + L1 b.
+template <class EMIT>
+__device__ void col_pass_out(EMIT emit, const RaggedTask &T, double s, double2 *lds, const double2 *tmp, int order)
+{
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int L1 = T.L1, L2 = T.L2, ca = T.ca;
+    for (int c0 = 0; c0 < L1; c0 += ca) {
+        const int cnt = min(ca, L1 - c0);
+        for (int e = tid; e < ca * L2; e += nthr) {
+            const int t = e % ca, k2 = e / ca;
+            lds[t * L2 + k2] = t < cnt ? tmp[(long long)(c0 + t) + (long long)L1 * k2] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        lds_fft(lds, L2, ca, s, order);
+        for (int e = tid; e < ca * L2; e += nthr) {
+            const int t = e % ca, b = e / ca;
+            if (t < cnt) emit((long long)(c0 + t) + (long long)L1 * b, lds[t * L2 + b]);
+        }
+        __syncthreads();
+    }
+    wg_sync_global();
+}
+
+// Output of a four-step transform left in row layout (tmp[k2 L1 + k1] =
+// Z[k2 + L2 k1]) in tiles of 16 x 64, transposed through LDS so that both
+// the row reads and the natural-order writes are contiguous.  Even n: the
+// half-length split X[k] = E + W_n^k O with Z[M-k] read from the mirrored
+// tile, and X[M] at the end; odd n: X[k] = Z[k], k <= n/2.
+constexpr int kTR = 16, kTC = 64;
+__device__ void post_tiles(const RaggedTask &T, double2 *lds, const double2 *tmp, double2 *y, bool half, double f,
+                           double fo)
+{
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int L1 = T.L1, L2 = T.L2, n = T.n;
+    const long long M = T.L;
+    const long long limit = half ? M - 1 : n / 2;
+    for (int r = 0; r < L2; r += kTR)
+        for (int c = 0; c < L1; c += kTC) {
+            for (int e = tid; e < kTR * kTC; e += nthr) {
+                const int t = e / kTC, u = e % kTC, k2 = r + t, k1 = c + u;
+                if (k2 >= L2 || k1 >= L1) continue;
+                const long long k = k2 + (long long)L2 * k1;
+                if (k > limit) continue;
+                const double2 zk = tmp[(long long)k2 * L1 + k1];
+                double2 v = zk;
+                if (half) {
+                    const long long pm = k2 == 0 ? (long long)((L1 - k1) % L1)
+                                                 : (long long)(L2 - k2) * L1 + (L1 - 1 - k1);
+                    const double2 zm = tmp[pm];
+                    const double2 w = twid32((int)k, n, -1.0);
+                    const double2 e1 = cscale(cadd(zk, cconj(zm)), 0.5), d1 = csub(zk, cconj(zm));
+                    v = cadd(e1, cmul(make_double2(0.5 * d1.y, -0.5 * d1.x), w));
+                }
+                lds[u * (kTR + 1) + t] = make_double2(f * v.x, fo * v.y);
+            }
+            __syncthreads();
+            for (int e = tid; e < kTR * kTC; e += nthr) {
+                const int t = e % kTR, u = e / kTR, k2 = r + t, k1 = c + u;
+                if (k2 >= L2 || k1 >= L1) continue;
+                const long long k = k2 + (long long)L2 * k1;
+                if (k <= limit) y[k] = lds[u * (kTR + 1) + t];
+            }
+            __syncthreads();
+        }
+    if (half && tid == 0) {
+        const double2 z0 = tmp[0];
+        y[M] = make_double2(f * (z0.x - z0.y), 0.0);
+    }
+}
+
+template <class LOAD>
+__device__ bool wg_fft(LOAD load, const RaggedTask &T, double s, double2 *lds, double2 *tmp, double2 *out, int order)
+{
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const long long L = T.L;
+    if (T.L1 == 0) {
+        for (long long m = tid; m < L; m += nthr) lds[m] = load(m);
+        __syncthreads();
+        lds_fft(lds, (int)L, 1, s, order);
+        return true;
+    }
+    col_pass(load, T, s, lds, tmp, order);
+    row_pass(T, s, lds, tmp, order, nullptr);
+    for (long long m = tid; m < L; m += nthr) out[m] = tmp[m];
     wg_sync_global();
     return false;
 }
@@ -339,8 +441,7 @@ __global__ void __launch_bounds__(MAXT) k_ragged(const RaggedParams P)
     extern __shared__ double2 lds[];
     __shared__ int s_task;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    double2 *S0 = P.scratch + (long long)blockIdx.x * 2 * P.slot;   // four-step temporary
-    double2 *S1 = S0 + P.slot;                                        // transform results
+    double2 *S0 = P.scratch + (long long)blockIdx.x * P.slot;   // four-step scratch
     for (;;) {
         if (tid == 0) s_task = atomicAdd(P.counter, 1);
         __syncthreads();
@@ -356,12 +457,11 @@ __global__ void __launch_bounds__(MAXT) k_ragged(const RaggedParams P)
             return half ? make_double2((double)x[2 * m], (double)x[2 * m + 1]) : make_double2((double)x[m], 0.0);
         };
         const double inv2m = 0.5 / (double)M;
-        bool in_lds;
-        if (!T.blue) {
-            in_lds = wg_fft(zin, T, -1.0, lds, S0, S1, P.order);
-        } else if (P.mode == 1) {
+        const double f = P.fct, fo = P.fct * P.osign;
+        if (P.mode == 1) {
             // chirp kernel b[m] = conj(w[m]) for 0 <= m < K, conj(w[n2 - m]) for n2 - m < M
             // (indices mod n2, zero between); its unnormalized spectrum goes to the table
+            // (row layout when the transform is split)
             const long long n2 = T.L, K = half ? M : n / 2 + 1;
             double2 *B = P.spec + T.spec_off;
             auto bload = [&](long long m) -> double2 {
@@ -373,6 +473,19 @@ __global__ void __launch_bounds__(MAXT) k_ragged(const RaggedParams P)
                 for (long long m = tid; m < n2; m += nthr) B[m] = lds[m];
             __syncthreads();
             continue;
+        }
+        bool in_lds = true;    // Z in LDS (natural order) or in S0 (natural order)
+        if (!T.blue) {
+            if (T.L1 == 0) {
+                wg_fft(zin, T, -1.0, lds, S0, nullptr, P.order);
+            } else {
+                // columns, rows in place, then the transposing output pass
+                col_pass(zin, T, -1.0, lds, S0, P.order);
+                row_pass(T, -1.0, lds, S0, P.order, nullptr);
+                post_tiles(T, lds, S0, y, half, f, fo);
+                __syncthreads();
+                continue;
+            }
         } else {
             // A = FFT(z w); C = A B; c = IFFT(C) (unnormalized; 1/n2 applied with the output chirp)
             const long long n2 = T.L;
@@ -380,23 +493,39 @@ __global__ void __launch_bounds__(MAXT) k_ragged(const RaggedParams P)
             auto aload = [&](long long m) -> double2 {
                 return m < M ? cmul(zin(m), chirp(m, M, inv2m)) : make_double2(0.0, 0.0);
             };
-            if (wg_fft(aload, T, -1.0, lds, S0, S1, P.order, B)) {
+            if (T.L1 == 0) {
+                wg_fft(aload, T, -1.0, lds, S0, nullptr, P.order);
                 for (long long m = tid; m < n2; m += nthr) lds[m] = cmul(lds[m], B[m]);
                 __syncthreads();
                 lds_fft(lds, (int)n2, 1, 1.0, P.order);
-                in_lds = true;
             } else {
-                auto cload = [&](long long m) -> double2 { return S1[m]; };
-                in_lds = wg_fft(cload, T, 1.0, lds, S0, S1, P.order);
+                col_pass(aload, T, -1.0, lds, S0, P.order);
+                row_pass(T, -1.0, lds, S0, P.order, B);
+                if (half) {
+                    // c in natural order back into S0 (each column rewrites the points it read)
+                    auto keep = [&](long long k, double2 v) { S0[k] = v; };
+                    col_pass_out(keep, T, 1.0, lds, S0, P.order);
+                    in_lds = false;
+                } else {
+                    const long long K = n / 2 + 1;
+                    const double sc = 1.0 / (double)n2;
+                    auto emit = [&](long long k, double2 v) {
+                        if (k < K) {
+                            const double2 z = cscale(cmul(v, chirp(k, M, inv2m)), sc);
+                            y[k] = make_double2(f * z.x, fo * z.y);
+                        }
+                    };
+                    col_pass_out(emit, T, 1.0, lds, S0, P.order);
+                    continue;
+                }
             }
         }
         // Z[k] = FFT_M(z)[k], k < K (Bluestein: c[k] w[k] / n2)
         const double bscale = T.blue ? 1.0 / (double)T.L : 1.0;
         auto Z = [&](long long k) -> double2 {
-            const double2 v = in_lds ? lds[k] : S1[k];
+            const double2 v = in_lds ? lds[k] : S0[k];
             return T.blue ? cscale(cmul(v, chirp(k, M, inv2m)), bscale) : v;
         };
-        const double f = P.fct, fo = P.fct * P.osign;
         if (half) {
             // X[k] = E + W_n^k O with E = (Z[k] + conj Z[M-k]) / 2, O = (Z[k] - conj Z[M-k]) / 2i;
             // X[M-k] from the same pair with W_n^{M-k} = -conj(W_n^k)
@@ -485,7 +614,33 @@ static long long transform_len(long long n)
     return rg_good7(M + K - 1);
 }
 
-// plan one signal for an LDS budget of lds_max points; false if no split fits
+// largest divisor of a 13-smooth L that is <= cap
+static long long largest_divisor_le(long long L, long long cap)
+{
+    std::vector<long long> divs = {1};
+    long long r = L;
+    for (long long p : {2, 3, 5, 7, 11, 13}) {
+        int e = 0;
+        while (r % p == 0) { r /= p; ++e; }
+        const size_t n0 = divs.size();
+        long long pw = 1;
+        for (int k = 1; k <= e; ++k) {
+            pw *= p;
+            for (size_t q = 0; q < n0; ++q)
+                if (divs[q] * pw <= cap) divs.push_back(divs[q] * pw);
+        }
+    }
+    return *std::max_element(divs.begin(), divs.end());
+}
+
+// Rows of a split Bluestein transform are at most this long: the spectrum
+// table is stored in the row layout of this split, whatever the configuration.
+constexpr int kBlueRow = 2048;
+
+// Plan one signal for an LDS budget of lds_max points (>= kBlueRow); false if
+// no split fits.  Split transforms L = L1 L2 take the longest rows L1 that fit
+// (L2 = L / L1 as short as possible), so the column passes move batches of
+// ca = lds_max / L2 columns and their strided accesses are wide.
 static bool plan_task(long long n, int lds_max, RaggedTask &t)
 {
     const long long M = (n % 2 == 0) ? n / 2 : n;
@@ -497,19 +652,14 @@ static bool plan_task(long long n, int lds_max, RaggedTask &t)
     if (L > (1LL << 30)) return false;
     t.L = (int)L;
     t.L1 = t.L2 = t.ca = t.cb = 0;
-    if (L <= lds_max) return true;
-    // L = L1 L2 with both factors in LDS; most balanced split
-    long long bestd = 1LL << 62;
-    for (long long a = 2; a <= lds_max && a < L; ++a) {
-        if (L % a) continue;
-        const long long b = L / a;
-        if (b > lds_max) continue;
-        const long long d = a > b ? a - b : b - a;
-        if (d < bestd) { bestd = d; t.L2 = (int)a; t.L1 = (int)b; }
-    }
-    if (!t.L1) return false;
-    t.ca = rg_batch(t.L2, lds_max);
-    t.cb = rg_batch(t.L1, lds_max);
+    const long long row_cap = t.blue ? kBlueRow : lds_max;
+    if (L <= row_cap) return true;
+    const long long L1 = largest_divisor_le(L, row_cap), L2 = L / L1;
+    if (L2 > lds_max || L1 < 2) return false;
+    t.L1 = (int)L1;
+    t.L2 = (int)L2;
+    t.ca = rg_batch(L2, lds_max);
+    t.cb = rg_batch(L1, lds_max);
     return true;
 }
 
@@ -545,7 +695,7 @@ static int build_tasks(Ragged &R, int threads, int lds_max, Error &err)
     }
     // persistent grid: every resident workgroup, with the per-workgroup scratch
     // held to kScratchCap in total (fewer resident workgroups past that)
-    const long long per_wg = 2 * slot * (long long)sizeof(double2);
+    const long long per_wg = slot * (long long)sizeof(double2);
     long long grid_ll = std::min<long long>((long long)ncu * per, (long long)tasks.size());
     grid_ll = std::max<long long>(1, std::min<long long>(grid_ll, std::max<long long>(ncu, kScratchCap / per_wg)));
     const int grid = (int)grid_ll;
@@ -605,7 +755,7 @@ static int build_spectra(Ragged &R, Error &err)
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, R.device) != hipSuccess || ncu <= 0)
         ncu = 256;
     const int grid = (int)std::min<long long>(ncu, (long long)st.size());
-    if (dt.alloc(st.size() * sizeof(RaggedTask)) || ws.alloc((size_t)grid * 2 * (size_t)slot * sizeof(double2))) {
+    if (dt.alloc(st.size() * sizeof(RaggedTask)) || ws.alloc((size_t)grid * (size_t)slot * sizeof(double2))) {
         err = {E_NOMEM, "spectrum workspace allocation failed"};
         return E_NOMEM;
     }
@@ -689,10 +839,10 @@ void ragged_destroy(Ragged *r) { delete r; }
 int ragged_configure(Ragged &R, int threads, int lds_max, int order, Error &err)
 {
     const bool pow2 = lds_max > 0 && (lds_max & (lds_max - 1)) == 0;
-    if ((threads != 256 && threads != 512 && threads != 1024) || !pow2 || lds_max < 1024 || lds_max > 8192 ||
+    if ((threads != 256 && threads != 512 && threads != 1024) || !pow2 || lds_max < kBlueRow || lds_max > 8192 ||
         lds_max > kPointsPerThread * threads || (order != 0 && order != 1)) {
         err = {E_INVALID, "ragged configuration out of range (threads 256/512/1024; lds_points a power of two in "
-                          "[1024, 8192] and at most 8 per thread; radix order 0/1)"};
+                          "[2048, 8192] and at most 8 per thread; radix order 0/1)"};
         return E_INVALID;
     }
     R.rorder = order;
@@ -788,7 +938,7 @@ int ragged_tune(Ragged &R, const float *x, double2 *y, hipStream_t s, int reps, 
     if (rc != OK) return rc;
     struct Cfg { int t, l, o; };
     const Cfg cands[] = {{1024, 8192, 0}, {1024, 8192, 1}, {1024, 4096, 0}, {512, 4096, 0}, {512, 4096, 1},
-                         {512, 2048, 0}, {256, 2048, 0}, {256, 2048, 1}, {256, 1024, 0}};
+                         {512, 2048, 0}, {512, 2048, 1}, {256, 2048, 0}, {256, 2048, 1}};
     hipEvent_t a = nullptr, b = nullptr;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
         (void)hipGetLastError();

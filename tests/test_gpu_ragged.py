@@ -22,10 +22,13 @@ from impulse_amd import ragged, synth
 pytestmark = pytest.mark.gpu
 
 CONFIGS = [(1024, 8192, 0), (1024, 8192, 1), (1024, 4096, 0), (512, 4096, 0), (512, 4096, 1), (512, 2048, 0),
-           (256, 2048, 0), (256, 2048, 1), (256, 1024, 0)]
+           (512, 2048, 1), (256, 2048, 0), (256, 2048, 1)]
 
-COVER = [1, 2, 3, 4, 5, 6, 7, 8, 9, 16, 26, 64, 97, 127, 128, 194, 1000, 1009, 2018, 4095, 4096, 8190, 8192,
-         16384, 16386, 17000, 20000, 26013, 65536, 100003, 131072, 200006, 262143, 262144]
+# every branch: LDS-resident (smooth / Bluestein, odd / even), split smooth (even with the paired
+# output, odd), split Bluestein with short columns (3001, 3022) and long ones, n = 1, 2, 3
+COVER = [1, 2, 3, 4, 5, 6, 7, 8, 9, 16, 26, 64, 97, 127, 128, 194, 1000, 1009, 2018, 3001, 3022, 4095, 4096,
+         8190, 8192, 15625, 16384, 16386, 17000, 19683, 20000, 26013, 65536, 100003, 131072, 200006, 262143,
+         262144]
 
 
 @pytest.fixture(scope="module", autouse=True)

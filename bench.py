@@ -198,7 +198,7 @@ def ragged_form(torch, reps, loop_sample=600):
         "ms": round(ms, 3), "gflops": round(gf, 1), "compulsory_GBs": round(comp / (ms * 1e-3) / 1e9, 1),
         "hbm_roofline_frac": round(comp / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "launches": 1,
         "dtype": "f32 in, f64 arithmetic, c128 out",
-        "tuned": {k: info[k] for k in ("threads", "lds_points", "radix_order", "grid")},
+        "tuned": {k: info[k] for k in ("threads", "lds_points", "radix_order", "waves_per_simd", "grid")},
         "tune_best_ms": round(tune_ms, 3), "scratch_bytes": info["scratch_bytes"],
         "fp64_plan_loop": {"sample": loop_sample, "gflops": round(loop_gflops, 2),
                            "projected_ms": round(flops / (loop_gflops * 1e9) * 1e3, 1),

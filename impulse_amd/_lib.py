@@ -75,9 +75,9 @@ def _sig(L):
     L.impulse_fft_ragged_create.argtypes = [ctypes.POINTER(_vp), sz, _szp, _szp, _szp, i]
     L.impulse_fft_ragged_execute.argtypes = [_vp, _vp, _vp, d, _vp]
     L.impulse_fft_ragged_tune.argtypes = [_vp, _vp, _vp, _vp, i, ctypes.POINTER(d)]
-    L.impulse_fft_ragged_configure.argtypes = [_vp, i, i, i]
+    L.impulse_fft_ragged_configure.argtypes = [_vp, i, i, i, i]
     ip = ctypes.POINTER(i)
-    L.impulse_fft_ragged_info.argtypes = [_vp, ip, ip, ip, ip, ctypes.POINTER(d), _szp]
+    L.impulse_fft_ragged_info.argtypes = [_vp, ip, ip, ip, ip, ip, ctypes.POINTER(d), _szp]
     L.impulse_fft_ragged_destroy.argtypes = [_vp]
     L.impulse_fft_mfma_create.argtypes = [ctypes.POINTER(_vp), sz, sz, i]
     L.impulse_fft_mfma_execute.argtypes = [_vp, _vp, _vp, i, d, _vp]

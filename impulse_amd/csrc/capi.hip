@@ -387,19 +387,20 @@ int impulse_fft_ragged_tune(impulse_fft_ragged batch, const float *in, void *out
     return rc == ifft::OK ? rc : fail(err);
 }
 
-int impulse_fft_ragged_configure(impulse_fft_ragged batch, int threads, int lds_points, int radix_order)
+int impulse_fft_ragged_configure(impulse_fft_ragged batch, int threads, int lds_points, int radix_order,
+                                 int waves_per_simd)
 {
     if (!batch || !batch->r) return fail(ifft::E_INVALID, "batch is NULL");
     ifft::Error err{0, ""};
-    int rc = ifft::ragged_configure(*batch->r, threads, lds_points, radix_order, err);
+    int rc = ifft::ragged_configure(*batch->r, threads, lds_points, radix_order, waves_per_simd, err);
     return rc == ifft::OK ? rc : fail(err);
 }
 
-int impulse_fft_ragged_info(impulse_fft_ragged batch, int *threads, int *lds_points, int *radix_order, int *grid,
-                            double *flops, size_t *scratch_bytes)
+int impulse_fft_ragged_info(impulse_fft_ragged batch, int *threads, int *lds_points, int *radix_order,
+                            int *waves_per_simd, int *grid, double *flops, size_t *scratch_bytes)
 {
     if (!batch || !batch->r) return fail(ifft::E_INVALID, "batch is NULL");
-    ifft::ragged_info(*batch->r, threads, lds_points, radix_order, grid, flops, scratch_bytes);
+    ifft::ragged_info(*batch->r, threads, lds_points, radix_order, waves_per_simd, grid, flops, scratch_bytes);
     return ifft::OK;
 }
 

@@ -433,10 +433,11 @@ __global__ void k_ragged_reset(int *counter)
     if (threadIdx.x == 0) counter[0] = 0;
 }
 
-// One instance per workgroup size: a bound of 1024 threads leaves 128 VGPRs
-// per lane, 512 and 256 leave more registers at lower occupancy.
-template <int MAXT>
-__global__ void __launch_bounds__(MAXT) k_ragged(const RaggedParams P)
+// One instance per workgroup size and register budget: WPE = 4 caps the
+// registers at 128 per lane (four waves per SIMD, some spills), WPE = 0 lets
+// the compiler take what the workgroup size allows (1024 threads: 128).
+template <int MAXT, int WPE>
+__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ? WPE : 1))) k_ragged(const RaggedParams P)
 {
     extern __shared__ double2 lds[];
     __shared__ int s_task;
@@ -564,7 +565,7 @@ struct Ragged {
     std::vector<int> order;                   // queue order (decreasing cost)
     int forward = 1;
     // configuration
-    int threads = 0, lds_max = 0, rorder = 0;
+    int threads = 0, lds_max = 0, rorder = 0, waves = 0;
     // derived for the configuration
     std::vector<RaggedTask> tasks;
     long long slot = 0;
@@ -591,17 +592,22 @@ struct Ragged {
     ~Ragged() { drop_graph(); }
 };
 
-static const void *ragged_fn(int threads)
+static const void *ragged_fn(int threads, int waves)
 {
-    return threads == 256 ? (const void *)k_ragged<256>
-                          : threads == 512 ? (const void *)k_ragged<512> : (const void *)k_ragged<1024>;
+    if (threads == 256) return waves == 4 ? (const void *)k_ragged<256, 4> : (const void *)k_ragged<256, 0>;
+    if (threads == 512) return waves == 4 ? (const void *)k_ragged<512, 4> : (const void *)k_ragged<512, 0>;
+    return (const void *)k_ragged<1024, 0>;
 }
 
-static hipError_t launch_ragged(const RaggedParams &P, int threads, unsigned grid, size_t lds, hipStream_t s)
+static hipError_t launch_ragged(const RaggedParams &P, int threads, int waves, unsigned grid, size_t lds,
+                                hipStream_t s)
 {
-    if (threads == 256) hipLaunchKernelGGL(k_ragged<256>, dim3(grid), dim3(256), lds, s, P);
-    else if (threads == 512) hipLaunchKernelGGL(k_ragged<512>, dim3(grid), dim3(512), lds, s, P);
-    else hipLaunchKernelGGL(k_ragged<1024>, dim3(grid), dim3(1024), lds, s, P);
+    const dim3 g(grid), b(threads);
+    if (threads == 256 && waves == 4) hipLaunchKernelGGL((k_ragged<256, 4>), g, b, lds, s, P);
+    else if (threads == 256) hipLaunchKernelGGL((k_ragged<256, 0>), g, b, lds, s, P);
+    else if (threads == 512 && waves == 4) hipLaunchKernelGGL((k_ragged<512, 4>), g, b, lds, s, P);
+    else if (threads == 512) hipLaunchKernelGGL((k_ragged<512, 0>), g, b, lds, s, P);
+    else hipLaunchKernelGGL((k_ragged<1024, 0>), g, b, lds, s, P);
     return hipGetLastError();
 }
 
@@ -665,7 +671,7 @@ static bool plan_task(long long n, int lds_max, RaggedTask &t)
 
 constexpr long long kScratchCap = 16LL << 30;
 
-static int build_tasks(Ragged &R, int threads, int lds_max, Error &err)
+static int build_tasks(Ragged &R, int threads, int lds_max, int waves, Error &err)
 {
     std::vector<RaggedTask> tasks(R.order.size());
     long long slot = 1;
@@ -688,7 +694,7 @@ static int build_tasks(Ragged &R, int threads, int lds_max, Error &err)
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, R.device) != hipSuccess || ncu <= 0)
         ncu = 256;
     const size_t lds = (size_t)lds_max * sizeof(double2);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, ragged_fn(threads), threads, lds) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, ragged_fn(threads, waves), threads, lds) != hipSuccess ||
         per <= 0) {
         (void)hipGetLastError();
         per = 1;
@@ -715,6 +721,7 @@ static int build_tasks(Ragged &R, int threads, int lds_max, Error &err)
     R.grid = grid;
     R.threads = threads;
     R.lds_max = lds_max;
+    R.waves = waves;
     R.drop_graph();
     return OK;
 }
@@ -775,7 +782,7 @@ static int build_spectra(Ragged &R, Error &err)
         hipLaunchKernelGGL(k_ragged_reset, dim3(1), dim3(64), 0, 0, (int *)R.counter.p);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = launch_ragged(P, 1024, grid, (size_t)8192 * sizeof(double2), 0);
+    if (e == hipSuccess) e = launch_ragged(P, 1024, 0, grid, (size_t)8192 * sizeof(double2), 0);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -819,8 +826,9 @@ int ragged_create(Ragged **out, size_t count, const size_t *lengths, const size_
         err = {E_NOMEM, "ragged workspace allocation failed"};
         return E_NOMEM;
     }
-    for (int t : {256, 512, 1024})
-        if (hipFuncSetAttribute(ragged_fn(t), hipFuncAttributeMaxDynamicSharedMemorySize,
+    for (int v = 0; v < 5; ++v)
+        if (hipFuncSetAttribute(ragged_fn(v < 2 ? 256 : v < 4 ? 512 : 1024, (v & 1) ? 4 : 0),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)(8192 * sizeof(double2))) != hipSuccess) {
             (void)hipGetLastError();
             err = {E_HIP, "cannot raise the LDS limit of the ragged kernel"};
@@ -828,7 +836,7 @@ int ragged_create(Ragged **out, size_t count, const size_t *lengths, const size_
         }
     rc = build_spectra(*R, err);
     if (rc != OK) return rc;
-    rc = build_tasks(*R, 1024, 8192, err);
+    rc = build_tasks(*R, 1024, 8192, 0, err);
     if (rc != OK) return rc;
     *out = R.release();
     return OK;
@@ -836,21 +844,22 @@ int ragged_create(Ragged **out, size_t count, const size_t *lengths, const size_
 
 void ragged_destroy(Ragged *r) { delete r; }
 
-int ragged_configure(Ragged &R, int threads, int lds_max, int order, Error &err)
+int ragged_configure(Ragged &R, int threads, int lds_max, int order, int waves, Error &err)
 {
     const bool pow2 = lds_max > 0 && (lds_max & (lds_max - 1)) == 0;
     if ((threads != 256 && threads != 512 && threads != 1024) || !pow2 || lds_max < kBlueRow || lds_max > 8192 ||
-        lds_max > kPointsPerThread * threads || (order != 0 && order != 1)) {
+        lds_max > kPointsPerThread * threads || (order != 0 && order != 1) || (waves != 0 && waves != 4)) {
         err = {E_INVALID, "ragged configuration out of range (threads 256/512/1024; lds_points a power of two in "
-                          "[2048, 8192] and at most 8 per thread; radix order 0/1)"};
+                          "[2048, 8192] and at most 8 per thread; radix order 0/1; waves per SIMD 0 or 4)"};
         return E_INVALID;
     }
+    if (threads == 1024) waves = 0;   // 1024 threads always leave 128 registers
     R.rorder = order;
-    if (threads == R.threads && lds_max == R.lds_max && !R.tasks.empty()) {
+    if (threads == R.threads && lds_max == R.lds_max && waves == R.waves && !R.tasks.empty()) {
         R.drop_graph();
         return OK;
     }
-    return build_tasks(R, threads, lds_max, err);
+    return build_tasks(R, threads, lds_max, waves, err);
 }
 
 static hipError_t ragged_launch(Ragged &R, const float *x, double2 *y, double fct, hipStream_t s)
@@ -871,7 +880,7 @@ static hipError_t ragged_launch(Ragged &R, const float *x, double2 *y, double fc
     hipLaunchKernelGGL(k_ragged_reset, dim3(1), dim3(64), 0, s, (int *)R.counter.p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return launch_ragged(P, R.threads, (unsigned)R.grid, (size_t)R.lds_max * sizeof(double2), s);
+    return launch_ragged(P, R.threads, R.waves, (unsigned)R.grid, (size_t)R.lds_max * sizeof(double2), s);
 }
 
 static int check_ptrs(const Ragged &R, const float *x, double2 *y, Error &err)
@@ -936,9 +945,10 @@ int ragged_tune(Ragged &R, const float *x, double2 *y, hipStream_t s, int reps, 
 {
     int rc = check_ptrs(R, x, y, err);
     if (rc != OK) return rc;
-    struct Cfg { int t, l, o; };
-    const Cfg cands[] = {{1024, 8192, 0}, {1024, 8192, 1}, {1024, 4096, 0}, {512, 4096, 0}, {512, 4096, 1},
-                         {512, 2048, 0}, {512, 2048, 1}, {256, 2048, 0}, {256, 2048, 1}};
+    struct Cfg { int t, l, o, w; };
+    const Cfg cands[] = {{1024, 8192, 0, 0}, {1024, 8192, 1, 0}, {1024, 4096, 0, 0}, {512, 4096, 0, 0},
+                         {512, 4096, 1, 0}, {512, 4096, 1, 4}, {512, 2048, 1, 0}, {512, 2048, 1, 4},
+                         {256, 2048, 0, 0}, {256, 2048, 1, 0}, {256, 2048, 0, 4}, {256, 2048, 1, 4}};
     hipEvent_t a = nullptr, b = nullptr;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
         (void)hipGetLastError();
@@ -947,12 +957,12 @@ int ragged_tune(Ragged &R, const float *x, double2 *y, hipStream_t s, int reps, 
         return E_HIP;
     }
     double best = 1e300;
-    Cfg bestc = {R.threads, R.lds_max, R.rorder};
+    Cfg bestc = {R.threads, R.lds_max, R.rorder, R.waves};
     reps = std::max(1, reps);
     rc = OK;
     for (const Cfg &c : cands) {
         Error e2{0, ""};
-        if (ragged_configure(R, c.t, c.l, c.o, e2) != OK) {
+        if (ragged_configure(R, c.t, c.l, c.o, c.w, e2) != OK) {
             if (e2.code == E_UNSUPPORTED) continue;   // this batch has a length the configuration cannot split
             err = e2;
             rc = e2.code;
@@ -977,13 +987,14 @@ int ragged_tune(Ragged &R, const float *x, double2 *y, hipStream_t s, int reps, 
     (void)hipEventDestroy(b);
     if (rc != OK) return rc;
     if (best_ms) *best_ms = best;
-    return ragged_configure(R, bestc.t, bestc.l, bestc.o, err);
+    return ragged_configure(R, bestc.t, bestc.l, bestc.o, bestc.w, err);
 }
 
-void ragged_info(const Ragged &R, int *threads, int *lds_max, int *order, int *grid, double *flops,
+void ragged_info(const Ragged &R, int *threads, int *lds_max, int *order, int *waves, int *grid, double *flops,
                  size_t *scratch_bytes)
 {
     if (threads) *threads = R.threads;
+    if (waves) *waves = R.waves;
     if (lds_max) *lds_max = R.lds_max;
     if (order) *order = R.rorder;
     if (grid) *grid = R.grid;

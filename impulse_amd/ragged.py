@@ -80,16 +80,17 @@ class RaggedBatch:
                                             ctypes.byref(ms)))
         return ms.value
 
-    def configure(self, threads, lds_points, radix_order=0):
-        check(lib().impulse_fft_ragged_configure(self.handle, int(threads), int(lds_points), int(radix_order)))
+    def configure(self, threads, lds_points, radix_order=0, waves_per_simd=0):
+        check(lib().impulse_fft_ragged_configure(self.handle, int(threads), int(lds_points), int(radix_order),
+                                                 int(waves_per_simd)))
 
     def info(self):
-        t, l, o, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        t, l, o, w, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         f, sb = ctypes.c_double(), ctypes.c_size_t()
         check(lib().impulse_fft_ragged_info(self.handle, ctypes.byref(t), ctypes.byref(l), ctypes.byref(o),
-                                            ctypes.byref(g), ctypes.byref(f), ctypes.byref(sb)))
-        return {"threads": t.value, "lds_points": l.value, "radix_order": o.value, "grid": g.value,
-                "flops": f.value, "scratch_bytes": sb.value}
+                                            ctypes.byref(w), ctypes.byref(g), ctypes.byref(f), ctypes.byref(sb)))
+        return {"threads": t.value, "lds_points": l.value, "radix_order": o.value, "waves_per_simd": w.value,
+                "grid": g.value, "flops": f.value, "scratch_bytes": sb.value}
 
     def output_views(self, y):
         """Per-signal views of the output buffer."""

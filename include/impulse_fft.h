@@ -183,19 +183,22 @@ int impulse_fft_ragged_create(impulse_fft_ragged *batch, size_t count, const siz
                               const size_t *in_offsets, const size_t *out_offsets, int forward);
 int impulse_fft_ragged_execute(impulse_fft_ragged batch, const float *in, void *out, double fct, void *stream);
 /* Autotuner: runs every candidate configuration (workgroup size, LDS budget
- * per transform, radix order) `reps` times on these buffers, keeps the
- * fastest and reports its time per execute (ms).  Synchronizes `stream`;
- * `out` is overwritten. */
+ * per transform, radix order, register budget) `reps` times on these buffers,
+ * keeps the fastest and reports its time per execute (ms).  Synchronizes
+ * `stream`; `out` is overwritten. */
 int impulse_fft_ragged_tune(impulse_fft_ragged batch, const float *in, void *out, void *stream, int reps,
                             double *best_ms);
 /* Pick a configuration by hand: threads 256/512/1024, lds_points
- * 2048/4096/8192 (longest transform kept in LDS), radix_order 0 (powers of
- * two first) or 1 (odd radices first). */
-int impulse_fft_ragged_configure(impulse_fft_ragged batch, int threads, int lds_points, int radix_order);
+ * 2048/4096/8192 (longest transform kept in LDS; at most 8 per thread),
+ * radix_order 0 (powers of two first) or 1 (odd radices first),
+ * waves_per_simd 0 (registers as the workgroup size allows) or 4 (at most 128
+ * registers per lane: more resident workgroups, some spills). */
+int impulse_fft_ragged_configure(impulse_fft_ragged batch, int threads, int lds_points, int radix_order,
+                                 int waves_per_simd);
 /* Current configuration, persistent grid size, nominal flops of one execute
  * (2.5 n log2 n per signal) and device scratch bytes.  Any pointer may be NULL. */
-int impulse_fft_ragged_info(impulse_fft_ragged batch, int *threads, int *lds_points, int *radix_order, int *grid,
-                            double *flops, size_t *scratch_bytes);
+int impulse_fft_ragged_info(impulse_fft_ragged batch, int *threads, int *lds_points, int *radix_order,
+                            int *waves_per_simd, int *grid, double *flops, size_t *scratch_bytes);
 int impulse_fft_ragged_destroy(impulse_fft_ragged batch);
 
 /* ------------------------------------------------------------------------- */

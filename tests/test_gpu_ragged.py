@@ -21,8 +21,9 @@ from impulse_amd import ragged, synth
 
 pytestmark = pytest.mark.gpu
 
-CONFIGS = [(1024, 8192, 0), (1024, 8192, 1), (1024, 4096, 0), (512, 4096, 0), (512, 4096, 1), (512, 2048, 0),
-           (512, 2048, 1), (256, 2048, 0), (256, 2048, 1)]
+CONFIGS = [(1024, 8192, 0, 0), (1024, 8192, 1, 0), (1024, 4096, 0, 0), (512, 4096, 0, 0), (512, 4096, 1, 0),
+           (512, 4096, 1, 4), (512, 2048, 1, 0), (512, 2048, 1, 4), (256, 2048, 0, 0), (256, 2048, 1, 0),
+           (256, 2048, 0, 4), (256, 2048, 1, 4)]
 
 # every branch: LDS-resident (smooth / Bluestein, odd / even), split smooth (even with the paired
 # output, odd), split Bluestein with short columns (3001, 3022) and long ones, n = 1, 2, 3
@@ -122,7 +123,7 @@ def test_tune_picks_a_valid_config():
     ms = b.tune(xd, yd, reps=1)
     info = b.info()
     print(f"tuned: {info} {ms:.3f} ms")
-    assert ms > 0 and (info["threads"], info["lds_points"], info["radix_order"]) in CONFIGS
+    assert ms > 0 and (info["threads"], info["lds_points"], info["radix_order"], info["waves_per_simd"]) in CONFIGS
     yd.zero_()
     b.execute(xd, yd)
     torch.cuda.synchronize()

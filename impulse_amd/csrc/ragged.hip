@@ -5,8 +5,9 @@
 // claim; the host orders the queue by decreasing cost, so the tail of the
 // launch is short signals).  For a real signal of length n the workgroup forms
 // a complex signal z of length M (M = n/2, z[m] = x[2m] + i x[2m+1] for even n;
-// M = n with zero imaginary part for odd n), transforms it, and writes the
-// n/2+1 bins.  The host plans each signal (task table):
+// for odd n, M = n and two signals of the same length share one transform as
+// z = x + i x2, separated as X = (Z[k] + conj Z[n-k]) / 2 and
+// X2 = (Z[k] - conj Z[n-k]) / 2i), transforms it, and writes the n/2+1 bins.  The host plans each signal (task table):
 //   M 13-smooth, fits the LDS budget : Stockham stages in LDS;
 //   M 13-smooth, longer              : four-step M = L1 L2 through the
 //                                      workgroup's global scratch (batches of
@@ -50,6 +51,10 @@ struct RaggedTask {
     int ca, cb;          // columns per LDS batch in pass A (length L2) / rows in pass B (length L1)
     int blue;            // Bluestein
     long long spec_off;  // Bluestein: first point of the chirp kernel's spectrum in the table
+    long long in_off2;   // pair: the second odd signal of the same length (z = x + i x2)
+    long long out_off2;
+    int pair;
+    int pad_;
 };
 
 struct RaggedParams {
@@ -364,18 +369,21 @@ __device__ void col_pass_out(EMIT emit, const RaggedTask &T, double s, double2 *
 }
 
 // Output of a four-step transform left in row layout (tmp[k2 L1 + k1] =
-// Z[k2 + L2 k1]) in tiles of 16 x 64, transposed through LDS so that both
+// Z[k2 + L2 k1]) in tiles of 16 x 32, transposed through LDS so that both
 // the row reads and the natural-order writes are contiguous.  Even n: the
 // half-length split X[k] = E + W_n^k O with Z[M-k] read from the mirrored
 // tile, and X[M] at the end; odd n: X[k] = Z[k], k <= n/2.
-constexpr int kTR = 16, kTC = 64;
-__device__ void post_tiles(const RaggedTask &T, double2 *lds, const double2 *tmp, double2 *y, bool half, double f,
-                           double fo)
+constexpr int kTR = 16, kTC = 32;
+static_assert(2 * kTC * (kTR + 1) <= 2048, "two padded output tiles must fit the smallest LDS budget");
+__device__ void post_tiles(const RaggedTask &T, double2 *lds, const double2 *tmp, double2 *y, double2 *y2,
+                           bool half, double f, double fo)
 {
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int L1 = T.L1, L2 = T.L2, n = T.n;
     const long long M = T.L;
+    const bool pair = y2 != nullptr;
     const long long limit = half ? M - 1 : n / 2;
+    double2 *lds2 = lds + kTC * (kTR + 1);   // second tile (pairs)
     for (int r = 0; r < L2; r += kTR)
         for (int c = 0; c < L1; c += kTC) {
             for (int e = tid; e < kTR * kTC; e += nthr) {
@@ -385,13 +393,19 @@ __device__ void post_tiles(const RaggedTask &T, double2 *lds, const double2 *tmp
                 if (k > limit) continue;
                 const double2 zk = tmp[(long long)k2 * L1 + k1];
                 double2 v = zk;
-                if (half) {
+                if (half || pair) {
+                    // Z[(M - k) mod M]: row L2 - k2 reversed, or row 0 reversed
                     const long long pm = k2 == 0 ? (long long)((L1 - k1) % L1)
                                                  : (long long)(L2 - k2) * L1 + (L1 - 1 - k1);
                     const double2 zm = tmp[pm];
-                    const double2 w = twid32((int)k, n, -1.0);
                     const double2 e1 = cscale(cadd(zk, cconj(zm)), 0.5), d1 = csub(zk, cconj(zm));
-                    v = cadd(e1, cmul(make_double2(0.5 * d1.y, -0.5 * d1.x), w));
+                    const double2 o1 = make_double2(0.5 * d1.y, -0.5 * d1.x);
+                    if (half) {
+                        v = cadd(e1, cmul(o1, twid32((int)k, n, -1.0)));
+                    } else {
+                        v = e1;
+                        lds2[u * (kTR + 1) + t] = make_double2(f * o1.x, fo * o1.y);
+                    }
                 }
                 lds[u * (kTR + 1) + t] = make_double2(f * v.x, fo * v.y);
             }
@@ -400,7 +414,10 @@ __device__ void post_tiles(const RaggedTask &T, double2 *lds, const double2 *tmp
                 const int t = e % kTR, u = e / kTR, k2 = r + t, k1 = c + u;
                 if (k2 >= L2 || k1 >= L1) continue;
                 const long long k = k2 + (long long)L2 * k1;
-                if (k <= limit) y[k] = lds[u * (kTR + 1) + t];
+                if (k <= limit) {
+                    y[k] = lds[u * (kTR + 1) + t];
+                    if (pair) y2[k] = lds2[u * (kTR + 1) + t];
+                }
             }
             __syncthreads();
         }
@@ -451,11 +468,14 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
         if (task >= P.ntasks) break;
         const RaggedTask T = P.tasks[task];
         const long long n = T.n, M = T.M;
-        const bool half = (n % 2) == 0;
+        const bool half = (n % 2) == 0, pair = T.pair != 0;
         const float *x = P.x + T.in_off;
+        const float *x2 = P.x + (pair ? T.in_off2 : T.in_off);
         double2 *y = P.y + T.out_off;
+        double2 *y2 = pair ? P.y + T.out_off2 : nullptr;
         auto zin = [&](long long m) -> double2 {
-            return half ? make_double2((double)x[2 * m], (double)x[2 * m + 1]) : make_double2((double)x[m], 0.0);
+            return half ? make_double2((double)x[2 * m], (double)x[2 * m + 1])
+                        : make_double2((double)x[m], pair ? (double)x2[m] : 0.0);
         };
         const double inv2m = 0.5 / (double)M;
         const double f = P.fct, fo = P.fct * P.osign;
@@ -463,7 +483,7 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
             // chirp kernel b[m] = conj(w[m]) for 0 <= m < K, conj(w[n2 - m]) for n2 - m < M
             // (indices mod n2, zero between); its unnormalized spectrum goes to the table
             // (row layout when the transform is split)
-            const long long n2 = T.L, K = half ? M : n / 2 + 1;
+            const long long n2 = T.L, K = (half || pair) ? M : n / 2 + 1;
             double2 *B = P.spec + T.spec_off;
             auto bload = [&](long long m) -> double2 {
                 if (m < K) return cconj(chirp(m, M, inv2m));
@@ -483,7 +503,7 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
                 // columns, rows in place, then the transposing output pass
                 col_pass(zin, T, -1.0, lds, S0, P.order);
                 row_pass(T, -1.0, lds, S0, P.order, nullptr);
-                post_tiles(T, lds, S0, y, half, f, fo);
+                post_tiles(T, lds, S0, y, y2, half, f, fo);
                 __syncthreads();
                 continue;
             }
@@ -502,7 +522,7 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
             } else {
                 col_pass(aload, T, -1.0, lds, S0, P.order);
                 row_pass(T, -1.0, lds, S0, P.order, B);
-                if (half) {
+                if (half || pair) {
                     // c in natural order back into S0 (each column rewrites the points it read)
                     auto keep = [&](long long k, double2 v) { S0[k] = v; };
                     col_pass_out(keep, T, 1.0, lds, S0, P.order);
@@ -545,6 +565,14 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
                     y[M - k] = make_double2(f * x2.x, fo * x2.y);
                 }
             }
+        } else if (pair) {
+            // X[k] = (Z[k] + conj Z[n-k]) / 2, X2[k] = (Z[k] - conj Z[n-k]) / 2i
+            for (long long k = tid; k <= n / 2; k += nthr) {
+                const double2 zk = Z(k), zm = Z((n - k) % n);
+                const double2 e1 = cscale(cadd(zk, cconj(zm)), 0.5), d1 = csub(zk, cconj(zm));
+                y[k] = make_double2(f * e1.x, fo * e1.y);
+                y2[k] = make_double2(f * 0.5 * d1.y, -fo * 0.5 * d1.x);
+            }
         } else {
             for (long long k = tid; k <= n / 2; k += nthr) {
                 const double2 v = Z(k);
@@ -562,7 +590,7 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
 struct Ragged {
     int device = 0;
     std::vector<size_t> n, in_off, out_off;   // as given
-    std::vector<int> order;                   // queue order (decreasing cost)
+    std::vector<std::pair<int, int>> order;   // queue of (signal, paired signal or -1), decreasing cost
     int forward = 1;
     // configuration
     int threads = 0, lds_max = 0, rorder = 0, waves = 0;
@@ -572,7 +600,7 @@ struct Ragged {
     int grid = 0;
     DevBuf dtasks, counter, scratch;
     // chirp-kernel spectra, one per distinct Bluestein length
-    std::vector<std::pair<long long, long long>> spec_index;   // (n, offset), sorted by n
+    std::vector<std::pair<long long, long long>> spec_index;   // (2 n + pair, offset), sorted
     DevBuf spec;
     double flops = 0.0;
     size_t in_elems = 0, out_elems = 0;
@@ -612,11 +640,11 @@ static hipError_t launch_ragged(const RaggedParams &P, int threads, int waves, u
 }
 
 // transform length of a signal: M, or the Bluestein length n2 >= M + K - 1
-static long long transform_len(long long n)
+static long long transform_len(long long n, bool pair)
 {
     const long long M = (n % 2 == 0) ? n / 2 : n;
     if (rg_smooth13(M)) return M;
-    const long long K = (n % 2 == 0) ? M : n / 2 + 1;
+    const long long K = (n % 2 == 0 || pair) ? M : n / 2 + 1;
     return rg_good7(M + K - 1);
 }
 
@@ -647,14 +675,17 @@ constexpr int kBlueRow = 2048;
 // no split fits.  Split transforms L = L1 L2 take the longest rows L1 that fit
 // (L2 = L / L1 as short as possible), so the column passes move batches of
 // ca = lds_max / L2 columns and their strided accesses are wide.
-static bool plan_task(long long n, int lds_max, RaggedTask &t)
+static bool plan_task(long long n, bool pair, int lds_max, RaggedTask &t)
 {
     const long long M = (n % 2 == 0) ? n / 2 : n;
     t.n = (int)n;
     t.M = (int)M;
     t.blue = rg_smooth13(M) ? 0 : 1;
     t.spec_off = 0;
-    const long long L = transform_len(n);
+    t.pair = pair ? 1 : 0;
+    t.pad_ = 0;
+    t.in_off2 = t.out_off2 = 0;
+    const long long L = transform_len(n, pair);
     if (L > (1LL << 30)) return false;
     t.L = (int)L;
     t.L1 = t.L2 = t.ca = t.cb = 0;
@@ -676,18 +707,23 @@ static int build_tasks(Ragged &R, int threads, int lds_max, int waves, Error &er
     std::vector<RaggedTask> tasks(R.order.size());
     long long slot = 1;
     for (size_t q = 0; q < R.order.size(); ++q) {
-        const size_t i = (size_t)R.order[q];
+        const size_t i = (size_t)R.order[q].first;
+        const int j = R.order[q].second;
         RaggedTask &t = tasks[q];
-        if (!plan_task((long long)R.n[i], lds_max, t)) {
+        if (!plan_task((long long)R.n[i], j >= 0, lds_max, t)) {
             err = {E_UNSUPPORTED, "ragged signal of length " + std::to_string(R.n[i]) +
                                       " has no four-step split for this configuration"};
             return E_UNSUPPORTED;
         }
         t.in_off = (long long)R.in_off[i];
         t.out_off = (long long)R.out_off[i];
+        if (j >= 0) {
+            t.in_off2 = (long long)R.in_off[(size_t)j];
+            t.out_off2 = (long long)R.out_off[(size_t)j];
+        }
         if (t.blue)
             t.spec_off = std::lower_bound(R.spec_index.begin(), R.spec_index.end(),
-                                          std::make_pair((long long)R.n[i], -1LL))->second;
+                                          std::make_pair(2 * (long long)R.n[i] + (j >= 0), -1LL))->second;
         if (t.L1 || t.blue) slot = std::max<long long>(slot, t.L);
     }
     int ncu = 0, per = 0;
@@ -730,11 +766,11 @@ static int build_tasks(Ragged &R, int threads, int lds_max, int waves, Error &er
 // persistent kernel in spectrum mode.
 static int build_spectra(Ragged &R, Error &err)
 {
-    std::vector<long long> ns;
-    for (size_t i = 0; i < R.n.size(); ++i) {
-        const long long n = (long long)R.n[i];
+    std::vector<long long> ns;   // keys 2 n + pair
+    for (const auto &u : R.order) {
+        const long long n = (long long)R.n[(size_t)u.first];
         const long long M = (n % 2 == 0) ? n / 2 : n;
-        if (!rg_smooth13(M)) ns.push_back(n);
+        if (!rg_smooth13(M)) ns.push_back(2 * n + (u.second >= 0 ? 1 : 0));
     }
     std::sort(ns.begin(), ns.end());
     ns.erase(std::unique(ns.begin(), ns.end()), ns.end());
@@ -743,9 +779,9 @@ static int build_spectra(Ragged &R, Error &err)
     std::vector<RaggedTask> st(ns.size());
     long long off = 0, slot = 1;
     for (size_t q = 0; q < ns.size(); ++q) {
-        if (!plan_task(ns[q], 8192, st[q])) {
+        if (!plan_task(ns[q] / 2, (ns[q] & 1) != 0, 8192, st[q])) {
             err = {E_UNSUPPORTED, "no four-step split for the Bluestein length of a signal of length " +
-                                      std::to_string(ns[q])};
+                                      std::to_string(ns[q] / 2)};
             return E_UNSUPPORTED;
         }
         st[q].in_off = st[q].out_off = 0;
@@ -807,22 +843,46 @@ int ragged_create(Ragged **out, size_t count, const size_t *lengths, const size_
     R->n.assign(lengths, lengths + count);
     R->in_off.assign(in_off, in_off + count);
     R->out_off.assign(out_off, out_off + count);
-    std::vector<std::pair<double, int>> cost(count);
     for (size_t i = 0; i < count; ++i) {
         const long long n = (long long)lengths[i];
         if (n < 1 || n > (1LL << 28)) { err = {E_INVALID, "ragged lengths must be in [1, 2^28]"}; return E_INVALID; }
-        const long long M = (n % 2 == 0) ? n / 2 : n;
-        const long long L = transform_len(n);
-        const double c = (rg_smooth13(M) ? 1.0 : 2.0) * (double)L * std::log2((double)L + 1.0);
-        cost[i] = {-c, (int)i};
         if (n > 1) R->flops += 2.5 * (double)n * std::log2((double)n);
         R->in_elems = std::max<size_t>(R->in_elems, in_off[i] + (size_t)n);
         R->out_elems = std::max<size_t>(R->out_elems, out_off[i] + (size_t)n / 2 + 1);
     }
+    // Odd lengths > 1 pair up (two real signals, one complex transform) unless
+    // the pair's transform would be longer than the longest unpaired one of the
+    // batch: each task runs on one workgroup, so the longest task bounds the launch.
+    long long longest = 0;
+    for (size_t i = 0; i < count; ++i) longest = std::max(longest, transform_len((long long)lengths[i], false));
+    std::vector<std::pair<int, int>> units;
+    {
+        std::vector<std::pair<long long, int>> odd;
+        for (size_t i = 0; i < count; ++i)
+            if (lengths[i] % 2 == 1 && lengths[i] > 1) odd.push_back({(long long)lengths[i], (int)i});
+            else units.push_back({(int)i, -1});
+        std::stable_sort(odd.begin(), odd.end());
+        for (size_t q = 0; q < odd.size(); ++q) {
+            if (q + 1 < odd.size() && odd[q + 1].first == odd[q].first &&
+                transform_len(odd[q].first, true) <= longest) {
+                units.push_back({odd[q].second, odd[q + 1].second});
+                ++q;
+            } else {
+                units.push_back({odd[q].second, -1});
+            }
+        }
+    }
+    std::vector<std::pair<double, size_t>> cost(units.size());
+    for (size_t u = 0; u < units.size(); ++u) {
+        const long long n = (long long)lengths[(size_t)units[u].first];
+        const long long M = (n % 2 == 0) ? n / 2 : n;
+        const long long L = transform_len(n, units[u].second >= 0);
+        cost[u] = {-(rg_smooth13(M) ? 1.0 : 2.0) * (double)L * std::log2((double)L + 1.0), u};
+    }
     std::stable_sort(cost.begin(), cost.end());
-    R->order.resize(count);
-    for (size_t q = 0; q < count; ++q) R->order[q] = cost[q].second;
-    if (R->dtasks.alloc(count * sizeof(RaggedTask)) || R->counter.alloc(256)) {
+    R->order.resize(units.size());
+    for (size_t q = 0; q < units.size(); ++q) R->order[q] = units[cost[q].second];
+    if (R->dtasks.alloc(units.size() * sizeof(RaggedTask)) || R->counter.alloc(256)) {
         err = {E_NOMEM, "ragged workspace allocation failed"};
         return E_NOMEM;
     }

@@ -77,6 +77,27 @@ def test_cover_lengths_every_config(cfg):
     print(f"  worst rel-L2 {worst:.2e}")
 
 
+# odd lengths that repeat share one complex transform per pair (z = x + i x2);
+# an odd count leaves one signal unpaired
+PAIRS = [3, 3, 5, 5, 5, 97, 97, 105, 105, 1009, 1009, 3001, 3001, 15625, 15625, 15625, 19683, 19683, 26013, 26013,
+         100003, 100003, 262143, 262143, 64, 100]
+
+
+@pytest.mark.parametrize("cfg", [CONFIGS[0], CONFIGS[5], CONFIGS[-1]])
+def test_paired_odd_lengths(cfg):
+    print(f"case ragged pairs cfg={cfg}", flush=True)
+    lengths = np.array(PAIRS, dtype=np.int64)
+    b = ragged.RaggedBatch(lengths)
+    x = _inputs(lengths, seed=23)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.full((b.out_elems,), complex(np.nan, np.nan), dtype=torch.complex128, device="cuda")
+    b.configure(*cfg)
+    b.execute(xd, yd, fct=0.5)
+    torch.cuda.synchronize()
+    worst = _check_against_oracle(b, x, yd, range(len(lengths)), fct=0.5)
+    print(f"  worst rel-L2 {worst:.2e}")
+
+
 def test_gaps_offsets_scale_and_conjugate():
     # signals at scattered offsets with gaps; outputs in reverse order; fct and forward=False
     rng = np.random.default_rng(5)

@@ -167,11 +167,16 @@ __device__ __forceinline__ double2 twid32(int k, int L, double sign)
     return make_double2(c, sign * s);
 }
 
+// LDS holds transform data at padded positions: one word skipped every 16, so
+// that the power-of-two strides of the stages (16 b, 16 p) spread over the banks.
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+constexpr int lds_points_padded(int n) { return n + n / 16 + 1; }
+
 // Stockham stage on one transform in LDS: butterflies b = i, i + NT, ...
 // The twiddle roots are generated before the values are loaded, so the
 // sincospi evaluation does not hold the R NB values live.
 template <int R, int NB>
-__device__ __noinline__ void rstage(double2 *buf, int N, int p, int i, int NT, double s)
+__device__ __noinline__ void rstage(double2 *lds, int base, int N, int p, int i, int NT, double s)
 {
     const int nb = N / R;
     double2 w1[NB];
@@ -186,7 +191,7 @@ __device__ __noinline__ void rstage(double2 *buf, int N, int p, int i, int NT, d
         const int b = i + c * NT;
         if (b < nb)
 #pragma unroll
-            for (int q = 0; q < R; ++q) v[c][q] = buf[b + q * nb];
+            for (int q = 0; q < R; ++q) v[c][q] = lds[lpad(base + b + q * nb)];
     }
     __syncthreads();
 #pragma unroll
@@ -205,7 +210,7 @@ __device__ __noinline__ void rstage(double2 *buf, int N, int p, int i, int NT, d
             dft<R>(v[c], s);
             const int j = (b - k) * R + k;
 #pragma unroll
-            for (int q = 0; q < R; ++q) buf[j + q * p] = v[c][q];
+            for (int q = 0; q < R; ++q) lds[lpad(base + j + q * p)] = v[c][q];
         }
     }
     __syncthreads();
@@ -245,19 +250,19 @@ __device__ __noinline__ void lds_fft(double2 *lds, int L, int count, double s, i
     unsigned long long codes = radix_codes(L, order, ns);
     const int NT = blockDim.x / count;
     const int t = threadIdx.x / NT, i = threadIdx.x % NT;
-    double2 *buf = lds + (long long)t * L;
+    const int base = t * L;
     int p = 1;
     for (int st = 0; st < ns; ++st, codes >>= 4) {
         switch ((int)(codes & 15)) {
-            case 1: rstage<2, nb_rg(2)>(buf, L, p, i, NT, s); p *= 2; break;
-            case 2: rstage<3, nb_rg(3)>(buf, L, p, i, NT, s); p *= 3; break;
-            case 3: rstage<4, nb_rg(4)>(buf, L, p, i, NT, s); p *= 4; break;
-            case 4: rstage<5, nb_rg(5)>(buf, L, p, i, NT, s); p *= 5; break;
-            case 5: rstage<7, nb_rg(7)>(buf, L, p, i, NT, s); p *= 7; break;
-            case 6: rstage<8, nb_rg(8)>(buf, L, p, i, NT, s); p *= 8; break;
-            case 7: rstage<11, nb_rg(11)>(buf, L, p, i, NT, s); p *= 11; break;
-            case 8: rstage<13, nb_rg(13)>(buf, L, p, i, NT, s); p *= 13; break;
-            default: rstage<16, nb_rg(16)>(buf, L, p, i, NT, s); p *= 16; break;
+            case 1: rstage<2, nb_rg(2)>(lds, base, L, p, i, NT, s); p *= 2; break;
+            case 2: rstage<3, nb_rg(3)>(lds, base, L, p, i, NT, s); p *= 3; break;
+            case 3: rstage<4, nb_rg(4)>(lds, base, L, p, i, NT, s); p *= 4; break;
+            case 4: rstage<5, nb_rg(5)>(lds, base, L, p, i, NT, s); p *= 5; break;
+            case 5: rstage<7, nb_rg(7)>(lds, base, L, p, i, NT, s); p *= 7; break;
+            case 6: rstage<8, nb_rg(8)>(lds, base, L, p, i, NT, s); p *= 8; break;
+            case 7: rstage<11, nb_rg(11)>(lds, base, L, p, i, NT, s); p *= 11; break;
+            case 8: rstage<13, nb_rg(13)>(lds, base, L, p, i, NT, s); p *= 13; break;
+            default: rstage<16, nb_rg(16)>(lds, base, L, p, i, NT, s); p *= 16; break;
         }
     }
     if (ns == 0) __syncthreads();
@@ -292,7 +297,7 @@ __device__ void col_pass(LOAD load, const RaggedTask &T, double s, double2 *lds,
         const int cnt = min(ca, L1 - c0);
         for (int e = tid; e < ca * L2; e += nthr) {
             const int t = e % ca, n2 = e / ca;
-            lds[t * L2 + n2] = t < cnt ? load((long long)(c0 + t) + (long long)L1 * n2) : make_double2(0.0, 0.0);
+            lds[lpad(t * L2 + n2)] = t < cnt ? load((long long)(c0 + t) + (long long)L1 * n2) : make_double2(0.0, 0.0);
         }
         __syncthreads();
         lds_fft(lds, L2, ca, s, order);
@@ -303,7 +308,7 @@ __device__ void col_pass(LOAD load, const RaggedTask &T, double s, double2 *lds,
             double2 w = twid(n1 * (long long)(tid / ca), L, s);
             const double2 ws = twid(n1 * (long long)step, L, s);
             for (int k2 = tid / ca; k2 < L2; k2 += step) {
-                if (t < cnt) tmp[n1 + (long long)L1 * k2] = cmul(lds[t * L2 + k2], w);
+                if (t < cnt) tmp[n1 + (long long)L1 * k2] = cmul(lds[lpad(t * L2 + k2)], w);
                 w = cmul(w, ws);
             }
         }
@@ -324,20 +329,20 @@ __device__ void row_pass(const RaggedTask &T, double s, double2 *lds, double2 *t
     for (int r0 = 0; r0 < L2; r0 += cb) {
         const int cnt = min(cb, L2 - r0);
         double2 *row = tmp + (long long)r0 * L1;
-        for (int e = tid; e < cb * L1; e += nthr) lds[e] = e < cnt * L1 ? row[e] : make_double2(0.0, 0.0);
+        for (int e = tid; e < cb * L1; e += nthr) lds[lpad(e)] = e < cnt * L1 ? row[e] : make_double2(0.0, 0.0);
         __syncthreads();
         lds_fft(lds, L1, cb, s, order);
         if (spec) {
             const double2 *sp = spec + (long long)r0 * L1;
-            for (int e = tid; e < cnt * L1; e += nthr) lds[e] = cmul(lds[e], sp[e]);
+            for (int e = tid; e < cnt * L1; e += nthr) lds[lpad(e)] = cmul(lds[lpad(e)], sp[e]);
             __syncthreads();
             lds_fft(lds, L1, cb, -s, order);
             for (int e = tid; e < cnt * L1; e += nthr) {
                 const int t = e / L1, a = e - t * L1;
-                row[e] = cmul(lds[e], twid((long long)a * (r0 + t), L, -s));
+                row[e] = cmul(lds[lpad(e)], twid((long long)a * (r0 + t), L, -s));
             }
         } else {
-            for (int e = tid; e < cnt * L1; e += nthr) row[e] = lds[e];
+            for (int e = tid; e < cnt * L1; e += nthr) row[e] = lds[lpad(e)];
         }
         __syncthreads();
     }
@@ -355,13 +360,13 @@ __device__ void col_pass_out(EMIT emit, const RaggedTask &T, double s, double2 *
         const int cnt = min(ca, L1 - c0);
         for (int e = tid; e < ca * L2; e += nthr) {
             const int t = e % ca, k2 = e / ca;
-            lds[t * L2 + k2] = t < cnt ? tmp[(long long)(c0 + t) + (long long)L1 * k2] : make_double2(0.0, 0.0);
+            lds[lpad(t * L2 + k2)] = t < cnt ? tmp[(long long)(c0 + t) + (long long)L1 * k2] : make_double2(0.0, 0.0);
         }
         __syncthreads();
         lds_fft(lds, L2, ca, s, order);
         for (int e = tid; e < ca * L2; e += nthr) {
             const int t = e % ca, b = e / ca;
-            if (t < cnt) emit((long long)(c0 + t) + (long long)L1 * b, lds[t * L2 + b]);
+            if (t < cnt) emit((long long)(c0 + t) + (long long)L1 * b, lds[lpad(t * L2 + b)]);
         }
         __syncthreads();
     }
@@ -433,7 +438,7 @@ __device__ bool wg_fft(LOAD load, const RaggedTask &T, double s, double2 *lds, d
     const int tid = threadIdx.x, nthr = blockDim.x;
     const long long L = T.L;
     if (T.L1 == 0) {
-        for (long long m = tid; m < L; m += nthr) lds[m] = load(m);
+        for (long long m = tid; m < L; m += nthr) lds[lpad((int)m)] = load(m);
         __syncthreads();
         lds_fft(lds, (int)L, 1, s, order);
         return true;
@@ -491,7 +496,7 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
                 return make_double2(0.0, 0.0);
             };
             if (wg_fft(bload, T, -1.0, lds, S0, B, P.order))
-                for (long long m = tid; m < n2; m += nthr) B[m] = lds[m];
+                for (long long m = tid; m < n2; m += nthr) B[m] = lds[lpad((int)m)];
             __syncthreads();
             continue;
         }
@@ -516,7 +521,7 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
             };
             if (T.L1 == 0) {
                 wg_fft(aload, T, -1.0, lds, S0, nullptr, P.order);
-                for (long long m = tid; m < n2; m += nthr) lds[m] = cmul(lds[m], B[m]);
+                for (long long m = tid; m < n2; m += nthr) lds[lpad((int)m)] = cmul(lds[lpad((int)m)], B[m]);
                 __syncthreads();
                 lds_fft(lds, (int)n2, 1, 1.0, P.order);
             } else {
@@ -544,7 +549,7 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
         // Z[k] = FFT_M(z)[k], k < K (Bluestein: c[k] w[k] / n2)
         const double bscale = T.blue ? 1.0 / (double)T.L : 1.0;
         auto Z = [&](long long k) -> double2 {
-            const double2 v = in_lds ? lds[k] : S0[k];
+            const double2 v = in_lds ? lds[lpad((int)k)] : S0[k];
             return T.blue ? cscale(cmul(v, chirp(k, M, inv2m)), bscale) : v;
         };
         if (half) {
@@ -622,7 +627,9 @@ struct Ragged {
 
 static const void *ragged_fn(int threads, int waves)
 {
-    if (threads == 256) return waves == 4 ? (const void *)k_ragged<256, 4> : (const void *)k_ragged<256, 0>;
+    if (threads == 256)
+        return waves == 4 ? (const void *)k_ragged<256, 4>
+                          : waves == 3 ? (const void *)k_ragged<256, 3> : (const void *)k_ragged<256, 0>;
     if (threads == 512) return waves == 4 ? (const void *)k_ragged<512, 4> : (const void *)k_ragged<512, 0>;
     return (const void *)k_ragged<1024, 0>;
 }
@@ -632,6 +639,7 @@ static hipError_t launch_ragged(const RaggedParams &P, int threads, int waves, u
 {
     const dim3 g(grid), b(threads);
     if (threads == 256 && waves == 4) hipLaunchKernelGGL((k_ragged<256, 4>), g, b, lds, s, P);
+    else if (threads == 256 && waves == 3) hipLaunchKernelGGL((k_ragged<256, 3>), g, b, lds, s, P);
     else if (threads == 256) hipLaunchKernelGGL((k_ragged<256, 0>), g, b, lds, s, P);
     else if (threads == 512 && waves == 4) hipLaunchKernelGGL((k_ragged<512, 4>), g, b, lds, s, P);
     else if (threads == 512) hipLaunchKernelGGL((k_ragged<512, 0>), g, b, lds, s, P);
@@ -729,7 +737,7 @@ static int build_tasks(Ragged &R, int threads, int lds_max, int waves, Error &er
     int ncu = 0, per = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, R.device) != hipSuccess || ncu <= 0)
         ncu = 256;
-    const size_t lds = (size_t)lds_max * sizeof(double2);
+    const size_t lds = (size_t)lds_points_padded(lds_max) * sizeof(double2);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, ragged_fn(threads, waves), threads, lds) != hipSuccess ||
         per <= 0) {
         (void)hipGetLastError();
@@ -818,7 +826,7 @@ static int build_spectra(Ragged &R, Error &err)
         hipLaunchKernelGGL(k_ragged_reset, dim3(1), dim3(64), 0, 0, (int *)R.counter.p);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = launch_ragged(P, 1024, 0, grid, (size_t)8192 * sizeof(double2), 0);
+    if (e == hipSuccess) e = launch_ragged(P, 1024, 0, grid, (size_t)lds_points_padded(8192) * sizeof(double2), 0);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -886,10 +894,11 @@ int ragged_create(Ragged **out, size_t count, const size_t *lengths, const size_
         err = {E_NOMEM, "ragged workspace allocation failed"};
         return E_NOMEM;
     }
-    for (int v = 0; v < 5; ++v)
-        if (hipFuncSetAttribute(ragged_fn(v < 2 ? 256 : v < 4 ? 512 : 1024, (v & 1) ? 4 : 0),
+    const int inst[6][2] = {{256, 0}, {256, 3}, {256, 4}, {512, 0}, {512, 4}, {1024, 0}};
+    for (const auto &v : inst)
+        if (hipFuncSetAttribute(ragged_fn(v[0], v[1]),
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)(8192 * sizeof(double2))) != hipSuccess) {
+                                (int)(lds_points_padded(8192) * sizeof(double2))) != hipSuccess) {
             (void)hipGetLastError();
             err = {E_HIP, "cannot raise the LDS limit of the ragged kernel"};
             return E_HIP;
@@ -908,9 +917,11 @@ int ragged_configure(Ragged &R, int threads, int lds_max, int order, int waves, 
 {
     const bool pow2 = lds_max > 0 && (lds_max & (lds_max - 1)) == 0;
     if ((threads != 256 && threads != 512 && threads != 1024) || !pow2 || lds_max < kBlueRow || lds_max > 8192 ||
-        lds_max > kPointsPerThread * threads || (order != 0 && order != 1) || (waves != 0 && waves != 4)) {
+        lds_max > kPointsPerThread * threads || (order != 0 && order != 1) ||
+        (waves != 0 && waves != 4 && !(waves == 3 && threads == 256))) {
         err = {E_INVALID, "ragged configuration out of range (threads 256/512/1024; lds_points a power of two in "
-                          "[2048, 8192] and at most 8 per thread; radix order 0/1; waves per SIMD 0 or 4)"};
+                          "[2048, 8192] and at most 8 per thread; radix order 0/1; waves per SIMD 0, 4, or 3 "
+                          "with 256 threads)"};
         return E_INVALID;
     }
     if (threads == 1024) waves = 0;   // 1024 threads always leave 128 registers
@@ -940,7 +951,8 @@ static hipError_t ragged_launch(Ragged &R, const float *x, double2 *y, double fc
     hipLaunchKernelGGL(k_ragged_reset, dim3(1), dim3(64), 0, s, (int *)R.counter.p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return launch_ragged(P, R.threads, R.waves, (unsigned)R.grid, (size_t)R.lds_max * sizeof(double2), s);
+    return launch_ragged(P, R.threads, R.waves, (unsigned)R.grid, (size_t)lds_points_padded(R.lds_max) * sizeof(double2),
+                         s);
 }
 
 static int check_ptrs(const Ragged &R, const float *x, double2 *y, Error &err)
@@ -1008,7 +1020,8 @@ int ragged_tune(Ragged &R, const float *x, double2 *y, hipStream_t s, int reps, 
     struct Cfg { int t, l, o, w; };
     const Cfg cands[] = {{1024, 8192, 0, 0}, {1024, 8192, 1, 0}, {1024, 4096, 0, 0}, {512, 4096, 0, 0},
                          {512, 4096, 1, 0}, {512, 4096, 1, 4}, {512, 2048, 1, 0}, {512, 2048, 1, 4},
-                         {256, 2048, 0, 0}, {256, 2048, 1, 0}, {256, 2048, 0, 4}, {256, 2048, 1, 4}};
+                         {256, 2048, 0, 0}, {256, 2048, 1, 0}, {256, 2048, 0, 4}, {256, 2048, 1, 4},
+                         {256, 2048, 0, 3}, {256, 2048, 1, 3}};
     hipEvent_t a = nullptr, b = nullptr;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
         (void)hipGetLastError();

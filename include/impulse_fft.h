@@ -191,8 +191,9 @@ int impulse_fft_ragged_tune(impulse_fft_ragged batch, const float *in, void *out
 /* Pick a configuration by hand: threads 256/512/1024, lds_points
  * 2048/4096/8192 (longest transform kept in LDS; at most 8 per thread),
  * radix_order 0 (powers of two first) or 1 (odd radices first),
- * waves_per_simd 0 (registers as the workgroup size allows) or 4 (at most 128
- * registers per lane: more resident workgroups, some spills). */
+ * waves_per_simd 0 (registers as the workgroup size allows), 4 (at most 128
+ * registers per lane: more resident workgroups, some spills) or, with 256
+ * threads, 3 (at most 168). */
 int impulse_fft_ragged_configure(impulse_fft_ragged batch, int threads, int lds_points, int radix_order,
                                  int waves_per_simd);
 /* Current configuration, persistent grid size, nominal flops of one execute

@@ -84,7 +84,8 @@ std::string cache_key(int kind, int dtype, size_t ndim, const size_t *shape, con
                       const ptrdiff_t *sout, size_t naxes, const size_t *axes, int forward, int dev)
 {
     std::ostringstream k;
-    k << kind << '|' << dtype << '|' << (forward ? 1 : 0) << '|' << dev << '|';
+    // `forward` is a flag for the FFT kinds and 2 type + ortho for the DCT / DST kinds
+    k << kind << '|' << dtype << '|' << (kind >= IMPULSE_FFT_DCT ? forward : (forward ? 1 : 0)) << '|' << dev << '|';
     for (size_t d = 0; d < ndim; ++d) k << shape[d] << ',' << sin[d] << ',' << sout[d] << ';';
     k << '|';
     for (size_t a = 0; a < naxes; ++a) k << axes[a] << ',';
@@ -111,7 +112,10 @@ int cached_plan(std::shared_ptr<ifft::Plan> &out, int kind, int dtype, size_t nd
     }
     std::unique_ptr<ifft::Plan> p;
     ifft::Error err{0, ""};
-    int rc = ifft::plan_create(p, kind, dtype, ndim, shape, sin, sout, naxes, axes, forward, err);
+    // DCT / DST (kinds 5, 6) carry the type and ortho flag in `forward` (2 type + ortho)
+    int rc = kind >= 5 ? ifft::r2r_plan_create(p, kind, forward >> 1, forward & 1, dtype, ndim, shape, sin, sout,
+                                               naxes, axes, err)
+                       : ifft::plan_create(p, kind, dtype, ndim, shape, sin, sout, naxes, axes, forward, err);
     if (rc != ifft::OK) return fail(err);
     std::shared_ptr<ifft::Plan> sp(std::move(p));
     {
@@ -198,13 +202,16 @@ size_t impulse_fft_plan_device_bytes(impulse_fft_plan plan)
 
 int impulse_fft_plan_num_launches(impulse_fft_plan plan)
 {
-    return (plan && plan->plan) ? (int)plan->plan->passes.size() : -1;
+    if (!plan || !plan->plan) return -1;
+    if (plan->plan->r2r) return ifft::r2r_num_launches(*plan->plan->r2r);
+    return (int)plan->plan->passes.size();
 }
 
 int impulse_fft_plan_pass_info(impulse_fft_plan plan, int pass, char *name, size_t name_len, size_t *bytes_in,
                                size_t *bytes_out, unsigned *grid, unsigned *block, size_t *lds_bytes)
 {
     if (!plan || !plan->plan) return fail(ifft::E_INVALID, "plan is NULL");
+    if (plan->plan->r2r) return fail(ifft::E_UNSUPPORTED, "per-launch information is not kept for DCT/DST plans");
     const auto &P = *plan->plan;
     if (pass < 0 || pass >= (int)P.passes.size()) return fail(ifft::E_INVALID, "pass index out of range");
     const auto &ps = P.passes[pass];
@@ -223,6 +230,7 @@ int impulse_fft_plan_pass_info(impulse_fft_plan plan, int pass, char *name, size
 int impulse_fft_plan_profile(impulse_fft_plan plan, int max_executes)
 {
     if (!plan || !plan->plan) return fail(ifft::E_INVALID, "plan is NULL");
+    if (plan->plan->r2r) return fail(ifft::E_UNSUPPORTED, "per-launch timing is not kept for DCT/DST plans");
     ifft::Error err{0, ""};
     int rc = ifft::plan_profile(*plan->plan, max_executes, err);
     return rc == ifft::OK ? rc : fail(err);
@@ -231,6 +239,7 @@ int impulse_fft_plan_profile(impulse_fft_plan plan, int max_executes)
 int impulse_fft_plan_profile_read(impulse_fft_plan plan, double *pass_ms, int npass, int *executes)
 {
     if (!plan || !plan->plan || (!pass_ms && npass > 0)) return fail(ifft::E_INVALID, "invalid argument");
+    if (plan->plan->r2r) return fail(ifft::E_UNSUPPORTED, "per-launch timing is not kept for DCT/DST plans");
     ifft::Error err{0, ""};
     int rc = ifft::plan_profile_read(*plan->plan, pass_ms, npass, executes, err);
     return rc == ifft::OK ? rc : fail(err);
@@ -258,6 +267,49 @@ int impulse_fft_r2c(size_t ndim, const size_t *shape_in, const ptrdiff_t *stride
     (void)nthreads;
     return oneshot(IMPULSE_FFT_R2C, ndim, shape_in, stride_in, stride_out, naxes, axes, forward, dtype, data_in,
                    data_out, fct);
+}
+
+int impulse_fft_r2r_plan_create(impulse_fft_plan *plan, int kind, int type, int ortho, int dtype, size_t ndim,
+                                const size_t *shape, const ptrdiff_t *stride_in, const ptrdiff_t *stride_out,
+                                size_t naxes, const size_t *axes)
+{
+    if (!plan) return fail(ifft::E_INVALID, "plan pointer is NULL");
+    *plan = nullptr;
+    std::unique_ptr<ifft::Plan> p;
+    ifft::Error err{0, ""};
+    int rc = ifft::r2r_plan_create(p, kind, type, ortho, dtype, ndim, shape, stride_in, stride_out, naxes, axes, err);
+    if (rc != ifft::OK) return fail(err);
+    auto *h = new (std::nothrow) impulse_fft_plan_s;
+    if (!h) return fail(ifft::E_NOMEM, "out of host memory");
+    h->plan = std::move(p);
+    *plan = h;
+    return ifft::OK;
+}
+
+static int oneshot_r2r(int kind, size_t ndim, const size_t *shape, const ptrdiff_t *sin, const ptrdiff_t *sout,
+                       size_t naxes, const size_t *axes, int type, int dtype, const void *in, void *out, double fct,
+                       int ortho)
+{
+    if (type < 1 || type > 4) return fail(ifft::E_INVALID, kind == 5 ? "invalid DCT type" : "invalid DST type");
+    return oneshot(kind, ndim, shape, sin, sout, naxes, axes, 2 * type + (ortho ? 1 : 0), dtype, in, out, fct);
+}
+
+int impulse_fft_dct(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in, const ptrdiff_t *stride_out,
+                    size_t naxes, const size_t *axes, int type, int dtype, const void *data_in, void *data_out,
+                    double fct, int ortho, size_t nthreads)
+{
+    (void)nthreads;
+    return oneshot_r2r(IMPULSE_FFT_DCT, ndim, shape, stride_in, stride_out, naxes, axes, type, dtype, data_in,
+                       data_out, fct, ortho);
+}
+
+int impulse_fft_dst(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in, const ptrdiff_t *stride_out,
+                    size_t naxes, const size_t *axes, int type, int dtype, const void *data_in, void *data_out,
+                    double fct, int ortho, size_t nthreads)
+{
+    (void)nthreads;
+    return oneshot_r2r(IMPULSE_FFT_DST, ndim, shape, stride_in, stride_out, naxes, axes, type, dtype, data_in,
+                       data_out, fct, ortho);
 }
 
 int impulse_fft_c2r(size_t ndim, const size_t *shape_out, const ptrdiff_t *stride_in, const ptrdiff_t *stride_out,

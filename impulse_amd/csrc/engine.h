@@ -66,6 +66,8 @@ struct Pass {
     FftParams fp_alt{};
 };
 
+struct R2r;   // DCT / DST plan data (dct.hip)
+
 struct Plan {
     int kind = 0, dtype = 0, forward = 1;
     int device = 0;
@@ -85,6 +87,7 @@ struct Plan {
     // per-launch event timing (impulse_fft_plan_profile)
     std::vector<hipEvent_t> prof_ev;    // 2 per launch per recorded execute
     int prof_max = 0, prof_count = 0;
+    R2r *r2r = nullptr;                 // DCT / DST plans: per-axis pre/post passes around c2c sub-plans
     size_t device_bytes() const;
     ~Plan();
 };
@@ -95,6 +98,15 @@ int plan_create(std::unique_ptr<Plan> &out, int kind, int dtype, size_t ndim, co
 int plan_execute(Plan &plan, const void *in, void *out, double fct, hipStream_t stream, Error &err);
 int plan_execute_host(Plan &plan, const void *in, void *out, double fct, Error &err);
 int plan_profile(Plan &plan, int max_executes, Error &err);
+void plan_spans(Plan &P, ptrdiff_t in_elem, ptrdiff_t out_elem);   // host byte spans of the shape/strides
+
+// DCT / DST of types 1-4 (dct.hip); kind 5 = DCT, 6 = DST
+int r2r_plan_create(std::unique_ptr<Plan> &out, int kind, int type, int ortho, int dtype, size_t ndim,
+                    const size_t *shape, const ptrdiff_t *stride_in, const ptrdiff_t *stride_out, size_t naxes,
+                    const size_t *axes, Error &err);
+int r2r_execute(Plan &P, const void *in, void *out, double fct, hipStream_t s, Error &err);
+int r2r_num_launches(const R2r &R);
+void r2r_destroy(R2r *r);
 int plan_profile_read(Plan &plan, double *pass_ms, int npass, int *executes, Error &err);
 
 // FFT convolution (conv.hip)

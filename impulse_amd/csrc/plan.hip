@@ -49,6 +49,7 @@ int DevBuf::alloc(size_t n)
 Plan::~Plan()
 {
     for (auto e : prof_ev) (void)hipEventDestroy(e);
+    r2r_destroy(r2r);
 }
 
 size_t Plan::device_bytes() const
@@ -861,6 +862,16 @@ static void span(const std::vector<size_t> &shape, const std::vector<ptrdiff_t> 
     hi += elem;
 }
 
+// byte spans of a plan whose input and output have the plan's shape
+void plan_spans(Plan &P, ptrdiff_t in_elem, ptrdiff_t out_elem)
+{
+    span(P.shape, P.sin, in_elem, P.in_lo, P.in_hi);
+    span(P.shape, P.sout, out_elem, P.out_lo, P.out_hi);
+    size_t n = 1;
+    for (auto s : P.shape) n *= s;
+    P.out_dense = (size_t)(P.out_hi - P.out_lo) == n * (size_t)out_elem;
+}
+
 int plan_create(std::unique_ptr<Plan> &out, int kind, int dtype, size_t ndim, const size_t *shape,
                 const ptrdiff_t *stride_in, const ptrdiff_t *stride_out, size_t naxes, const size_t *axes,
                 int forward, Error &err)
@@ -983,6 +994,14 @@ int plan_execute(Plan &P, const void *in, void *out, double fct, hipStream_t str
 {
     if (P.empty) return OK;
     if (!in || !out) { err = {E_INVALID, "null data pointer"}; return E_INVALID; }
+    if (P.r2r) {
+        if (in == out && P.sin != P.sout) { err = {E_INVALID, "stride mismatch"}; return E_INVALID; }
+        if (((uintptr_t)in | (uintptr_t)out) % 8) {
+            err = {E_UNSUPPORTED, "data pointers must be aligned to the element size"};
+            return E_UNSUPPORTED;
+        }
+        return r2r_execute(P, in, out, fct, stream, err);
+    }
     if (in == out && P.kind != J_C2C) {
         err = {E_INVALID, "in-place r2c/c2r is not supported"};
         return E_INVALID;

@@ -269,14 +269,40 @@ class FFTDesc:
         return cls(axes, forward, scalingFactor, nthreads)
 
 
+class DCTDesc:
+    """cpp_pocketfft/pocketfft.nim:151-156,217-233: a DCT of ``dctType`` (1-4)
+    over ``axes``; ``ortho`` and ``scalingFactor`` as pocketfft::dct takes them.
+    ``sine=True`` selects the DST of the same type (pocketfft::dst,
+    cpp_pocketfft/pocketfft.nim:121-132)."""
+
+    def __init__(self, axes, dctType=2, ortho=False, scalingFactor=1.0, nthreads=1, sine=False):
+        if not 1 <= int(dctType) <= 4:
+            raise ValueError("dctType must be in 1..4")
+        self.axes = [int(a) for a in axes]
+        self.dctType = int(dctType)
+        self.ortho = bool(ortho)
+        self.scalingFactor = float(scalingFactor)
+        self.nthreads = int(nthreads)
+        self.sine = bool(sine)
+
+    @classmethod
+    def init(cls, axes, dctType=2, ortho=False, scalingFactor=1.0, nthreads=1, sine=False):
+        return cls(axes, dctType, ortho, scalingFactor, nthreads, sine)
+
+
 class Plan:
     """An n-D plan (impulse_fft_plan_create) for device execution."""
 
-    def __init__(self, kind, shape, stride_in, stride_out, axes, forward, dtype=_lib.F64):
+    def __init__(self, kind, shape, stride_in, stride_out, axes, forward, dtype=_lib.F64, r2r_type=2, ortho=False):
         h = ctypes.c_void_p()
-        check(lib().impulse_fft_plan_create(ctypes.byref(h), int(kind), int(dtype), len(shape), sizes(shape),
-                                            strides(stride_in), strides(stride_out), len(axes), sizes(axes),
-                                            1 if forward else 0))
+        if kind in (_lib.DCT, _lib.DST):
+            check(lib().impulse_fft_r2r_plan_create(ctypes.byref(h), int(kind), int(r2r_type), int(bool(ortho)),
+                                                    int(dtype), len(shape), sizes(shape), strides(stride_in),
+                                                    strides(stride_out), len(axes), sizes(axes)))
+        else:
+            check(lib().impulse_fft_plan_create(ctypes.byref(h), int(kind), int(dtype), len(shape), sizes(shape),
+                                                strides(stride_in), strides(stride_out), len(axes), sizes(axes),
+                                                1 if forward else 0))
         self.handle = h
         self.kind = kind
 
@@ -305,12 +331,13 @@ _plan_cache = {}
 _plan_lock = threading.Lock()
 
 
-def get_plan(kind, shape, stride_in, stride_out, axes, forward):
-    key = (kind, tuple(shape), tuple(stride_in), tuple(stride_out), tuple(axes), bool(forward))
+def get_plan(kind, shape, stride_in, stride_out, axes, forward, r2r_type=2, ortho=False):
+    key = (kind, tuple(shape), tuple(stride_in), tuple(stride_out), tuple(axes), bool(forward), int(r2r_type),
+           bool(ortho))
     with _plan_lock:
         p = _plan_cache.get(key)
     if p is None:
-        p = Plan(kind, shape, stride_in, stride_out, axes, forward)
+        p = Plan(kind, shape, stride_in, stride_out, axes, forward, r2r_type=r2r_type, ortho=ortho)
         with _plan_lock:
             _plan_cache[key] = p
     return p
@@ -322,7 +349,22 @@ def _current_stream():
 
 
 def apply(fft, descOut, descIn):
-    """Dispatch c2c / r2c / c2r by element types (cpp_pocketfft/pocketfft.nim:235-277)."""
+    """Dispatch c2c / r2c / c2r by element types (cpp_pocketfft/pocketfft.nim:235-277),
+    or a DCT / DST for a DCTDesc (cpp_pocketfft/pocketfft.nim:279-295, which passes
+    ``descIn.shape``)."""
+    if isinstance(fft, DCTDesc):
+        if descIn.is_complex or descOut.is_complex:
+            raise TypeError("DCT / DST take real arrays")
+        kind = _lib.DST if fft.sine else _lib.DCT
+        if _is_torch(descIn.buf) and descIn.buf.is_cuda:
+            p = get_plan(kind, descIn.shape, descIn.stride, descOut.stride, fft.axes, True, fft.dctType, fft.ortho)
+            p.execute(descIn.ptr(), descOut.ptr(), fft.scalingFactor, _current_stream())
+            return
+        fn = lib().impulse_fft_dst if fft.sine else lib().impulse_fft_dct
+        check(fn(len(descIn.shape), sizes(descIn.shape), strides(descIn.stride), strides(descOut.stride),
+                 len(fft.axes), sizes(fft.axes), fft.dctType, _lib.F64, descIn.ptr(), descOut.ptr(),
+                 fft.scalingFactor, 1 if fft.ortho else 0, fft.nthreads))
+        return
     if descIn.is_complex and descOut.is_complex:
         kind, shape = _lib.C2C, descIn.shape
     elif descOut.is_complex:
@@ -384,4 +426,25 @@ def c2r(x, n, axes, forward=False, fct=1.0, out=None):
     check(lib().impulse_fft_c2r(len(oshape), sizes(oshape), strides(x.strides), strides(out.strides), len(axes),
                                 sizes(axes), 1 if forward else 0, _lib.F64, x.ctypes.data, out.ctypes.data,
                                 float(fct), 1))
+    return out
+
+
+def dct(x, axes, type=2, fct=1.0, ortho=False, out=None):
+    """pocketfft::dct on a float64 numpy array (any byte strides)."""
+    return _r2r(lib().impulse_fft_dct, x, axes, type, fct, ortho, out)
+
+
+def dst(x, axes, type=2, fct=1.0, ortho=False, out=None):
+    """pocketfft::dst on a float64 numpy array (any byte strides)."""
+    return _r2r(lib().impulse_fft_dst, x, axes, type, fct, ortho, out)
+
+
+def _r2r(fn, x, axes, type, fct, ortho, out):
+    x = np.asarray(x)
+    if x.dtype != np.float64:
+        raise TypeError("DCT / DST need float64")
+    if out is None:
+        out = np.empty(x.shape, np.float64)
+    check(fn(x.ndim, sizes(x.shape), strides(x.strides), strides(out.strides), len(axes), sizes(axes), int(type),
+             _lib.F64, x.ctypes.data, out.ctypes.data, float(fct), 1 if ortho else 0, 1))
     return out

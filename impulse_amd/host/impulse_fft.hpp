@@ -319,5 +319,39 @@ template <class F, class In, class Out> void apply(const FFTDesc<F> &fft, DataDe
     }
 }
 
+// DCT of dctType (1-4) over axes (cpp_pocketfft/pocketfft.nim:151-156,217-233);
+// `sine` selects the DST of the same type (pocketfft::dst).
+template <class T> struct DCTDesc {
+    std::vector<size_t> axes;
+    int dctType = 2;
+    T scalingFactor = 1;
+    size_t nthreads = 1;   // accepted, ignored
+    bool ortho = false;
+    bool sine = false;
+
+    static DCTDesc init(const std::vector<int> &axes_, int dctType_ = 2, bool ortho_ = false, T scalingFactor_ = 1,
+                        size_t nthreads_ = 1, bool sine_ = false)
+    {
+        DCTDesc d;
+        for (int a : axes_) d.axes.push_back((size_t)a);
+        d.dctType = dctType_;
+        d.ortho = ortho_;
+        d.scalingFactor = scalingFactor_;
+        d.nthreads = nthreads_;
+        d.sine = sine_;
+        return d;
+    }
+};
+
+// cpp_pocketfft/pocketfft.nim:279-295: pocketfft::dct (or dst) over descIn's shape.
+template <class F, class T> void apply(const DCTDesc<F> &dct, DataDesc<T> &descOut, const DataDesc<T> &descIn)
+{
+    static_assert(std::is_same<T, double>::value && std::is_same<F, double>::value, "the engine computes in FP64");
+    auto fn = dct.sine ? impulse_fft_dst : impulse_fft_dct;
+    check_status(fn(descIn.shape.size(), descIn.shape.data(), descIn.stride.data(), descOut.stride.data(),
+                    dct.axes.size(), dct.axes.data(), dct.dctType, IMPULSE_FFT_F64, descIn.buf, descOut.buf,
+                    (double)dct.scalingFactor, dct.ortho ? 1 : 0, dct.nthreads));
+}
+
 }  // namespace fft
 }  // namespace impulse

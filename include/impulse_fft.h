@@ -55,7 +55,10 @@ typedef enum impulse_fft_kind {
      *   opposite direction with those bins weighted 2 (`shape` is the real shape).
      * The adjoint of C2C is C2C of the opposite direction. */
     IMPULSE_FFT_R2C_ADJOINT = 3,
-    IMPULSE_FFT_C2R_ADJOINT = 4
+    IMPULSE_FFT_C2R_ADJOINT = 4,
+    /* Real-to-real transforms (impulse_fft_r2r_plan_create, impulse_fft_dct/dst). */
+    IMPULSE_FFT_DCT = 5,
+    IMPULSE_FFT_DST = 6
 } impulse_fft_kind;
 
 typedef enum impulse_fft_dtype {
@@ -132,6 +135,33 @@ int impulse_fft_r2c(size_t ndim, const size_t *shape_in, const ptrdiff_t *stride
 int impulse_fft_c2r(size_t ndim, const size_t *shape_out, const ptrdiff_t *stride_in,
                     const ptrdiff_t *stride_out, size_t naxes, const size_t *axes, int forward,
                     int dtype, const void *data_in, void *data_out, double fct, size_t nthreads);
+
+/* DCT / DST of types 1-4 over `axes` with the argument list of pocketfft::dct
+ * and pocketfft::dst (pocketfft_hdronly.h:3284-3318, bound at
+ * cpp_pocketfft/pocketfft.nim:108-132 and applied by `apply(DCTDesc, ...)`,
+ * cpp_pocketfft/pocketfft.nim:279-295), plus `dtype` (IMPULSE_FFT_F64).  Real
+ * doubles with byte strides on HOST memory; in == out is allowed with equal
+ * strides.  Unnormalised; `ortho` applies only the end-point scalings
+ * PocketFFT applies (DCT-I: x[0], x[n-1] times sqrt2 before and y[0],
+ * y[n-1] times sqrt2/2 after; DCT-II and DST-II: y[0] times sqrt2/2;
+ * DCT-III and DST-III: x[0] times sqrt2; ignored for DST-I and type 4) and
+ * the 1/sqrt normalisation is the caller's `fct`, which scales the first
+ * axis only.  DCT-I needs at least 2 points per axis. */
+int impulse_fft_dct(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in,
+                    const ptrdiff_t *stride_out, size_t naxes, const size_t *axes, int type,
+                    int dtype, const void *data_in, void *data_out, double fct, int ortho,
+                    size_t nthreads);
+int impulse_fft_dst(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in,
+                    const ptrdiff_t *stride_out, size_t naxes, const size_t *axes, int type,
+                    int dtype, const void *data_in, void *data_out, double fct, int ortho,
+                    size_t nthreads);
+/* The same as a plan (kind IMPULSE_FFT_DCT or IMPULSE_FFT_DST), executed with
+ * impulse_fft_execute (device pointers) or impulse_fft_execute_host and freed
+ * with impulse_fft_plan_destroy.  Per-launch profiling is not kept for these
+ * plans (IMPULSE_FFT_ERR_UNSUPPORTED); num_launches counts every launch. */
+int impulse_fft_r2r_plan_create(impulse_fft_plan *plan, int kind, int type, int ortho, int dtype,
+                                size_t ndim, const size_t *shape, const ptrdiff_t *stride_in,
+                                const ptrdiff_t *stride_out, size_t naxes, const size_t *axes);
 
 /* Plan cache control (PocketFFT's is a fixed 16-entry LRU, pocketfft_hdronly.h:50-52). */
 int impulse_fft_cache_set_capacity(size_t entries);

@@ -41,6 +41,10 @@ def lib():
             getattr(L, name).restype = ctypes.c_int
         L.oracle_dft_naive.argtypes = [ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         L.oracle_dft_naive.restype = ctypes.c_int
+        for name in ("oracle_dct", "oracle_dst"):
+            getattr(L, name).argtypes = [ctypes.c_size_t, _szp, _pdp, _pdp, ctypes.c_size_t, _szp, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+            getattr(L, name).restype = ctypes.c_int
         L.oracle_twiddle.argtypes = [ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, _dp, _dp]
         L.oracle_twiddle.restype = None
         _LIB = L
@@ -128,3 +132,25 @@ def twiddle(k, n, sign=-1):
     re, im = ctypes.c_double(), ctypes.c_double()
     lib().oracle_twiddle(k, n, sign, ctypes.byref(re), ctypes.byref(im))
     return complex(re.value, im.value)
+
+
+def _r2r(fn, what, x, axes, type, fct, ortho, out):
+    x = np.asarray(x)
+    assert x.dtype == np.float64
+    if out is None:
+        out = np.empty(x.shape, np.float64)
+    rc = fn(x.ndim, _arr(list(x.shape), ctypes.c_size_t), _arr(list(x.strides), ctypes.c_ssize_t),
+            _arr(list(out.strides), ctypes.c_ssize_t), len(axes), _arr(list(axes), ctypes.c_size_t), int(type),
+            x.ctypes.data, out.ctypes.data, float(fct), int(bool(ortho)))
+    _check(rc, what)
+    return out
+
+
+def dct(x, axes, type=2, fct=1.0, ortho=False, out=None):
+    """pocketfft::dct semantics (naive long-double sums)."""
+    return _r2r(lib().oracle_dct, "oracle_dct", x, axes, type, fct, ortho, out)
+
+
+def dst(x, axes, type=2, fct=1.0, ortho=False, out=None):
+    """pocketfft::dst semantics (naive long-double sums)."""
+    return _r2r(lib().oracle_dst, "oracle_dst", x, axes, type, fct, ortho, out)

@@ -486,3 +486,137 @@ int oracle_c2r(size_t ndim, const size_t *shape_out, const ptrdiff_t *stride_in,
     free(c.buf); free(c.rbuf); free(tmp);
     return rc;
 }
+
+
+/* ---- DCT / DST (naive sums, long double) ------------------------------- */
+
+typedef struct {
+    const char *in; char *out; ptrdiff_t is, os; size_t n; int type, cosine, ortho; long double fct;
+    long double *x, *y;
+} r2r_ctx;
+
+/* cos / sin (pi a / b) from the octant-exact twiddle: exp(2 pi i a / 2b) */
+static long double cospi_q(uint64_t a, uint64_t b) { return twiddle_ld(a % (2 * b), 2 * b, 1).r; }
+static long double sinpi_q(uint64_t a, uint64_t b) { return twiddle_ld(a % (2 * b), 2 * b, 1).i; }
+
+static int r2r_line(void *vctx, ptrdiff_t io, ptrdiff_t oo)
+{
+    r2r_ctx *c = (r2r_ctx *)vctx;
+    const size_t N = c->n;
+    const long double s2 = 1.414213562373095048801688724209698L;
+    long double *x = c->x, *y = c->y;
+    for (size_t j = 0; j < N; ++j) x[j] = *(const double *)(c->in + io + (ptrdiff_t)j * c->is);
+    if (c->cosine) {
+        switch (c->type) {
+            case 1:
+                if (c->ortho) { x[0] *= s2; x[N - 1] *= s2; }
+                for (size_t k = 0; k < N; ++k) {
+                    long double a = x[0] + ((k & 1) ? -x[N - 1] : x[N - 1]);
+                    for (size_t m = 1; m + 1 < N; ++m) a += 2 * x[m] * cospi_q(k * m, N - 1);
+                    y[k] = a;
+                }
+                if (c->ortho) { y[0] *= s2 * 0.5L; y[N - 1] *= s2 * 0.5L; }
+                break;
+            case 2:
+                for (size_t k = 0; k < N; ++k) {
+                    long double a = 0;
+                    for (size_t m = 0; m < N; ++m) a += x[m] * cospi_q(k * (2 * m + 1), 2 * N);
+                    y[k] = 2 * a;
+                }
+                if (c->ortho) y[0] *= s2 * 0.5L;
+                break;
+            case 3:
+                if (c->ortho) x[0] *= s2;
+                for (size_t k = 0; k < N; ++k) {
+                    long double a = x[0];
+                    for (size_t m = 1; m < N; ++m) a += 2 * x[m] * cospi_q(m * (2 * k + 1), 2 * N);
+                    y[k] = a;
+                }
+                break;
+            default:
+                for (size_t k = 0; k < N; ++k) {
+                    long double a = 0;
+                    for (size_t m = 0; m < N; ++m) a += x[m] * cospi_q((2 * k + 1) * (2 * m + 1), 4 * N);
+                    y[k] = 2 * a;
+                }
+        }
+    } else {
+        switch (c->type) {
+            case 1:
+                for (size_t k = 0; k < N; ++k) {
+                    long double a = 0;
+                    for (size_t m = 0; m < N; ++m) a += x[m] * sinpi_q((k + 1) * (m + 1), N + 1);
+                    y[k] = 2 * a;
+                }
+                break;
+            case 2:
+                for (size_t k = 0; k < N; ++k) {
+                    long double a = 0;
+                    for (size_t m = 0; m < N; ++m) a += x[m] * sinpi_q((k + 1) * (2 * m + 1), 2 * N);
+                    y[k] = 2 * a;
+                }
+                if (c->ortho) y[0] *= s2 * 0.5L;
+                break;
+            case 3:
+                if (c->ortho) x[0] *= s2;
+                for (size_t k = 0; k < N; ++k) {
+                    long double a = (k & 1) ? -x[N - 1] : x[N - 1];
+                    for (size_t m = 0; m + 1 < N; ++m) a += 2 * x[m] * sinpi_q((2 * k + 1) * (m + 1), 2 * N);
+                    y[k] = a;
+                }
+                break;
+            default:
+                for (size_t k = 0; k < N; ++k) {
+                    long double a = 0;
+                    for (size_t m = 0; m < N; ++m) a += x[m] * sinpi_q((2 * k + 1) * (2 * m + 1), 4 * N);
+                    y[k] = 2 * a;
+                }
+        }
+    }
+    for (size_t k = 0; k < N; ++k) *(double *)(c->out + oo + (ptrdiff_t)k * c->os) = (double)(y[k] * c->fct);
+    return 0;
+}
+
+static int r2r_nd(size_t ndim, const size_t *shape, const ptrdiff_t *sin, const ptrdiff_t *sout, size_t naxes,
+                  const size_t *axes, int type, int cosine, const double *in, double *out, double fct, int ortho)
+{
+    if (type < 1 || type > 4) return 1;
+    if (check_args(ndim, shape, naxes, axes)) return 1;
+    if (prod(ndim, shape) == 0) return 0;
+    if (in == out)
+        for (size_t d = 0; d < ndim; ++d) if (sin[d] != sout[d]) return 1;
+    for (size_t a = 0; a < naxes; ++a)
+        if (cosine && type == 1 && shape[axes[a]] < 2) return 1;   /* DCT-I needs N >= 2 */
+    int rc = 0;
+    for (size_t a = 0; a < naxes && !rc; ++a) {
+        r2r_ctx c;
+        const size_t ax = axes[a];
+        c.in = a ? (const char *)out : (const char *)in;
+        c.out = (char *)out;
+        c.is = a ? sout[ax] : sin[ax];
+        c.os = sout[ax];
+        c.n = shape[ax];
+        c.type = type; c.cosine = cosine; c.ortho = ortho;
+        c.fct = a ? 1.0L : (long double)fct;
+        c.x = (long double *)malloc(c.n * sizeof(long double));
+        c.y = (long double *)malloc(c.n * sizeof(long double));
+        if (!c.x || !c.y) { free(c.x); free(c.y); return 2; }
+        rc = for_each_line(ndim, shape, a ? sout : sin, sout, ax, r2r_line, &c);
+        free(c.x); free(c.y);
+    }
+    return rc;
+}
+
+int oracle_dct(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in, const ptrdiff_t *stride_out,
+               size_t naxes, const size_t *axes, int type, const double *data_in, double *data_out, double fct,
+               int ortho)
+{
+    return r2r_nd(ndim, shape, stride_in, stride_out, naxes, axes, type, 1, data_in, data_out, fct, ortho);
+}
+
+int oracle_dst(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in, const ptrdiff_t *stride_out,
+               size_t naxes, const size_t *axes, int type, const double *data_in, double *data_out, double fct,
+               int ortho)
+{
+    return r2r_nd(ndim, shape, stride_in, stride_out, naxes, axes, type, 0, data_in, data_out, fct, ortho);
+}

@@ -58,6 +58,32 @@ int oracle_c2r(size_t ndim, const size_t *shape_out, const ptrdiff_t *stride_in,
                const ptrdiff_t *stride_out, size_t naxes, const size_t *axes,
                int forward, const double *data_in, double *data_out, double fct);
 
+/* DCT / DST of types 1-4 along `axes` (pocketfft::dct / pocketfft::dst,
+ * impulse/fft/cpp_pocketfft/pocketfft_hdronly.h:3284-3318 over T_dct1, T_dst1,
+ * T_dcst23, T_dcst4 at :2424-2648), as the naive sums they compute, in long
+ * double, unnormalised (N = length along the axis):
+ *   DCT-I   y[k] = x[0] + (-1)^k x[N-1] + 2 sum_{n=1}^{N-2} x[n] cos(pi k n / (N-1))      (N >= 2)
+ *   DCT-II  y[k] = 2 sum_n x[n] cos(pi k (2n+1) / 2N)
+ *   DCT-III y[k] = x[0] + 2 sum_{n>=1} x[n] cos(pi n (2k+1) / 2N)
+ *   DCT-IV  y[k] = 2 sum_n x[n] cos(pi (2k+1)(2n+1) / 4N)
+ *   DST-I   y[k] = 2 sum_n x[n] sin(pi (k+1)(n+1) / (N+1))
+ *   DST-II  y[k] = 2 sum_n x[n] sin(pi (k+1)(2n+1) / 2N)
+ *   DST-III y[k] = (-1)^k x[N-1] + 2 sum_{n<N-1} x[n] sin(pi (2k+1)(n+1) / 2N)
+ *   DST-IV  y[k] = 2 sum_n x[n] sin(pi (2k+1)(2n+1) / 4N)
+ * `ortho` applies exactly the end-point scalings that code applies (the
+ * overall 1/sqrt factor is the caller's fct): DCT-I x[0], x[N-1] times sqrt2
+ * before and y[0], y[N-1] times sqrt2/2 after (:2438,2448); DCT-II and DST-II
+ * y[0] times sqrt2/2 (:2522, after DST-II's reversal); DCT-III and DST-III
+ * x[0] times sqrt2 (:2526, before DST-III's reversal); DST-I, DCT-IV and
+ * DST-IV ignore it.  fct scales the first axis only.  Real doubles with byte
+ * strides; in == out allowed with equal strides. */
+int oracle_dct(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in, const ptrdiff_t *stride_out,
+               size_t naxes, const size_t *axes, int type, const double *data_in, double *data_out, double fct,
+               int ortho);
+int oracle_dst(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in, const ptrdiff_t *stride_out,
+               size_t naxes, const size_t *axes, int type, const double *data_in, double *data_out, double fct,
+               int ortho);
+
 /* Twiddle exp(sign * 2 pi i k / n) in long double, exact at multiples of pi/4
  * (returned through re/im as doubles of the long-double value). */
 void oracle_twiddle(size_t k, size_t n, int sign, double *re, double *im);

@@ -47,6 +47,11 @@ def lib():
             getattr(L, name).restype = ctypes.c_int
         L.destroy_cfft_plan.argtypes = [ctypes.c_void_p]
         L.destroy_rfft_plan.argtypes = [ctypes.c_void_p]
+        for name in ("ref_dct_f64", "ref_dst_f64"):
+            f = getattr(L, name)
+            f.argtypes = [ctypes.c_size_t, szp, pdp, pdp, ctypes.c_size_t, szp, ctypes.c_int,
+                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_size_t]
+            f.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -124,3 +129,26 @@ def rfft_packed(x, forward=True, fct=1.0):
     if rc:
         raise ValueError("rfft failed")
     return c
+
+
+def _r2r(name, x, axes, type, fct, ortho, nthreads, out):
+    x = np.asarray(x)
+    assert x.dtype == np.float64
+    if out is None:
+        out = np.empty(x.shape, np.float64)
+    rc = getattr(lib(), name)(x.ndim, _a(x.shape, ctypes.c_size_t), _a(x.strides, ctypes.c_ssize_t),
+                              _a(out.strides, ctypes.c_ssize_t), len(axes), _a(list(axes), ctypes.c_size_t),
+                              int(type), x.ctypes.data, out.ctypes.data, float(fct), int(bool(ortho)), nthreads)
+    if rc != 0:
+        raise ValueError(f"reference pocketfft raised (code {rc})")
+    return out
+
+
+def dct(x, axes, type=2, fct=1.0, ortho=False, nthreads=1, out=None):
+    """pocketfft::dct<double> (pocketfft_hdronly.h:3284-3300)."""
+    return _r2r("ref_dct_f64", x, axes, type, fct, ortho, nthreads, out)
+
+
+def dst(x, axes, type=2, fct=1.0, ortho=False, nthreads=1, out=None):
+    """pocketfft::dst<double> (pocketfft_hdronly.h:3302-3318)."""
+    return _r2r("ref_dst_f64", x, axes, type, fct, ortho, nthreads, out)

@@ -1,8 +1,9 @@
 // C-ABI shim over the reference's own PocketFFT C++ header, compiled in place
 // from the reference tree by oracle/Makefile into oracle/_ref/ (never copied
 // into this repository).  TEST INFRASTRUCTURE ONLY: it lets the tests and the
-// benchmark's CPU-baseline leg call pocketfft::c2c/r2c/c2r exactly as
-// impulse/fft/cpp_pocketfft/pocketfft.nim:35-69 binds them, with byte strides.
+// benchmark's CPU-baseline leg call pocketfft::c2c/r2c/c2r and dct/dst exactly
+// as impulse/fft/cpp_pocketfft/pocketfft.nim:35-69,108-132 binds them, with
+// byte strides.
 //
 // Return codes: 0 ok, 1 exception (invalid argument / runtime error).
 #include <complex>
@@ -76,6 +77,26 @@ int ref_r2c_f32(size_t ndim, const size_t *shape_in, const ptrdiff_t *sin, const
         pocketfft::r2c<float>(mk_shape(ndim, shape_in), mk_stride(ndim, sin), mk_stride(ndim, sout),
                               mk_shape(naxes, axes), forward != 0, in,
                               reinterpret_cast<std::complex<float> *>(out), fct, nthreads);
+    } catch (...) { return 1; }
+    return 0;
+}
+
+int ref_dct_f64(size_t ndim, const size_t *shape, const ptrdiff_t *sin, const ptrdiff_t *sout, size_t naxes,
+                const size_t *axes, int type, const double *in, double *out, double fct, int ortho, size_t nthreads)
+{
+    try {
+        pocketfft::dct<double>(mk_shape(ndim, shape), mk_stride(ndim, sin), mk_stride(ndim, sout),
+                               mk_shape(naxes, axes), type, in, out, fct, ortho != 0, nthreads);
+    } catch (...) { return 1; }
+    return 0;
+}
+
+int ref_dst_f64(size_t ndim, const size_t *shape, const ptrdiff_t *sin, const ptrdiff_t *sout, size_t naxes,
+                const size_t *axes, int type, const double *in, double *out, double fct, int ortho, size_t nthreads)
+{
+    try {
+        pocketfft::dst<double>(mk_shape(ndim, shape), mk_stride(ndim, sin), mk_stride(ndim, sout),
+                               mk_shape(naxes, axes), type, in, out, fct, ortho != 0, nthreads);
     } catch (...) { return 1; }
     return 0;
 }

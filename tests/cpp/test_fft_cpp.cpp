@@ -4,6 +4,8 @@
 //                           (raw C-path plan, in-place fft on a vector)
 //   tests/test_fft2.nim  -- exact known answers for fft([0,1,2,3]) and normalisations
 //   README.md:41-98      -- fft([1,2,1,-1,1.5]) and the C++ DataDesc/FFTDesc/apply example
+//   cpp_pocketfft/pocketfft.nim:322-337 -- DCTDesc(axes=[0]).apply on [4, 3, 5, 10]
+//                           (checked against the defining sum in long double)
 // Exit status 0 = all passed.  Usage: test_fft_cpp [maxlen] [eps]
 #include <cmath>
 #include <complex>
@@ -137,6 +139,28 @@ int main(int argc, char **argv)
         auto inv = FFTDesc<double>::init({0}, false, 1.0 / 5.0);
         apply(inv, backDesc, dOutDesc);
         CHECK(errcalc(back.data(), dIn.data(), 5) < 1e-15);
+    }
+    // cpp_pocketfft/pocketfft.nim:322-337: default DCT (type 2) of [4, 3, 5, 10]
+    {
+        const std::vector<double> dIn = {4.0, 3.0, 5.0, 10.0};
+        std::vector<double> dOut(dIn.size());
+        auto dInDesc = DataDesc<double>::init(const_cast<double *>(dIn.data()), std::vector<size_t>{dIn.size()});
+        auto dOutDesc = DataDesc<double>::init(dOut.data(), std::vector<size_t>{dOut.size()});
+        apply(DCTDesc<double>::init({0}), dOutDesc, dInDesc);
+        const long double pi = 3.141592653589793238462643383279502884L;
+        for (size_t k = 0; k < 4; ++k) {
+            long double want = 0;
+            for (size_t n = 0; n < 4; ++n) want += 2 * dIn[n] * std::cos(pi * k * (2 * n + 1) / 8.0L);
+            CHECK(std::fabs(dOut[k] - (double)want) < 1e-13);
+        }
+        CHECK(dOut[0] == 44.0);
+        // DST-II of the same values, then DST-III back: 2N x
+        std::vector<double> s(4), back(4);
+        auto sDesc = DataDesc<double>::init(s.data(), std::vector<size_t>{4});
+        auto bDesc = DataDesc<double>::init(back.data(), std::vector<size_t>{4});
+        apply(DCTDesc<double>::init({0}, 2, false, 1.0, 1, true), sDesc, dInDesc);
+        apply(DCTDesc<double>::init({0}, 3, false, 1.0 / 8.0, 1, true), bDesc, sDesc);
+        CHECK(errcalc(back.data(), dIn.data(), 4) < 1e-15);
     }
     // argument errors surface as exceptions (pocketfft_hdronly.h:446-477)
     {

@@ -70,7 +70,38 @@ def main():
     out = os.path.join(HERE, "golden.npz")
     np.savez_compressed(out, **g)
     print(out, os.path.getsize(out), "bytes,", len(g), "arrays")
+    r2r()
+
+
+R2R_LENGTHS = [1, 2, 3, 4, 5, 6, 7, 8, 13, 16, 17, 31, 32, 33, 64, 97, 100, 128, 255, 256, 1009, 2048]
+R2R_ORTHO = [2, 3, 5, 8, 17, 100]
+
+
+def r2r():
+    """pocketfft::dct / dst of every type on 1-D inputs (plain and ortho) and n-D views."""
+    g = {}
+    for n in R2R_LENGTHS:
+        x = synth.uniform((n,), -1.0, 1.0, seed=4000 + n)
+        g[f"r2r_{n}_in"] = x
+        for name, fn in (("dct", ref.dct), ("dst", ref.dst)):
+            for t in (1, 2, 3, 4):
+                if name == "dct" and t == 1 and n == 1:
+                    continue
+                g[f"{name}{t}_{n}"] = fn(x, [0], t)
+                if n in R2R_ORTHO:
+                    g[f"{name}{t}_{n}_ortho"] = fn(x, [0], t, 0.5, True)
+    v = synth.uniform((6, 9, 10), -1.0, 1.0, seed=11)
+    g["nd_r2r_in"] = v
+    for t in (1, 2, 3, 4):
+        g[f"nd_dct{t}_20"] = ref.dct(v[:, ::2, :], [2, 0], t, 0.25, t == 2)
+        g[f"nd_dst{t}_012"] = ref.dst(v[::-1], [0, 1, 2], t, 1.0, t == 3)
+    out = os.path.join(HERE, "golden_r2r.npz")
+    np.savez_compressed(out, **g)
+    print(out, os.path.getsize(out), "bytes,", len(g), "arrays")
 
 
 if __name__ == "__main__":
-    main()
+    if "--r2r" in sys.argv:
+        r2r()
+    else:
+        main()

@@ -1,7 +1,8 @@
 """Pins of the CPU oracle (oracle/fft_oracle.c) -- no GPU needed.
 
 Every oracle function is checked against (a) the golden vectors the
-reference's own PocketFFT wrote (tests/golden/golden.npz), (b) the reference
+reference's own PocketFFT wrote (tests/golden/golden.npz and, for the DCT /
+DST, tests/golden/golden_r2r.npz), (b) the reference
 library built in place (oracle/_ref, when present), (c) a long-double naive
 DFT, and (d) closed forms and the reference's known-answer tests, chosen so
 that a wrong sign, a dropped term, a transposed operand or a wrong scale
@@ -144,3 +145,118 @@ def test_oracle_rejects_bad_args():
         oracle.c2c(x, [1])        # bad axis number
     with pytest.raises(ValueError):
         oracle.c2c(np.zeros((2, 2), complex), [0, 0])   # axis specified repeatedly
+
+
+# ---------------------------------------------------------------------------
+# DCT / DST types 1-4 (pocketfft_hdronly.h:3284-3318)
+# ---------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def golden_r2r():
+    import os
+    from tests.conftest import ROOT
+    return np.load(os.path.join(ROOT, "tests", "golden", "golden_r2r.npz"))
+
+
+def _r2r_fn(name):
+    return oracle.dct if name == "dct" else oracle.dst
+
+
+def test_r2r_golden(golden_r2r):
+    g = golden_r2r
+    for key in g.files:
+        if not key.startswith(("dct", "dst")):
+            continue
+        name, t, n = key[:3], int(key[3]), int(key.split("_")[1])
+        ortho = key.endswith("_ortho")
+        y = _r2r_fn(name)(g[f"r2r_{n}_in"], [0], t, 0.5 if ortho else 1.0, ortho)
+        assert rel_l2(y, g[key]) < TOL, key
+    v = g["nd_r2r_in"]
+    for t in (1, 2, 3, 4):
+        assert rel_l2(oracle.dct(v[:, ::2, :], [2, 0], t, 0.25, t == 2), g[f"nd_dct{t}_20"]) < TOL, t
+        assert rel_l2(oracle.dst(v[::-1], [0, 1, 2], t, 1.0, t == 3), g[f"nd_dst{t}_012"]) < TOL, t
+
+
+def test_r2r_kat_nim():
+    # cpp_pocketfft/pocketfft.nim:322-337 applies DCTDesc(axes=[0]) (type 2) to [4, 3, 5, 10]
+    y = oracle.dct(np.array([4.0, 3.0, 5.0, 10.0]), [0], 2)
+    assert y[0] == 44.0
+    # X[2] = 2 * sum x[n] cos(pi (2n+1) / 4) = sqrt(2) * (4 - 3 - 5 + 10)
+    assert abs(y[2] - 6 * np.sqrt(2)) < 1e-14
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 8, 13, 16, 31])
+def test_r2r_inverse_pairs(n):
+    """Unnormalised pairs: I.I, II.III, III.II and IV.IV give x times 2(N-1), 2N or 2(N+1)."""
+    x = np.random.default_rng(n).uniform(-1, 1, n)
+    for name in ("dct", "dst"):
+        f = _r2r_fn(name)
+        for a, b, scale in ((1, 1, 2 * (n - 1) if name == "dct" else 2 * (n + 1)),
+                            (2, 3, 2 * n), (3, 2, 2 * n), (4, 4, 2 * n)):
+            y = f(f(x, [0], a), [0], b, 1.0 / scale)
+            assert rel_l2(y, x) < 1e-14, (name, a, b, n)
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 8, 17])
+def test_r2r_ortho_scalings(n):
+    """With ortho=True (end-point scalings only) and fct = 1/sqrt(2N') the DCTs of every type and DST-I / DST-IV
+    are orthonormal matrices; a missing or misplaced sqrt(2) breaks M^T M = I.  The reference's DST-II / DST-III
+    scale element 0 (output 0 of DST-II by 1/sqrt(2), input 0 of DST-III by sqrt(2),
+    pocketfft_hdronly.h:2519-2526 after the sine reversal) rather than element N-1, so those two are pinned
+    as that exact row / column scaling of the plain matrix."""
+    eye = np.eye(n)
+    for name in ("dct", "dst"):
+        f = _r2r_fn(name)
+        for t in (1, 2, 3, 4):
+            span = 2 * (n - 1) if (t == 1 and name == "dct") else 2 * (n + 1) if t == 1 else 2 * n
+            m = f(eye, [0], t, 1.0 / np.sqrt(span), True)
+            if name == "dst" and t in (2, 3):
+                plain = f(eye, [0], t, 1.0 / np.sqrt(span), False)
+                if t == 2:
+                    plain[0, :] *= np.sqrt(0.5)
+                else:
+                    plain[:, 0] *= np.sqrt(2.0)
+                assert np.max(np.abs(m - plain)) < 1e-15, (name, t, n)
+            else:
+                assert np.max(np.abs(m.T @ m - eye)) < 1e-14, (name, t, n)
+
+
+def test_r2r_via_fft_of_even_extension():
+    """DCT-II as the real part of a 2N-point DFT of [x, reversed x] rotated by e^{-i pi k / 2N}."""
+    for n in (1, 2, 7, 64, 97):
+        x = np.random.default_rng(n).uniform(-1, 1, n)
+        z = oracle.dft_naive(np.concatenate([x, x[::-1]]).astype(complex))[:n]
+        want = (np.exp(-1j * np.pi * np.arange(n) / (2 * n)) * z).real
+        assert np.max(np.abs(oracle.dct(x, [0], 2) - want)) < 1e-13 * max(1, n), n
+        # DST-I from the odd extension [0, x, 0, -reversed x]
+        z = oracle.dft_naive(np.concatenate([[0.0], x, [0.0], -x[::-1]]).astype(complex))[1:n + 1]
+        assert np.max(np.abs(oracle.dst(x, [0], 1) + z.imag)) < 1e-13 * max(1, n), n
+
+
+def test_r2r_constant_input():
+    n = 9
+    y = oracle.dct(np.ones(n), [0], 2)
+    assert y[0] == 2 * n and np.max(np.abs(y[1:])) < 1e-14
+    y = oracle.dct(np.ones(n), [0], 1)
+    assert abs(y[0] - 2 * (n - 1)) < 1e-14 and np.max(np.abs(y[1:])) < 1e-14
+
+
+def test_r2r_rejects_bad_args():
+    with pytest.raises(ValueError):
+        oracle.dct(np.zeros(1), [0], 1)     # DCT-I needs N >= 2 (pocketfft_hdronly.h:2431)
+    with pytest.raises(ValueError):
+        oracle.dct(np.zeros(4), [0], 5)
+    with pytest.raises(ValueError):
+        oracle.dst(np.zeros(4), [1], 2)
+
+
+@pytest.mark.skipif(not ref.available(), reason="reference PocketFFT not built here")
+def test_r2r_against_reference_library():
+    rng = np.random.default_rng(12)
+    for n in list(range(1, 70)) + [127, 128, 500, 1023]:
+        x = rng.uniform(-1, 1, n)
+        for ortho in (False, True):
+            for t in (1, 2, 3, 4):
+                if not (t == 1 and n == 1):
+                    assert rel_l2(oracle.dct(x, [0], t, 0.7, ortho), ref.dct(x, [0], t, 0.7, ortho)) < TOL, (n, t)
+                assert rel_l2(oracle.dst(x, [0], t, 0.7, ortho), ref.dst(x, [0], t, 0.7, ortho)) < TOL, (n, t)

@@ -30,11 +30,14 @@ EXPORTS = (
     "impulse_fft_mfma_create", "impulse_fft_mfma_execute", "impulse_fft_mfma_workspace_bytes",
     "impulse_fft_mfma_destroy",
     "impulse_fft_dct", "impulse_fft_dst", "impulse_fft_r2r_plan_create",
+    "impulse_fft_upfirdn_create", "impulse_fft_upfirdn_info", "impulse_fft_upfirdn_execute",
+    "impulse_fft_upfirdn_execute_host", "impulse_fft_upfirdn_destroy",
 )
 
 C2C, R2C, C2R, R2C_ADJOINT, C2R_ADJOINT, DCT, DST = 0, 1, 2, 3, 4, 5, 6
 F64, F32 = 0, 1
 MFMA_F32, MFMA_BF16X = 0, 1
+UPFIRDN_AUTO, UPFIRDN_DIRECT, UPFIRDN_FFT = 0, 1, 2
 
 _szp = ctypes.POINTER(ctypes.c_size_t)
 _pdp = ctypes.POINTER(ctypes.c_ssize_t)
@@ -87,6 +90,11 @@ def _sig(L):
     for n in ("impulse_fft_dct", "impulse_fft_dst"):
         getattr(L, n).argtypes = [sz, _szp, _pdp, _pdp, sz, _szp, i, i, _vp, _vp, d, i, sz]
     L.impulse_fft_r2r_plan_create.argtypes = [ctypes.POINTER(_vp), i, i, i, i, sz, _szp, _pdp, _pdp, sz, _szp]
+    L.impulse_fft_upfirdn_create.argtypes = [ctypes.POINTER(_vp), sz, sz, _vp, sz, i, i, i]
+    L.impulse_fft_upfirdn_info.argtypes = [_vp, _szp, ctypes.POINTER(i), _szp, _szp]
+    L.impulse_fft_upfirdn_execute.argtypes = [_vp, _vp, _vp, _vp]
+    L.impulse_fft_upfirdn_execute_host.argtypes = [_vp, _vp, _vp]
+    L.impulse_fft_upfirdn_destroy.argtypes = [_vp]
     L.impulse_fft_status_string.argtypes = [i]
     L.impulse_fft_status_string.restype = ctypes.c_char_p
     L.impulse_fft_last_error.restype = ctypes.c_char_p

@@ -30,6 +30,11 @@ struct impulse_fft_mfma_s {
     ~impulse_fft_mfma_s() { ifft::mfma_destroy(m); }
 };
 
+struct impulse_fft_upfirdn_s {
+    ifft::Upfirdn *u = nullptr;
+    ~impulse_fft_upfirdn_s() { ifft::upfirdn_destroy(u); }
+};
+
 struct impulse_cfft_plan_s {
     size_t length = 0;
     std::unique_ptr<ifft::Plan> fwd, bwd;
@@ -343,6 +348,57 @@ int impulse_fft_cache_clear(void)
     std::lock_guard<std::mutex> lk(c.mtx);
     c.lru.clear();
     c.map.clear();
+    return ifft::OK;
+}
+
+// ---- upfirdn --------------------------------------------------------------------
+
+int impulse_fft_upfirdn_create(impulse_fft_upfirdn *plan, size_t batch, size_t nt, const double *h, size_t nh,
+                               int up, int down, int algorithm)
+{
+    if (!plan) return fail(ifft::E_INVALID, "plan pointer is NULL");
+    *plan = nullptr;
+    ifft::Error err{0, ""};
+    ifft::Upfirdn *u = nullptr;
+    int rc = ifft::upfirdn_create(&u, batch, nt, h, nh, up, down, algorithm, err);
+    if (rc != ifft::OK) return fail(err);
+    auto *p = new (std::nothrow) impulse_fft_upfirdn_s;
+    if (!p) {
+        ifft::upfirdn_destroy(u);
+        return fail(ifft::E_NOMEM, "out of host memory");
+    }
+    p->u = u;
+    *plan = p;
+    return ifft::OK;
+}
+
+int impulse_fft_upfirdn_info(impulse_fft_upfirdn plan, size_t *nout, int *algorithm, size_t *fft_length,
+                             size_t *blocks)
+{
+    if (!plan || !plan->u) return fail(ifft::E_INVALID, "plan is NULL");
+    ifft::upfirdn_info(*plan->u, nout, algorithm, fft_length, blocks);
+    return ifft::OK;
+}
+
+int impulse_fft_upfirdn_execute(impulse_fft_upfirdn plan, const double *t, double *y, void *stream)
+{
+    if (!plan || !plan->u) return fail(ifft::E_INVALID, "plan is NULL");
+    ifft::Error err{0, ""};
+    int rc = ifft::upfirdn_execute(*plan->u, t, y, (hipStream_t)stream, err);
+    return rc == ifft::OK ? rc : fail(err);
+}
+
+int impulse_fft_upfirdn_execute_host(impulse_fft_upfirdn plan, const double *t, double *y)
+{
+    if (!plan || !plan->u) return fail(ifft::E_INVALID, "plan is NULL");
+    ifft::Error err{0, ""};
+    int rc = ifft::upfirdn_execute_host(*plan->u, t, y, err);
+    return rc == ifft::OK ? rc : fail(err);
+}
+
+int impulse_fft_upfirdn_destroy(impulse_fft_upfirdn plan)
+{
+    delete plan;
     return ifft::OK;
 }
 

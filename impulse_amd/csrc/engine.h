@@ -118,6 +118,15 @@ int conv2d_backward(Conv2d &c, const void *x, const void *k, const void *gy, voi
 void conv2d_grid(const Conv2d &c, long long &P, long long &Q);
 void conv2d_destroy(Conv2d *c);
 
+// upfirdn: upsample, FIR filter, downsample (upfirdn.hip)
+struct Upfirdn;
+int upfirdn_create(Upfirdn **out, size_t batch, size_t nt, const double *h, size_t nh, int up, int down, int algo,
+                   Error &err);
+int upfirdn_execute(Upfirdn &U, const double *t, double *y, hipStream_t s, Error &err);
+int upfirdn_execute_host(Upfirdn &U, const double *t, double *y, Error &err);
+void upfirdn_info(const Upfirdn &U, size_t *nout, int *algo, size_t *fft_length, size_t *blocks);
+void upfirdn_destroy(Upfirdn *u);
+
 // ragged batches of real transforms (ragged.hip)
 struct Ragged;
 int ragged_create(Ragged **out, size_t count, const size_t *lengths, const size_t *in_off, const size_t *out_off,

@@ -192,6 +192,38 @@ int impulse_fft_conv2d_grid(impulse_fft_conv2d conv, size_t *P, size_t *Q);
 int impulse_fft_conv2d_destroy(impulse_fft_conv2d conv);
 
 /* ------------------------------------------------------------------------- */
+/* upfirdn: upsample, FIR filter, downsample (impulse/signal.nim:545-594)     */
+/* ------------------------------------------------------------------------- */
+
+/* Replaces the body of impulse/signal.nim:545-594, `upfirdn(t, h, up, down)`
+ * = t.upsample(up).convolve(h, mode = full, down = down), which resample
+ * (impulse/signal.nim:690-744, 746-809) calls.  For each of `batch` signals
+ * of length nt, the upsampled signal (up-1 zeros after every sample, length
+ * (nt-1) up + 1) is convolved in full with the nh taps h, and every down-th
+ * sample from the first is kept: nout = ceil(((nt-1) up + nh) / down)
+ * outputs per signal.  FP64 throughout.  The taps are copied at creation
+ * (host pointer).  algorithm: IMPULSE_FFT_UPFIRDN_AUTO picks by a cost
+ * model, _DIRECT sums the polyphase taps per output, _FFT runs overlap-save
+ * blocks through the engine's r2c / c2r plans. */
+enum impulse_fft_upfirdn_algorithm {
+    IMPULSE_FFT_UPFIRDN_AUTO = 0,
+    IMPULSE_FFT_UPFIRDN_DIRECT = 1,
+    IMPULSE_FFT_UPFIRDN_FFT = 2
+};
+typedef struct impulse_fft_upfirdn_s *impulse_fft_upfirdn;
+int impulse_fft_upfirdn_create(impulse_fft_upfirdn *plan, size_t batch, size_t nt, const double *h, size_t nh,
+                               int up, int down, int algorithm);
+/* Outputs per signal, the algorithm chosen, and for _FFT the block FFT length and blocks per signal (0 otherwise). */
+int impulse_fft_upfirdn_info(impulse_fft_upfirdn plan, size_t *nout, int *algorithm, size_t *fft_length,
+                             size_t *blocks);
+/* t: [batch, nt], y: [batch, nout], C-contiguous DEVICE pointers; asynchronous on `stream`.
+ * Calls on one plan must be ordered by the caller (the FFT algorithm keeps its workspace in the plan). */
+int impulse_fft_upfirdn_execute(impulse_fft_upfirdn plan, const double *t, double *y, void *stream);
+/* Same on HOST pointers; synchronous. */
+int impulse_fft_upfirdn_execute_host(impulse_fft_upfirdn plan, const double *t, double *y);
+int impulse_fft_upfirdn_destroy(impulse_fft_upfirdn plan);
+
+/* ------------------------------------------------------------------------- */
 /* Ragged batches of real transforms (no PocketFFT counterpart)               */
 /* ------------------------------------------------------------------------- */
 

@@ -154,3 +154,28 @@ def dct(x, axes, type=2, fct=1.0, ortho=False, out=None):
 def dst(x, axes, type=2, fct=1.0, ortho=False, out=None):
     """pocketfft::dst semantics (naive long-double sums)."""
     return _r2r(lib().oracle_dst, "oracle_dst", x, axes, type, fct, ortho, out)
+
+
+def upfirdn(t, h, up=1, down=1):
+    """impulse/signal.nim:545-594 on 1-D float64 arrays (long-double direct sums)."""
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    L = lib()
+    L.oracle_upfirdn_length.argtypes = [ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]
+    L.oracle_upfirdn_length.restype = ctypes.c_size_t
+    L.oracle_upfirdn.argtypes = [ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_void_p]
+    L.oracle_upfirdn.restype = ctypes.c_int
+    n = L.oracle_upfirdn_length(t.size, h.size, int(up), int(down))
+    y = np.empty(n, np.float64)
+    _check(L.oracle_upfirdn(t.size, t.ctypes.data, h.size, h.ctypes.data, int(up), int(down), y.ctypes.data),
+           "oracle_upfirdn")
+    return y
+
+
+def upfirdn_length(nt, nh, up=1, down=1):
+    """Outputs of upfirdn: ceil(((nt - 1) up + nh) / down)."""
+    L = lib()
+    L.oracle_upfirdn_length.argtypes = [ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]
+    L.oracle_upfirdn_length.restype = ctypes.c_size_t
+    return int(L.oracle_upfirdn_length(int(nt), int(nh), int(up), int(down)))

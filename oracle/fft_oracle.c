@@ -620,3 +620,28 @@ int oracle_dst(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in, con
 {
     return r2r_nd(ndim, shape, stride_in, stride_out, naxes, axes, type, 0, data_in, data_out, fct, ortho);
 }
+
+size_t oracle_upfirdn_length(size_t nt, size_t nh, int up, int down)
+{
+    if (nt < 1 || nh < 1 || up < 1 || down < 1) return 0;
+    const size_t lc = (nt - 1) * (size_t)up + nh;
+    return (lc + (size_t)down - 1) / (size_t)down;
+}
+
+int oracle_upfirdn(size_t nt, const double *t, size_t nh, const double *h, int up, int down, double *y)
+{
+    const size_t nout = oracle_upfirdn_length(nt, nh, up, down);
+    if (!nout || !t || !h || !y) return 1;
+    for (size_t j = 0; j < nout; ++j) {
+        const size_t m = j * (size_t)down;          /* index into the full convolution */
+        long double acc = 0.0L;
+        for (size_t i = 0; i < nh && i <= m; ++i) {  /* c[m] = sum_i u[m - i] h[i] */
+            const size_t mu = m - i;
+            if (mu % (size_t)up) continue;
+            const size_t q = mu / (size_t)up;
+            if (q < nt) acc += (long double)t[q] * (long double)h[i];
+        }
+        y[j] = (double)acc;
+    }
+    return 0;
+}

@@ -84,6 +84,14 @@ int oracle_dst(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in, con
                size_t naxes, const size_t *axes, int type, const double *data_in, double *data_out, double fct,
                int ortho);
 
+/* upfirdn (impulse/signal.nim:545-594): t upsampled by `up` (up-1 zeros after
+ * every sample, length (nt-1) up + 1, the zero insertion the reference's KATs
+ * at tests/test_signal.nim:67-85 fix), convolved in full with the nh taps h,
+ * every down-th sample from the first kept; y holds
+ * ceil(((nt-1) up + nh) / down) values, each a long-double direct sum. */
+size_t oracle_upfirdn_length(size_t nt, size_t nh, int up, int down);
+int oracle_upfirdn(size_t nt, const double *t, size_t nh, const double *h, int up, int down, double *y);
+
 /* Twiddle exp(sign * 2 pi i k / n) in long double, exact at multiples of pi/4
  * (returned through re/im as doubles of the long-double value). */
 void oracle_twiddle(size_t k, size_t n, int sign, double *re, double *im);

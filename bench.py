@@ -39,7 +39,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-forms", action="store_true", help="skip the ragged-batch form")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the forms and widened paths (ragged, MFMA, convolution, DCT, upfirdn)")
+    ap.add_argument("--only-forms", default="",
+                    help="comma list of forms to run alone (ragged,mfma,conv,dct,upfirdn), no drop-in step")
     ap.add_argument("--form-reps", type=int, default=10)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (the reference "
@@ -302,6 +305,85 @@ def conv_form(torch, reps):
             "note": "backward recomputes the image and kernel spectra instead of storing them"}
 
 
+def _timed(torch, fn, reps):
+    for _ in range(2):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def dct_form(torch, reps):
+    """DCT-II (pocketfft::dct type 2) of a float64 4096 x 4096 array over axes [0, 1] and over its rows,
+    uniform[-1, 1].  Compulsory bytes: the array read once and written once per call.  Accuracy:
+    DCT-III of the result divided by (2N)^d returns the input."""
+    from impulse_amd import _lib, synth
+    from impulse_amd import fft as F
+
+    n = 4096
+    x = synth.uniform_device(n * n, -1.0, 1.0, seed=21, dtype=torch.float64).view(n, n)
+    y = torch.empty_like(x)
+    z = torch.empty_like(x)
+    sp = torch.cuda.current_stream().cuda_stream
+    st = [n * 8, 8]
+    out = {"form": "dct", "workload": "DCT-II f64 4096 x 4096", "cases": []}
+    for axes in ([0, 1], [1]):
+        p2 = F.Plan(_lib.DCT, [n, n], st, st, axes, True, r2r_type=2)
+        p3 = F.Plan(_lib.DCT, [n, n], st, st, axes, True, r2r_type=3)
+        ms = _timed(torch, lambda: p2.execute(x.data_ptr(), y.data_ptr(), 1.0, sp), reps)
+        p3.execute(y.data_ptr(), z.data_ptr(), 1.0 / (2.0 * n) ** len(axes), sp)
+        torch.cuda.synchronize()
+        err = float(torch.linalg.vector_norm(z - x) / torch.linalg.vector_norm(x))
+        comp = 2 * n * n * 8
+        out["cases"].append({"axes": axes, "ms": round(ms, 3), "launches": p2.num_launches,
+                             "compulsory_GBs": round(comp / (ms * 1e-3) / 1e9, 1),
+                             "hbm_roofline_frac": round(comp / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                             "round_trip_rel_l2": err})
+        del p2, p3
+    return out
+
+
+def upfirdn_form(torch, reps):
+    """upfirdn / resample (impulse/signal.nim:545-809) on float64 signals, uniform[-1, 1]:
+    resample by 3/2 of 2^24 samples with the generated 61-tap filter (direct algorithm), and a
+    4001-tap FIR over 2^22 samples (overlap-save FFT algorithm, against the direct one).
+    Compulsory bytes: the signal read once, the output written once."""
+    from impulse_amd import _lib, synth
+    from impulse_amd import signal as S
+
+    out = {"form": "upfirdn", "cases": []}
+    rng = np.random.default_rng(31)
+    cases = [("resample 3/2, 2^24 samples, 61-tap generated filter", 1 << 24, None, 3, 2),
+             ("FIR 4001 taps, 2^22 samples", 1 << 22, rng.standard_normal(4001) / 64.0, 1, 1)]
+    for name, nt, h, up, down in cases:
+        if h is None:
+            h0 = S.generate_resampling_filter(up, down)
+            h, _ = S.adjust_filter_position(h0, nt, int(math.ceil(nt * up / down)), up, down)
+        t = synth.uniform_device(nt, -1.0, 1.0, seed=32, dtype=torch.float64)
+        res = {"case": name, "taps": int(h.size), "up": up, "down": down}
+        ys = {}
+        for algo, label in ((_lib.UPFIRDN_DIRECT, "direct"), (_lib.UPFIRDN_FFT, "fft")):
+            p = S.UpfirdnPlan(nt, h, up, down, algorithm=algo)
+            y = torch.empty((1, p.nout), dtype=torch.float64, device="cuda")
+            ms = _timed(torch, lambda: p.execute(t, y), reps)
+            comp = (nt + p.nout) * 8
+            res[label] = {"ms": round(ms, 3), "compulsory_GBs": round(comp / (ms * 1e-3) / 1e9, 1),
+                          "hbm_roofline_frac": round(comp / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                          "fft_length": p.fft_length, "blocks": p.blocks}
+            ys[label] = y
+            del p
+        res["auto"] = "fft" if S.UpfirdnPlan(nt, h, up, down).algorithm == _lib.UPFIRDN_FFT else "direct"
+        res["direct_vs_fft_rel_l2"] = float(torch.linalg.vector_norm(ys["fft"] - ys["direct"]) /
+                                            torch.linalg.vector_norm(ys["direct"]))
+        out["cases"].append(res)
+    return out
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -320,6 +402,11 @@ def main():
     torch.cuda.set_device(0)
     L = _lib.lib()
     _lib.check(L.impulse_fft_device_check())
+    form_fns = {"ragged": ragged_form, "mfma": mfma_form, "conv": conv_form, "dct": dct_form,
+                "upfirdn": upfirdn_form}
+    if args.only_forms:
+        print(json.dumps({"forms": [form_fns[f](torch, args.form_reps) for f in args.only_forms.split(",")]}))
+        return
     plist = parts()
 
     # device-resident inputs and outputs, plans
@@ -402,7 +489,8 @@ def main():
                           "launches": npass})
 
     forms = [] if args.no_forms else [ragged_form(torch, args.form_reps), mfma_form(torch, args.form_reps),
-                                       conv_form(torch, args.form_reps)]
+                                       conv_form(torch, args.form_reps), dct_form(torch, args.form_reps),
+                                       upfirdn_form(torch, args.form_reps)]
     cpu = None if args.no_cpu else cpu_baseline(plist, args.cpu_seconds)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,

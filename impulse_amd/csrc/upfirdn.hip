@@ -24,31 +24,94 @@
 
 namespace ifft {
 
-constexpr int kUfdLdsTaps = 4096;   // taps staged in LDS by the direct kernel (32 KiB)
+constexpr int kUfdTileR = 4;             // outputs per thread of the tiled direct kernel
+constexpr int kUfdLdsBytes = 64 * 1024;  // LDS per workgroup of the tiled kernel (two per CU)
 
-template <bool LDS_H>
+// Polyphase sum of output j: with m = j down, qhi = floor(m / up) and phase
+// r = m - qhi up, y[j] = sum_l t[qhi - l] h[r + l up] over the taps l with
+// r + l up < nh and 0 <= qhi - l < nt.  `ts` holds t from index q0 on.
+__device__ __forceinline__ double ufd_output(const double *ts, long long q0, const double *hs, long long j,
+                                             long long nt, int nh, int up, int down)
+{
+    const long long m = j * down;
+    long long qhi;
+    int r;
+    if (m <= 0xffffffffLL) {
+        const unsigned mm = (unsigned)m, q = mm / (unsigned)up;
+        qhi = q;
+        r = (int)(mm - q * (unsigned)up);
+    } else {
+        qhi = m / up;
+        r = (int)(m - qhi * up);
+    }
+    if (r >= nh) return 0.0;
+    long long lmax = (nh - 1 - r) / up;                     // last tap of this phase
+    if (lmax > qhi) lmax = qhi;                             // t[qhi - l] exists only for l <= qhi
+    const long long l0 = qhi > nt - 1 ? qhi - (nt - 1) : 0; // taps past the end of t
+    const double *tp = ts + (qhi - l0 - q0);
+    const double *hp = hs + r + l0 * up;
+    int c = (int)(lmax - l0 + 1);
+    // four independent chains keep several loads in flight per lane
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    for (; c >= 4; c -= 4) {
+        const double t0 = tp[0], t1 = tp[-1], t2 = tp[-2], t3 = tp[-3];
+        const double h0 = hp[0], h1 = hp[up], h2 = hp[2 * up], h3 = hp[3 * up];
+        a0 = fma(t0, h0, a0);
+        a1 = fma(t1, h1, a1);
+        a2 = fma(t2, h2, a2);
+        a3 = fma(t3, h3, a3);
+        tp -= 4;
+        hp += 4 * up;
+    }
+    for (; c > 0; --c) {
+        a0 = fma(*tp, *hp, a0);
+        --tp;
+        hp += up;
+    }
+    return (a0 + a1) + (a2 + a3);
+}
+
+// Tiled polyphase kernel: workgroup (blockIdx.x, row blockIdx.y) computes the
+// outputs J0 .. J0 + 256 R - 1 from LDS copies of the taps and of the signal
+// segment those outputs read (its length, seg_cap, is bounded on the host).
+template <int R>
+__global__ void __launch_bounds__(256) k_upfirdn_tile(const double *__restrict__ t, const double *__restrict__ h,
+                                                      double *__restrict__ y, long long nt, int nh, int up, int down,
+                                                      long long nout, int seg_cap)
+{
+    extern __shared__ double sm[];
+    double *hs = sm;
+    double *ts = sm + nh;
+    const double *tb = t + (long long)blockIdx.y * nt;
+    double *yb = y + (long long)blockIdx.y * nout;
+    const long long J0 = (long long)blockIdx.x * (256 * R);
+    const long long J1 = (J0 + 256 * R < nout ? J0 + 256 * R : nout) - 1;
+    const long long first = J0 * down - (nh - 1);
+    const long long q0 = first <= 0 ? 0 : first / up;
+    long long q1 = J1 * down / up;
+    if (q1 > nt - 1) q1 = nt - 1;
+    const int nseg = (int)(q1 - q0 + 1);
+    for (int i = threadIdx.x; i < nh; i += 256) hs[i] = h[i];
+    for (int i = threadIdx.x; i < nseg && i < seg_cap; i += 256) ts[i] = tb[q0 + i];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const long long j = J0 + k * 256 + threadIdx.x;
+        if (j < nout) yb[j] = ufd_output(ts, q0, hs, j, nt, nh, up, down);
+    }
+}
+
+// Fallback for filters or rate ratios whose tile does not fit LDS: taps and
+// signal read from global memory, one output per thread.
 __global__ void __launch_bounds__(256) k_upfirdn_direct(const double *__restrict__ t, const double *__restrict__ h,
                                                         double *__restrict__ y, long long nt, int nh, int up, int down,
-                                                        long long nout, long long total)
+                                                        long long nout)
 {
-    __shared__ double hs[LDS_H ? kUfdLdsTaps : 1];
-    if constexpr (LDS_H) {
-        for (int i = threadIdx.x; i < nh; i += blockDim.x) hs[i] = h[i];
-        __syncthreads();
-    }
-    const double *taps = LDS_H ? hs : h;
-    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (long long)gridDim.x * blockDim.x) {
-        const long long b = e / nout, j = e - b * nout;
-        const long long m = j * down;
-        const long long qhi = std::min(m / up, nt - 1);
-        const long long first = m - (nh - 1);
-        const long long qlo = first <= 0 ? 0 : (first + up - 1) / up;
-        const double *tb = t + b * nt;
-        double acc = 0.0;
-        for (long long q = qlo; q <= qhi; ++q) acc = fma(tb[q], taps[m - q * up], acc);
-        y[e] = acc;
-    }
+    const double *tb = t + (long long)blockIdx.y * nt;
+    double *yb = y + (long long)blockIdx.y * nout;
+    for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < nout;
+         j += (long long)gridDim.x * blockDim.x)
+        yb[j] = ufd_output(tb, 0, h, j, nt, nh, up, down);
 }
 
 __global__ void k_upfirdn_gather(const double *__restrict__ t, double *__restrict__ seg, long long nt, int up,
@@ -109,6 +172,7 @@ struct Upfirdn {
     long long batch = 0, nt = 0, nh = 0, nout = 0, Lu = 0, Lc = 0;
     int up = 1, down = 1, algo = 1;
     DevBuf h;                                  // taps (direct)
+    int seg_cap = 0;                           // signal points per tile in LDS (0: untiled direct kernel)
     long long P = 0, B = 0, nblk = 0, nspec = 0;
     std::unique_ptr<Plan> r2c, c2r;
     DevBuf hspec, seg, spec;
@@ -142,6 +206,12 @@ static void fft_shape(Upfirdn &U)
     U.nspec = U.P / 2 + 1;
 }
 
+static bool tiles_fit(const Upfirdn &U)
+{
+    const long long tiles = (U.nout + 256 * kUfdTileR - 1) / (256 * kUfdTileR);
+    return tiles < (1LL << 31);
+}
+
 static double fft_cost(const Upfirdn &U)
 {
     const double rows = (double)U.batch * U.nblk;
@@ -155,7 +225,7 @@ int upfirdn_create(Upfirdn **out, size_t batch, size_t nt, const double *h, size
         err = {E_INVALID, "upfirdn needs batch, signal and filter lengths >= 1, up and down >= 1, algorithm 0-2"};
         return E_INVALID;
     }
-    if (nh > (size_t)INT32_MAX || nt > (size_t)1 << 40) {
+    if (nh > (size_t)INT32_MAX || nt > (size_t)1 << 40 || batch > 65535) {
         err = {E_UNSUPPORTED, "upfirdn signal or filter too long"};
         return E_UNSUPPORTED;
     }
@@ -172,6 +242,11 @@ int upfirdn_create(Upfirdn **out, size_t batch, size_t nt, const double *h, size
     U->nout = (U->Lc + down - 1) / down;
     fft_shape(*U);
     U->algo = algo ? algo : (fft_cost(*U) < direct_cost(*U) ? 2 : 1);
+    {
+        // signal points a tile of 256 R outputs reads: floor(J1 down / up) - floor((J0 down - nh + 1) / up) + 1
+        const long long span = ((256LL * kUfdTileR - 1) * down + U->nh - 1) / up + 2;
+        if ((U->nh + span) * 8 <= kUfdLdsBytes && tiles_fit(*U)) U->seg_cap = (int)span;
+    }
     if (U->algo == 1) {
         if (U->h.alloc(nh * 8) || hipMemcpy(U->h.p, h, nh * 8, hipMemcpyHostToDevice) != hipSuccess) {
             err = {E_NOMEM, "upfirdn filter upload failed"};
@@ -229,12 +304,15 @@ int upfirdn_execute(Upfirdn &U, const double *t, double *y, hipStream_t s, Error
     }
     const long long total = U.batch * U.nout;
     if (U.algo == 1) {
-        if (U.nh <= kUfdLdsTaps)
-            hipLaunchKernelGGL(k_upfirdn_direct<true>, dim3(ufd_blocks(total)), dim3(256), 0, s, t,
-                               (const double *)U.h.p, y, U.nt, (int)U.nh, U.up, U.down, U.nout, total);
-        else
-            hipLaunchKernelGGL(k_upfirdn_direct<false>, dim3(ufd_blocks(total)), dim3(256), 0, s, t,
-                               (const double *)U.h.p, y, U.nt, (int)U.nh, U.up, U.down, U.nout, total);
+        if (U.seg_cap > 0) {
+            const long long tiles = (U.nout + 256 * kUfdTileR - 1) / (256 * kUfdTileR);
+            const size_t lds = (size_t)(U.nh + U.seg_cap) * 8;
+            hipLaunchKernelGGL(k_upfirdn_tile<kUfdTileR>, dim3((unsigned)tiles, (unsigned)U.batch), dim3(256), lds, s,
+                               t, (const double *)U.h.p, y, U.nt, (int)U.nh, U.up, U.down, U.nout, U.seg_cap);
+        } else {
+            hipLaunchKernelGGL(k_upfirdn_direct, dim3(ufd_blocks(U.nout), (unsigned)U.batch), dim3(256), 0, s, t,
+                               (const double *)U.h.p, y, U.nt, (int)U.nh, U.up, U.down, U.nout);
+        }
         UFD_TRY_HIP(hipGetLastError());
         return OK;
     }

@@ -3,24 +3,42 @@
 // sums, `ortho` applying only the end-point scalings the reference applies
 // (the overall 1/sqrt factor is the caller's fct), fct on the first axis.
 //
-// Each axis of length N is one c2c transform per line, of length L = 2N (types
-// 2-4) or of the even / odd extension (DCT-I: 2(N-1), DST-I: 2(N+1)), run by
-// the engine's own FP64 plan over a contiguous workspace, between a pre-pass
-// that builds the complex sequence from the strided input and a post-pass that
-// forms the outputs:
-//   DCT-I    z = even extension of x            forward   y[k] = Re Z[k]
-//   DST-I    z = odd extension (0, x, 0, -x~)   forward   y[k] = -Im Z[k+1]
-//   DCT-II   z = x, zero-padded                 forward   y[k] = 2 Re(e^{-i pi k/2N} Z[k])
-//   DST-II   z = x, zero-padded                 forward   y[k] = -2 Im(e^{-i pi (k+1)/2N} Z[k+1])
-//   DCT-III  z[n] = x'[n] e^{i pi n/2N}, x'[0] = x[0], x'[n] = 2 x[n]
-//                                               backward  y[k] = Re c[k]
+// Each axis of length N runs one c2c transform per line through the engine's
+// own FP64 plan over a workspace, between a pre-pass that builds the complex
+// sequence from the strided input and a post-pass that forms the outputs.
+//
+// Types 2-4 at even N (M = N/2 points per line, after Makhoul): with x' the
+// input as the DCT sees it -- x'[n] = (-1)^n x[n] for DST-II, x'[n] = x[N-1-n]
+// for DST-III / DST-IV (DST-II(x)[k] = DCT-II(x')[N-1-k], DST-III(x)[k] =
+// (-1)^k DCT-III(x')[k], DST-IV(x)[k] = (-1)^k DCT-IV(x')[k]) --
+//   DCT-II   z[p] = v[2p] + i v[2p+1], v[q] = x'[2q] (q < M), x'[2N-2q-1] (q >= M)
+//            forward; V[k] = E[k] + e^{-2 pi i k/N} O[k] from Z[k], conj Z[M-k];
+//            W = e^{-i pi k/2N} V[k]; y[k] = 2 Re W, y[N-k] = -2 Im W (k = 0..M)
+//   DCT-III  V(k) = e^{i pi k/2N} (x'[k] - i x'[N-k]) (x'[N] = 0), k = 0..M;
+//            z[k] = (V(k) + conj V(M-k)) + i e^{2 pi i k/N} (V(k) - conj V(M-k));
+//            backward; v[2n] = Re Z[n], v[2n+1] = Im Z[n]; y[2q] = v[q],
+//            y[2q+1] = v[N-1-q]
+//   DCT-IV   z[p] = (x'[2p] + i x'[N-1-2p]) e^{-i pi p/N}; forward;
+//            S = e^{-i pi (4k+1)/4N} Z[k]; y[2k] = 2 Re S, y[N-1-2k] = -2 Im S
+// Types 1 and odd N (L points per line):
+//   DCT-I    z = even extension of x, L = 2(N-1)  forward   y[k] = Re Z[k]
+//   DST-I    z = (0, x, 0, -x~), L = 2(N+1)       forward   y[k] = -Im Z[k+1]
+//   DCT-II   z = x, zero-padded, L = 2N           forward   y[k] = 2 Re(e^{-i pi k/2N} Z[k])
+//   DST-II   z = x, zero-padded                   forward   y[k] = -2 Im(e^{-i pi (k+1)/2N} Z[k+1])
+//   DCT-III  z[n] = x''[n] e^{i pi n/2N}, x''[0] = x[0], x''[n] = 2 x[n]
+//                                                 backward  y[k] = Re c[k]
 //   DST-III  z[m] = x''[m] e^{i pi m/2N}, m = 1..N, x''[m] = 2 x[m-1] (m < N), x''[N] = x[N-1]
-//                                               backward  y[k] = Im c[k]
-//   DCT-IV   z[n] = x[n] e^{-i pi (2n+1)/4N}    forward   y[k] = 2 Re(e^{-i pi k/2N} Z[k])
-//   DST-IV   as DCT-IV                          forward   y[k] = -2 Im(e^{-i pi k/2N} Z[k])
+//                                                 backward  y[k] = Im c[k]
+//   DCT-IV   z[n] = x[n] e^{-i pi (2n+1)/4N}      forward   y[k] = 2 Re(e^{-i pi k/2N} Z[k])
+//   DST-IV   as DCT-IV                            forward   y[k] = -2 Im(e^{-i pi k/2N} Z[k])
 // (each follows from writing the cosine / sine as the real / imaginary part of
-// one exponential and collecting the length-2N DFT kernel).  Twiddles come from
-// FP64 sincospi.
+// one exponential and collecting the DFT kernel).  Twiddles come from FP64
+// sincospi.
+//
+// Workspace layout per axis: [lines, L] when the axis is the input's unit-
+// stride dimension (threads walk along a line), else [L, lines] (threads walk
+// across lines, which keeps the strided gathers and scatters coalesced; the
+// c2c then runs over the workspace's strided axis).
 #include <cmath>
 #include <vector>
 
@@ -46,8 +64,27 @@ struct R2rParams {
     long long sa_in, sa_out;   // element stride along the axis
     long long N, L, lines;
     int cosine, type, ortho;
+    int half;                  // types 2-4 at even N: L = N/2 points per line
+    int col;                   // workspace [L, lines] (line fastest) instead of [lines, L]
     double fct;
 };
+
+// (line, point) of work item e and the workspace index of that point
+__device__ __forceinline__ void r2r_item(const R2rParams &P, long long e, long long cnt, long long &line, long long &m)
+{
+    if (P.col) {
+        m = e / P.lines;
+        line = e - m * P.lines;
+    } else {
+        line = e / cnt;
+        m = e - line * cnt;
+    }
+}
+
+__device__ __forceinline__ long long r2r_widx(const R2rParams &P, long long line, long long m)
+{
+    return P.col ? m * P.lines + line : line * P.L + m;
+}
 
 __device__ __forceinline__ long long line_off(const LineMap &m, long long line)
 {
@@ -70,16 +107,46 @@ __device__ __forceinline__ double2 expipi(double a, double b)
 
 constexpr double kSqrt2 = 1.4142135623730951;
 
+// input point n as the half-length DCT sees it (DST sign / reversal, type-3 ortho)
+__device__ __forceinline__ double r2r_xp(const R2rParams &P, const double *x, long long n)
+{
+    const long long N = P.N;
+    const long long orig = (!P.cosine && P.type != 2) ? N - 1 - n : n;
+    double v = x[orig * P.sa_in];
+    if (P.type == 3 && P.ortho && orig == 0) v *= kSqrt2;
+    if (!P.cosine && P.type == 2 && (n & 1)) v = -v;
+    return v;
+}
+
 __global__ void k_r2r_pre(const R2rParams P)
 {
     const long long total = P.lines * P.L, N = P.N, L = P.L;
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (long long)gridDim.x * blockDim.x) {
-        const long long line = e / L, m = e - line * L;
+        long long line, m;
+        r2r_item(P, e, L, line, m);
         const double *x = P.src + line_off(P.in, line);
         auto X = [&](long long j) { return x[j * P.sa_in]; };
         double2 z = make_double2(0.0, 0.0);
-        if (P.cosine) {
+        if (P.half) {
+            const long long M = L;
+            if (P.type == 2) {
+                auto v = [&](long long q) { return q < M ? r2r_xp(P, x, 2 * q) : r2r_xp(P, x, 2 * N - 2 * q - 1); };
+                z = make_double2(v(2 * m), v(2 * m + 1));
+            } else if (P.type == 3) {
+                auto xe = [&](long long k) { return k < N ? r2r_xp(P, x, k) : 0.0; };
+                auto V = [&](long long k) {
+                    return cmul(expipi((double)k, 2.0 * N), make_double2(xe(k), -xe(N - k)));
+                };
+                const double2 vk = V(m), vm = cconj(V(M - m));
+                const double2 sum = cadd(vk, vm);
+                const double2 dif = cmul(expipi(2.0 * m, (double)N), csub(vk, vm));
+                z = make_double2(sum.x - dif.y, sum.y + dif.x);
+            } else {
+                z = cmul(make_double2(r2r_xp(P, x, 2 * m), r2r_xp(P, x, N - 1 - 2 * m)),
+                         cconj(expipi((double)m, (double)N)));
+            }
+        } else if (P.cosine) {
             switch (P.type) {
                 case 1: {
                     const long long j = m < N ? m : L - m;
@@ -120,51 +187,102 @@ __global__ void k_r2r_pre(const R2rParams P)
                     if (m < N) z = cscale(cconj(expipi((double)(2 * m + 1), 4.0 * N)), X(m));
             }
         }
-        P.work[e] = z;
+        P.work[r2r_widx(P, line, m)] = z;
     }
+}
+
+// output i (as the DCT forms it) of a half-length line: DST reversal / signs, scale
+__device__ __forceinline__ void r2r_put(const R2rParams &P, double *y, long long i, double v)
+{
+    const long long N = P.N;
+    long long o = i;
+    if (!P.cosine) {
+        if (P.type == 2) o = N - 1 - i;
+        else if (i & 1) v = -v;
+    }
+    y[o * P.sa_out] = v * P.fct;
+}
+
+// work items per line of the post-pass
+__host__ __device__ inline long long r2r_post_count(int half, int type, long long N)
+{
+    return half ? (type == 2 ? N / 2 + 1 : N / 2) : N;
 }
 
 __global__ void k_r2r_post(const R2rParams P)
 {
-    const long long total = P.lines * P.N, N = P.N, L = P.L;
+    const long long N = P.N, L = P.L;
+    const long long cnt = r2r_post_count(P.half, P.type, N), total = P.lines * cnt;
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (long long)gridDim.x * blockDim.x) {
-        const long long line = e / N, k = e - line * N;
-        const double2 *Z = P.work + line * L;
+        long long line, k;
+        r2r_item(P, e, cnt, line, k);
+        double *yl = P.dst + line_off(P.out, line);
+        auto Zat = [&](long long m) { return P.work[r2r_widx(P, line, m)]; };
+        if (P.half) {
+            const long long M = L;
+            if (P.type == 2) {
+                const double2 zk = Zat(k == M ? 0 : k), zm = cconj(Zat(k == 0 ? 0 : M - k));
+                const double2 ev = cscale(cadd(zk, zm), 0.5);
+                const double2 d = csub(zk, zm);
+                const double2 od = make_double2(0.5 * d.y, -0.5 * d.x);
+                const double2 V = cadd(ev, cmul(cconj(expipi(2.0 * k, (double)N)), od));
+                const double2 W = cmul(cconj(expipi((double)k, 2.0 * N)), V);
+                const long long oi = P.cosine ? 0 : N - 1;   // the ortho end point as the DCT indexes it
+                double lo = 2.0 * W.x;
+                if (P.ortho && k == oi) lo *= 0.5 * kSqrt2;
+                r2r_put(P, yl, k, lo);
+                if (k > 0 && k < M) {
+                    double hi = -2.0 * W.y;
+                    if (P.ortho && N - k == oi) hi *= 0.5 * kSqrt2;
+                    r2r_put(P, yl, N - k, hi);
+                }
+            } else if (P.type == 3) {
+                const double2 z = Zat(k);
+                auto sigma = [&](long long q) { return q < M ? 2 * q : 2 * N - 2 * q - 1; };
+                r2r_put(P, yl, sigma(2 * k), z.x);
+                r2r_put(P, yl, sigma(2 * k + 1), z.y);
+            } else {
+                const double2 S = cmul(Zat(k), cconj(expipi((double)(4 * k + 1), 4.0 * N)));
+                r2r_put(P, yl, 2 * k, 2.0 * S.x);
+                r2r_put(P, yl, N - 1 - 2 * k, -2.0 * S.y);
+            }
+            continue;
+        }
         double y;
         if (P.cosine) {
             switch (P.type) {
                 case 1:
-                    y = Z[k].x;
+                    y = Zat(k).x;
                     if (P.ortho && (k == 0 || k == N - 1)) y *= 0.5 * kSqrt2;
                     break;
                 case 2:
-                    y = 2.0 * cmul(cconj(expipi((double)k, 2.0 * N)), Z[k]).x;
+                    y = 2.0 * cmul(cconj(expipi((double)k, 2.0 * N)), Zat(k)).x;
                     if (P.ortho && k == 0) y *= 0.5 * kSqrt2;
                     break;
                 case 3:
-                    y = Z[k].x;
+                    y = Zat(k).x;
                     break;
                 default:
-                    y = 2.0 * cmul(cconj(expipi((double)k, 2.0 * N)), Z[k]).x;
+                    y = 2.0 * cmul(cconj(expipi((double)k, 2.0 * N)), Zat(k)).x;
             }
         } else {
             switch (P.type) {
                 case 1:
-                    y = -Z[k + 1].y;
+                    y = -Zat(k + 1).y;
                     break;
                 case 2:
-                    y = -2.0 * cmul(cconj(expipi((double)(k + 1), 2.0 * N)), Z[k + 1]).y;
+                    y = -2.0 * cmul(cconj(expipi((double)(k + 1), 2.0 * N)), Zat(k + 1)).y;
                     if (P.ortho && k == 0) y *= 0.5 * kSqrt2;
                     break;
                 case 3:
-                    y = Z[k].y;
+                    y = Zat(k).y;
                     break;
                 default:
-                    y = -2.0 * cmul(cconj(expipi((double)k, 2.0 * N)), Z[k]).y;
+                    y = -2.0 * cmul(cconj(expipi((double)k, 2.0 * N)), Zat(k)).y;
             }
         }
-        P.dst[line_off(P.out, line) + k * P.sa_out] = y * P.fct;
+        yl[k * P.sa_out] = y * P.fct;
     }
 }
 
@@ -173,7 +291,7 @@ struct R2r {
     struct Axis {
         int axis;
         long long N, L, lines;
-        int forward;
+        int forward, half, col;
         LineMap in_first, in_next, out;
         long long sa_in_first, sa_in_next, sa_out;
         std::unique_ptr<Plan> fft;
@@ -249,7 +367,9 @@ int r2r_plan_create(std::unique_ptr<Plan> &out, int kind, int type, int ortho, i
             err = {E_INVALID, "DCT-I needs at least 2 points along every axis"};
             return E_INVALID;
         }
-        A.L = (R->cosine && type == 1) ? 2 * (A.N - 1) : (!R->cosine && type == 1) ? 2 * (A.N + 1) : 2 * A.N;
+        A.half = type != 1 && A.N % 2 == 0;
+        A.L = A.half ? A.N / 2
+                     : (R->cosine && type == 1) ? 2 * (A.N - 1) : (!R->cosine && type == 1) ? 2 * (A.N + 1) : 2 * A.N;
         A.lines = 1;
         for (size_t d = 0; d < ndim; ++d)
             if ((int)d != A.axis) A.lines *= (long long)shape[d];
@@ -259,11 +379,14 @@ int r2r_plan_create(std::unique_ptr<Plan> &out, int kind, int type, int ortho, i
         A.out = A.in_next;
         A.sa_in_first = P.sin[A.axis] / 8;
         A.sa_in_next = A.sa_out = P.sout[A.axis] / 8;
-        // c2c over the workspace [lines, L], in place
-        const size_t fshape[2] = {(size_t)A.lines, (size_t)A.L};
-        const ptrdiff_t fstr[2] = {(ptrdiff_t)(A.L * 16), 16};
-        const size_t fax[1] = {1};
-        rc = plan_create(A.fft, 0, 0, 2, fshape, fstr, fstr, 1, fax, A.forward, err);
+        // threads walk along lines when the axis is the input's unit-stride dimension, else across lines
+        A.col = A.lines > 1 && std::llabs(a == 0 ? A.sa_in_first : A.sa_in_next) != 1;
+        // c2c over the workspace, in place: [lines, L] along axis 1, or [L, lines] along axis 0
+        const size_t fshape_r[2] = {(size_t)A.lines, (size_t)A.L}, fshape_c[2] = {(size_t)A.L, (size_t)A.lines};
+        const ptrdiff_t fstr_r[2] = {(ptrdiff_t)(A.L * 16), 16}, fstr_c[2] = {(ptrdiff_t)(A.lines * 16), 16};
+        const size_t fax_r[1] = {1}, fax_c[1] = {0};
+        rc = A.col ? plan_create(A.fft, 0, 0, 2, fshape_c, fstr_c, fstr_c, 1, fax_c, A.forward, err)
+                   : plan_create(A.fft, 0, 0, 2, fshape_r, fstr_r, fstr_r, 1, fax_r, A.forward, err);
         if (rc != OK) return rc;
         work = std::max(work, (size_t)(A.lines * A.L) * sizeof(double2));
         R->ax.push_back(std::move(A));
@@ -300,8 +423,10 @@ int r2r_execute(Plan &P, const void *in, void *out, double fct, hipStream_t s, E
         Q.cosine = R.cosine;
         Q.type = R.type;
         Q.ortho = R.ortho;
+        Q.half = A.half;
+        Q.col = A.col;
         Q.fct = a == 0 ? fct : 1.0;
-        const long long pre = A.lines * A.L, post = A.lines * A.N;
+        const long long pre = A.lines * A.L, post = A.lines * r2r_post_count(A.half, R.type, A.N);
         const unsigned gpre = (unsigned)std::min<long long>((pre + 255) / 256, 1 << 20);
         const unsigned gpost = (unsigned)std::min<long long>((post + 255) / 256, 1 << 20);
         hipLaunchKernelGGL(k_r2r_pre, dim3(gpre), dim3(256), 0, s, Q);

@@ -43,7 +43,9 @@ enum StoreMode : int {
     ST_R2C_PAIR = 3,  // Xa[k], Xb[k], k = 0..L/2, from Z[k], Z[L-k] (odd r2c pair)
     ST_C2R_HALF = 4,  // x[2 pos] = Re z, x[2 pos + 1] = Im z
     ST_C2R_PAIR = 5,  // xa[pos] = Re z, xb[pos] = Im z
-    ST_MUL_TABLE = 6  // y[pos] = z[n] tab[pos]                      (Bluestein spectrum product)
+    ST_MUL_TABLE = 6, // y[pos] = z[n] tab[pos]                      (Bluestein spectrum product)
+    ST_R2C_ROWPAIR = 7 // even r2c, last four-step pass: rows j and L2-j of one workgroup hold
+                       // Z[k] and Z[M-k], stored as the half spectrum X[k], X[M-k]
 };
 
 enum SideKind : int { SIDE_DIMS = 0, SIDE_DIMS_PAIR = 1, SIDE_LINEAR = 2 };

@@ -63,7 +63,56 @@ template <int N, int MAP> constexpr int pow2_ld() { return N + N / 16 + (MAP == 
 // W^m, W^2m, W^4m, W^8m come from the table, every other power is one to three
 // products of those, and each twiddle is used as soon as it is formed so that
 // at most five are live at a time.
-enum Pow2Io { IO_GENERIC = 0, IO_PLAIN = 1, IO_TWIDDLE = 2 };
+enum Pow2Io { IO_GENERIC = 0, IO_PLAIN = 1, IO_TWIDDLE = 2, IO_R2C_ROWPAIR = 3 };
+
+// Row assignment of the fused real post-process (ST_R2C_ROWPAIR, rows of L1 = N
+// points, L2 rows per signal, Z[j + L2 n] in row j).  Z[M - k] for k = j + L2 n
+// sits in row L2 - j at point L1 - 1 - n, so a workgroup of TB rows takes TB/2
+// pairs (j, L2 - j), j = 1 .. L2/2 - 1 (rows j in slots 0 .. TB/2-1, partners in
+// slots TB/2 ..); one more workgroup per signal takes the self-paired rows 0
+// and L2/2.  Returns the logical transform of slot t, or -1 for an idle slot.
+__device__ __forceinline__ long long rowpair_transform(long long blk, int t, int TB, long long L2)
+{
+    const int pp = TB / 2;
+    const long long npairs = L2 / 2 - 1;
+    const long long W = (npairs + pp - 1) / pp;   // pair workgroups per signal
+    const long long u = blk / (W + 1), w = blk - u * (W + 1);
+    long long row;
+    if (w < W) {
+        const long long j = w * pp + 1 + (t % pp);
+        if (j > npairs) return -1;
+        row = t < pp ? j : L2 - j;
+    } else {
+        if (t > 1) return -1;
+        row = t == 0 ? 0 : L2 / 2;
+    }
+    return u * L2 + row;
+}
+
+// X[k] and X[M-k] of an even real transform from Z[k], Z[M-k] (M complex
+// points, W = W_{2M}); adj_w doubles the interior bins, fwd_flag 0 conjugates
+__device__ __forceinline__ void real_split_store(const FftParams &P, const TfInfo &U, long long k, double2 zk,
+                                                 double2 zm)
+{
+    const long long M = P.M;
+    const double2 w = twolevel(P.rtw_lo, P.rtw_hi, P.rtw_logS, k);
+    const double2 e1 = cscale(cadd(zk, cconj(zm)), 0.5), d1 = csub(zk, cconj(zm));
+    const double2 o1 = make_double2(0.5 * d1.y, -0.5 * d1.x);
+    const double2 e2 = cscale(cadd(zm, cconj(zk)), 0.5), d2 = csub(zm, cconj(zk));
+    const double2 o2 = make_double2(0.5 * d2.y, -0.5 * d2.x);
+    const long long km = M - k;
+    const double fk = (P.adj_w && k != 0 && k != M) ? 2.0 * P.fct : P.fct;
+    const double fm = (P.adj_w && km != 0 && km != M) ? 2.0 * P.fct : P.fct;
+    double2 Xk = cscale(cadd(e1, cmul(o1, w)), fk);
+    double2 Xm = cscale(cadd(e2, cmul(o2, make_double2(-w.x, w.y))), fm);
+    if (!P.fwd_flag) {
+        Xk = cconj(Xk);
+        Xm = cconj(Xm);
+    }
+    double2 *y = (double2 *)P.dst;
+    y[U.oa + k * P.out.sax] = Xk;
+    if (km != k) y[U.oa + km * P.out.sax] = Xm;
+}
 
 template <int R>
 __device__ __forceinline__ void apply_twiddles(double2 *v, __amdgpu_buffer_rsrc_t tw, int m, double s)
@@ -133,7 +182,14 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
     const int t = COL ? tid % TB : tid / NT;
     const int i = COL ? tid / TB : tid % NT;
     const double s = (double)P.sign;
-    const TfInfo T = make_tfinfo(P, (long long)xcd_block(blockIdx.x, gridDim.x) * TB + t);
+    long long gidx;
+    if constexpr (IO == IO_R2C_ROWPAIR) {
+        const long long g = rowpair_transform(xcd_block(blockIdx.x, gridDim.x), t, TB, P.ni);
+        gidx = g < 0 ? P.ntf : g;
+    } else {
+        gidx = (long long)xcd_block(blockIdx.x, gridDim.x) * TB + t;
+    }
+    const TfInfo T = make_tfinfo(P, gidx);
     const bool valid = T.flags & 1;
     constexpr bool FAST = IO != IO_GENERIC;
     // twiddle table through a buffer resource: 32-bit offsets, scalar base
@@ -264,6 +320,74 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
                 for (int q = 0; q < RL; ++q) store_value(P, T, b + q * (N / RL), v[c * RL + q]);
             }
         }
+    } else if constexpr (IO == IO_R2C_ROWPAIR) {
+        // rows j (slots t < TB/2) and L2 - j (slots t + TB/2): point n of row j
+        // pairs with point L1 - 1 - n of row L2 - j.  Round r keeps half r of the
+        // j rows and half 1 - r of their partners in LDS, so every pair of a
+        // round is resident; the self-paired rows 0 and L2/2 keep both halves
+        // (slots t and t + 2).
+        if (i == 0) info[t] = T;
+        double2 *ldc = (double2 *)lds;
+        constexpr int H = N / 2;
+        constexpr int SH = LDP / 2;
+        constexpr int PP = TB / 2;
+        constexpr int nthr = TB * NT;
+        static_assert(TB >= 4 && TB % 2 == 0, "row pairs and the two self-paired rows need four slots");
+        const long long L2 = P.ni;
+        __syncthreads();
+        const bool selfwg = (info[0].flags & 1) && info[0].j == 0;
+        if (selfwg) {
+            if (t < 2 && valid) {
+#pragma unroll
+                for (int c = 0; c < NB; ++c) {
+                    const int b = i + c * NT;
+#pragma unroll
+                    for (int q = 0; q < RL; ++q) {
+                        const int n = b + q * (N / RL);
+                        const int slot = n < H ? t : t + 2;
+                        ldc[slot * SH + padx(n < H ? n : n - H)] = v[c * RL + q];
+                    }
+                }
+            }
+            __syncthreads();
+            auto zat = [&](int tt, int n) { return n < H ? ldc[tt * SH + padx(n)] : ldc[(tt + 2) * SH + padx(n - H)]; };
+            // row 0: pairs (n, L1 - n), n = 0 .. L1/2; row L2/2: pairs (n, L1 - 1 - n), n < L1/2
+            for (int e = tid; e < N + 1; e += nthr) {
+                if (e <= N / 2) {
+                    const int n = e;
+                    const TfInfo &U = info[0];
+                    real_split_store(P, U, L2 * (long long)n, zat(0, n), zat(0, (N - n) % N));
+                } else {
+                    const int n = e - (N / 2 + 1);
+                    const TfInfo &U = info[1];
+                    if (U.flags & 1)
+                        real_split_store(P, U, L2 / 2 + L2 * (long long)n, zat(1, n), zat(1, N - 1 - n));
+                }
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                __syncthreads();
+                const int hsel = t < PP ? r : 1 - r;
+#pragma unroll
+                for (int c = 0; c < NB; ++c) {
+                    const int b = i + c * NT;
+#pragma unroll
+                    for (int q = 0; q < RL; ++q) {
+                        const int n = b + q * (N / RL) - hsel * H;
+                        if (n >= 0 && n < H) ldc[t * SH + padx(n)] = v[c * RL + q];
+                    }
+                }
+                __syncthreads();
+                for (int e = tid; e < PP * H; e += nthr) {
+                    const int pp = e % PP, nl = e / PP;
+                    const TfInfo &U = info[pp];
+                    if (!(U.flags & 1)) continue;
+                    const long long k = U.j + L2 * (long long)(r * H + nl);
+                    real_split_store(P, U, k, ldc[pp * SH + padx(nl)], ldc[(pp + PP) * SH + padx(H - 1 - nl)]);
+                }
+            }
+        }
     } else {
         // transpose through LDS in two halves of the point range (one component
         // row holds half the points as complex values)
@@ -321,10 +445,31 @@ static hipError_t launch_io(const FftParams &P, int map, unsigned grid, unsigned
     return hipGetLastError();
 }
 
+// the fused real post-process runs on the row-load / column-store mapping
+// with two (N = 2048) or four (N = 1024) row pairs per workgroup
+template <int N>
+static hipError_t launch_rowpair(const FftParams &P, unsigned grid, unsigned block, size_t lds, hipStream_t s)
+{
+    if constexpr (N == 1024 || N == 2048) {
+        static bool attr_done = false;
+        if (!attr_done) {
+            if (hipFuncSetAttribute((const void *)k_fft_pow2<N, MAP_ROWCOL, IO_R2C_ROWPAIR>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 8192) != hipSuccess)
+                (void)hipGetLastError();
+            attr_done = true;
+        }
+        hipLaunchKernelGGL((k_fft_pow2<N, MAP_ROWCOL, IO_R2C_ROWPAIR>), dim3(grid), dim3(block), lds, s, P);
+        return hipGetLastError();
+    } else {
+        return hipErrorInvalidValue;
+    }
+}
+
 template <int N>
 static hipError_t launch_n(const FftParams &P, int map, int io, unsigned grid, unsigned block, size_t lds,
                            hipStream_t s)
 {
+    if (io == IO_R2C_ROWPAIR) return map == MAP_ROWCOL ? launch_rowpair<N>(P, grid, block, lds, s) : hipErrorInvalidValue;
     switch (io) {
         case IO_PLAIN: return launch_io<N, IO_PLAIN>(P, map, grid, block, lds, s);
         case IO_TWIDDLE: return launch_io<N, IO_TWIDDLE>(P, map, grid, block, lds, s);
@@ -350,9 +495,26 @@ bool pow2_fast(const FftParams &P)
            P.dst_span <= win;
 }
 
+// plain complex loads inside a 31-bit byte window (the fused row-pair store
+// takes only this load form)
+bool pow2_fast_load(const FftParams &P)
+{
+    const long long imax = (P.ni - 1) * P.in.ipos + (long long)(P.N - 1) * P.in.pst;
+    return P.load_mode == LD_C2C && !P.conj_in && !P.chirp_in && imax < P.ilim && P.src_span > 0 &&
+           P.src_span <= 0x7fffffffLL;
+}
+
 int pow2_io(const FftParams &P)
 {
+    if (P.store_mode == ST_R2C_ROWPAIR) return IO_R2C_ROWPAIR;
     return pow2_fast(P) ? (P.store_mode == ST_TWIDDLE ? IO_TWIDDLE : IO_PLAIN) : IO_GENERIC;
+}
+
+// workgroups of the fused row-pair pass: pair groups plus one self-paired group per signal
+long long pow2_rowpair_grid(long long signals, long long L2, int TB)
+{
+    const long long pp = TB / 2, npairs = L2 / 2 - 1;
+    return signals * ((npairs + pp - 1) / pp + 1);
 }
 
 hipError_t launch_fft_pow2(const FftParams &P, int map, unsigned grid, unsigned block, size_t lds, hipStream_t s)

@@ -699,6 +699,15 @@ struct Builder {
                 FftParams B = passB(P0);
                 B.sign = dir_sign;
                 if (J.kind == J_R2C) {
+                    // even lengths: the real post-process runs inside pass B on row pairs
+                    if (!pair && rowpair_ok(L1, L2, ws)) {
+                        B.store_mode = ST_R2C_ROWPAIR;
+                        side_dims(B.out, J, false, false);
+                        B.out.u0 = u0;
+                        B.M = (int)Ls;
+                        B.rtw_lo = rlo; B.rtw_hi = rhi; B.rtw_S = rS; B.rtw_logS = ilog2(rS);
+                        return add_rowpair(B, REF_S0, J.dst, J.fct, cn, Ls);
+                    }
                     side_lin(B.out, n2, 1, L2);
                     B.store_mode = ST_C2C;
                     if (!add_fft(B, REF_S0, REF_S1, false, false, true)) return false;
@@ -751,6 +760,50 @@ struct Builder {
             cn = std::min(chunk, nlog - u0);
             if (!emit(u0)) return false;
         }
+        return true;
+    }
+
+    // pass B with the fused real post-process: the power-of-two row kernel with
+    // two or four row pairs per workgroup, loads from a workspace addressable
+    // with 31-bit byte offsets
+    static bool rowpair_ok(long long L1, long long L2, size_t ws)
+    {
+        if (env_ll("IMPULSE_FFT_NO_ROWPAIR", 0)) return false;
+        const int tb = (L1 == 1024 || L1 == 2048) ? pow2_tb_host((int)L1, 2) : 0;
+        return (tb == 4 || tb == 8) && L2 >= 4 && L2 % 2 == 0 && ws <= (size_t)0x7fffffff;
+    }
+
+    bool add_rowpair(FftParams P, BufRef src, BufRef dst, bool uses_fct, long long signals, long long Ls)
+    {
+        const int map = 2;
+        const long long tb = pow2_tb_host(P.N, map);
+        const long long per = (long long)pow2_lds_bytes(P.N, map);
+        const long long cap = max_lds_bytes() - (long long)sizeof(long long) * 8 * kMaxTB - 1024;
+        if (tb * per > cap) {
+            err = {E_UNSUPPORTED, "transform does not fit in LDS"};
+            return false;
+        }
+        P.TB = (int)tb;
+        P.NT = P.N / 16;
+        Pass ps;
+        ps.kind = Pass::FFT_POW2;
+        ps.map = map;
+        ps.fp = P;
+        ps.block = (unsigned)(tb * P.NT);
+        const long long grid = pow2_rowpair_grid(signals, P.ni, (int)tb);
+        if (grid > INT_MAX) {
+            err = {E_UNSUPPORTED, "batch too large for one launch"};
+            return false;
+        }
+        ps.grid = (unsigned)grid;
+        ps.lds = (size_t)(per * tb);
+        ps.src = src;
+        ps.dst = dst;
+        ps.uses_fct = uses_fct;
+        ps.name = "k_fft_pow2";
+        ps.bytes_in = (size_t)(P.ntf * P.N * 16);
+        ps.bytes_out = (size_t)(signals * (Ls + 1) * 16);
+        plan.passes.push_back(ps);
         return true;
     }
 

@@ -285,6 +285,44 @@ def test_r2c_long(n, batch):
     assert rt <= 1e-10
 
 
+@pytest.mark.parametrize("n", [1 << 20, 1 << 21, 1572864, 1 << 23])
+def test_r2c_fused_real_post(n):
+    """Even r2c lengths whose last four-step pass forms the half spectrum from row pairs (j, L2-j)
+    (rows of 1024 or 2048 points; 1572864 = 2 * 1024 * 768 leaves a group with fewer pairs): both
+    signs and the adjoint weights against the oracle, and against the unfused three-pass plan."""
+    import os
+    torch = _torch()
+    print(f"case fused r2c n={n}", flush=True)
+    batch = 2
+    x = synth.uniform((batch, n), -1.0, 1.0, seed=n % 977)
+    xd = _dev(x)
+    st_in, st_out = [n * 8, 8], [(n // 2 + 1) * 16, 16]
+    sp = torch.cuda.current_stream().cuda_stream
+
+    def run(kind, fwd, fct=1.0):
+        y = torch.empty((batch, n // 2 + 1), dtype=torch.complex128, device="cuda")
+        F.Plan(kind, [batch, n], st_in, st_out, [1], fwd).execute(xd.data_ptr(), y.data_ptr(), fct, sp)
+        torch.cuda.synchronize()
+        return y.cpu().numpy()
+
+    for fwd in (True, False):
+        y = run(_lib.R2C, fwd, 0.5)
+        for row in range(batch):
+            assert rel_l2(y[row], oracle.r2c(x[row], [0], fwd, 0.5)) < TOL, (n, fwd, row)
+    yf = run(_lib.R2C, True)
+    # the adjoint of c2r runs as this r2c with interior bins doubled (bins 0 and n/2 kept)
+    ya = run(_lib.C2R_ADJOINT, False)
+    w = np.full(n // 2 + 1, 2.0)
+    w[0] = w[-1] = 1.0
+    assert rel_l2(ya, yf * w) < 1e-15
+    os.environ["IMPULSE_FFT_NO_ROWPAIR"] = "1"
+    try:
+        yu = run(_lib.R2C, True)
+    finally:
+        del os.environ["IMPULSE_FFT_NO_ROWPAIR"]
+    assert rel_l2(yf, yu) < 1e-15
+
+
 # ---------------------------------------------------------------------------
 # error behaviour (pocketfft_hdronly.h:446-477 cases, forced from the host)
 # ---------------------------------------------------------------------------

@@ -109,9 +109,10 @@ __device__ __forceinline__ void real_split_store(const FftParams &P, const TfInf
         Xk = cconj(Xk);
         Xm = cconj(Xm);
     }
-    double2 *y = (double2 *)P.dst;
-    y[U.oa + k * P.out.sax] = Xk;
-    if (km != k) y[U.oa + km * P.out.sax] = Xm;
+    double2 *y = (double2 *)P.dst + U.oa;
+    const long long sax = P.out.sax;
+    y[k * sax] = Xk;
+    if (km != k) y[km * sax] = Xm;
 }
 
 template <int R>
@@ -379,10 +380,15 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
                     }
                 }
                 __syncthreads();
-                for (int e = tid; e < PP * H; e += nthr) {
+                // a fixed number of items per thread, unrolled so that the twiddle and LDS
+                // loads of all items issue before the first product
+                constexpr int ITEMS = (PP * H + nthr - 1) / nthr;
+#pragma unroll
+                for (int it = 0; it < ITEMS; ++it) {
+                    const int e = tid + it * nthr;
                     const int pp = e % PP, nl = e / PP;
                     const TfInfo &U = info[pp];
-                    if (!(U.flags & 1)) continue;
+                    if (e >= PP * H || !(U.flags & 1)) continue;
                     const long long k = U.j + L2 * (long long)(r * H + nl);
                     real_split_store(P, U, k, ldc[pp * SH + padx(nl)], ldc[(pp + PP) * SH + padx(H - 1 - nl)]);
                 }

@@ -32,9 +32,11 @@ EXPORTS = (
     "impulse_fft_dct", "impulse_fft_dst", "impulse_fft_r2r_plan_create",
     "impulse_fft_upfirdn_create", "impulse_fft_upfirdn_info", "impulse_fft_upfirdn_execute",
     "impulse_fft_upfirdn_execute_host", "impulse_fft_upfirdn_destroy",
+    "impulse_fft_r2r_fftpack", "impulse_fft_r2r_separable_hartley", "impulse_fft_r2r_genuine_hartley",
+    "impulse_fft_fftpack_plan_create",
 )
 
-C2C, R2C, C2R, R2C_ADJOINT, C2R_ADJOINT, DCT, DST = 0, 1, 2, 3, 4, 5, 6
+C2C, R2C, C2R, R2C_ADJOINT, C2R_ADJOINT, DCT, DST, FFTPACK, HARTLEY, HARTLEY_GENUINE = range(10)
 F64, F32 = 0, 1
 MFMA_F32, MFMA_BF16X = 0, 1
 UPFIRDN_AUTO, UPFIRDN_DIRECT, UPFIRDN_FFT = 0, 1, 2
@@ -90,6 +92,10 @@ def _sig(L):
     for n in ("impulse_fft_dct", "impulse_fft_dst"):
         getattr(L, n).argtypes = [sz, _szp, _pdp, _pdp, sz, _szp, i, i, _vp, _vp, d, i, sz]
     L.impulse_fft_r2r_plan_create.argtypes = [ctypes.POINTER(_vp), i, i, i, i, sz, _szp, _pdp, _pdp, sz, _szp]
+    L.impulse_fft_r2r_fftpack.argtypes = [sz, _szp, _pdp, _pdp, sz, _szp, i, i, i, _vp, _vp, d, sz]
+    for n in ("impulse_fft_r2r_separable_hartley", "impulse_fft_r2r_genuine_hartley"):
+        getattr(L, n).argtypes = [sz, _szp, _pdp, _pdp, sz, _szp, i, _vp, _vp, d, sz]
+    L.impulse_fft_fftpack_plan_create.argtypes = [ctypes.POINTER(_vp), i, i, i, i, sz, _szp, _pdp, _pdp, sz, _szp]
     L.impulse_fft_upfirdn_create.argtypes = [ctypes.POINTER(_vp), sz, sz, _vp, sz, i, i, i]
     L.impulse_fft_upfirdn_info.argtypes = [_vp, _szp, ctypes.POINTER(i), _szp, _szp]
     L.impulse_fft_upfirdn_execute.argtypes = [_vp, _vp, _vp, _vp]

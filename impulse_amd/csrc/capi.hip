@@ -117,10 +117,13 @@ int cached_plan(std::shared_ptr<ifft::Plan> &out, int kind, int dtype, size_t nd
     }
     std::unique_ptr<ifft::Plan> p;
     ifft::Error err{0, ""};
-    // DCT / DST (kinds 5, 6) carry the type and ortho flag in `forward` (2 type + ortho)
-    int rc = kind >= 5 ? ifft::r2r_plan_create(p, kind, forward >> 1, forward & 1, dtype, ndim, shape, sin, sout,
-                                               naxes, axes, err)
-                       : ifft::plan_create(p, kind, dtype, ndim, shape, sin, sout, naxes, axes, forward, err);
+    // DCT / DST (kinds 5, 6) carry the type and ortho flag in `forward` (2 type + ortho), the
+    // FFTPACK / Hartley kinds (7-9) real2hermitian and the direction (2 real2hermitian + forward)
+    int rc = kind >= 7   ? ifft::rfp_plan_create(p, kind, forward >> 1, forward & 1, dtype, ndim, shape, sin, sout,
+                                                 naxes, axes, err)
+             : kind >= 5 ? ifft::r2r_plan_create(p, kind, forward >> 1, forward & 1, dtype, ndim, shape, sin, sout,
+                                                 naxes, axes, err)
+                         : ifft::plan_create(p, kind, dtype, ndim, shape, sin, sout, naxes, axes, forward, err);
     if (rc != ifft::OK) return fail(err);
     std::shared_ptr<ifft::Plan> sp(std::move(p));
     {
@@ -209,6 +212,7 @@ int impulse_fft_plan_num_launches(impulse_fft_plan plan)
 {
     if (!plan || !plan->plan) return -1;
     if (plan->plan->r2r) return ifft::r2r_num_launches(*plan->plan->r2r);
+    if (plan->plan->rfp) return ifft::rfp_num_launches(*plan->plan->rfp);
     return (int)plan->plan->passes.size();
 }
 
@@ -216,7 +220,8 @@ int impulse_fft_plan_pass_info(impulse_fft_plan plan, int pass, char *name, size
                                size_t *bytes_out, unsigned *grid, unsigned *block, size_t *lds_bytes)
 {
     if (!plan || !plan->plan) return fail(ifft::E_INVALID, "plan is NULL");
-    if (plan->plan->r2r) return fail(ifft::E_UNSUPPORTED, "per-launch information is not kept for DCT/DST plans");
+    if (plan->plan->r2r || plan->plan->rfp)
+        return fail(ifft::E_UNSUPPORTED, "per-launch information is not kept for real-to-real plans");
     const auto &P = *plan->plan;
     if (pass < 0 || pass >= (int)P.passes.size()) return fail(ifft::E_INVALID, "pass index out of range");
     const auto &ps = P.passes[pass];
@@ -235,7 +240,8 @@ int impulse_fft_plan_pass_info(impulse_fft_plan plan, int pass, char *name, size
 int impulse_fft_plan_profile(impulse_fft_plan plan, int max_executes)
 {
     if (!plan || !plan->plan) return fail(ifft::E_INVALID, "plan is NULL");
-    if (plan->plan->r2r) return fail(ifft::E_UNSUPPORTED, "per-launch timing is not kept for DCT/DST plans");
+    if (plan->plan->r2r || plan->plan->rfp)
+        return fail(ifft::E_UNSUPPORTED, "per-launch timing is not kept for real-to-real plans");
     ifft::Error err{0, ""};
     int rc = ifft::plan_profile(*plan->plan, max_executes, err);
     return rc == ifft::OK ? rc : fail(err);
@@ -244,7 +250,8 @@ int impulse_fft_plan_profile(impulse_fft_plan plan, int max_executes)
 int impulse_fft_plan_profile_read(impulse_fft_plan plan, double *pass_ms, int npass, int *executes)
 {
     if (!plan || !plan->plan || (!pass_ms && npass > 0)) return fail(ifft::E_INVALID, "invalid argument");
-    if (plan->plan->r2r) return fail(ifft::E_UNSUPPORTED, "per-launch timing is not kept for DCT/DST plans");
+    if (plan->plan->r2r || plan->plan->rfp)
+        return fail(ifft::E_UNSUPPORTED, "per-launch timing is not kept for real-to-real plans");
     ifft::Error err{0, ""};
     int rc = ifft::plan_profile_read(*plan->plan, pass_ms, npass, executes, err);
     return rc == ifft::OK ? rc : fail(err);
@@ -297,6 +304,51 @@ static int oneshot_r2r(int kind, size_t ndim, const size_t *shape, const ptrdiff
 {
     if (type < 1 || type > 4) return fail(ifft::E_INVALID, kind == 5 ? "invalid DCT type" : "invalid DST type");
     return oneshot(kind, ndim, shape, sin, sout, naxes, axes, 2 * type + (ortho ? 1 : 0), dtype, in, out, fct);
+}
+
+int impulse_fft_fftpack_plan_create(impulse_fft_plan *plan, int kind, int real2hermitian, int forward, int dtype,
+                                    size_t ndim, const size_t *shape, const ptrdiff_t *stride_in,
+                                    const ptrdiff_t *stride_out, size_t naxes, const size_t *axes)
+{
+    if (!plan) return fail(ifft::E_INVALID, "plan pointer is NULL");
+    *plan = nullptr;
+    std::unique_ptr<ifft::Plan> p;
+    ifft::Error err{0, ""};
+    int rc = ifft::rfp_plan_create(p, kind, real2hermitian ? 1 : 0, forward ? 1 : 0, dtype, ndim, shape, stride_in,
+                                   stride_out, naxes, axes, err);
+    if (rc != ifft::OK) return fail(err);
+    auto *h = new (std::nothrow) impulse_fft_plan_s;
+    if (!h) return fail(ifft::E_NOMEM, "out of host memory");
+    h->plan = std::move(p);
+    *plan = h;
+    return ifft::OK;
+}
+
+int impulse_fft_r2r_fftpack(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in, const ptrdiff_t *stride_out,
+                            size_t naxes, const size_t *axes, int real2hermitian, int forward, int dtype,
+                            const void *data_in, void *data_out, double fct, size_t nthreads)
+{
+    (void)nthreads;
+    return oneshot(IMPULSE_FFT_FFTPACK, ndim, shape, stride_in, stride_out, naxes, axes,
+                   2 * (real2hermitian ? 1 : 0) + (forward ? 1 : 0), dtype, data_in, data_out, fct);
+}
+
+int impulse_fft_r2r_separable_hartley(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in,
+                                      const ptrdiff_t *stride_out, size_t naxes, const size_t *axes, int dtype,
+                                      const void *data_in, void *data_out, double fct, size_t nthreads)
+{
+    (void)nthreads;
+    return oneshot(IMPULSE_FFT_HARTLEY, ndim, shape, stride_in, stride_out, naxes, axes, 0, dtype, data_in,
+                   data_out, fct);
+}
+
+int impulse_fft_r2r_genuine_hartley(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in,
+                                    const ptrdiff_t *stride_out, size_t naxes, const size_t *axes, int dtype,
+                                    const void *data_in, void *data_out, double fct, size_t nthreads)
+{
+    (void)nthreads;
+    return oneshot(IMPULSE_FFT_HARTLEY_GENUINE, ndim, shape, stride_in, stride_out, naxes, axes, 0, dtype, data_in,
+                   data_out, fct);
 }
 
 int impulse_fft_dct(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in, const ptrdiff_t *stride_out,

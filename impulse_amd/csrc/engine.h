@@ -68,6 +68,7 @@ struct Pass {
 };
 
 struct R2r;   // DCT / DST plan data (dct.hip)
+struct Rfp;   // r2r_fftpack / Hartley plan data (hartley.hip)
 
 struct Plan {
     int kind = 0, dtype = 0, forward = 1;
@@ -89,6 +90,7 @@ struct Plan {
     std::vector<hipEvent_t> prof_ev;    // 2 per launch per recorded execute
     int prof_max = 0, prof_count = 0;
     R2r *r2r = nullptr;                 // DCT / DST plans: per-axis pre/post passes around c2c sub-plans
+    Rfp *rfp = nullptr;                 // r2r_fftpack / Hartley plans: r2c / c2r sub-plans and a pack pass
     size_t device_bytes() const;
     ~Plan();
 };
@@ -109,6 +111,14 @@ int r2r_execute(Plan &P, const void *in, void *out, double fct, hipStream_t s, E
 int r2r_num_launches(const R2r &R);
 void r2r_destroy(R2r *r);
 int plan_profile_read(Plan &plan, double *pass_ms, int npass, int *executes, Error &err);
+
+// r2r_fftpack (kind 7) and the separable (8) / genuine (9) Hartley transforms (hartley.hip)
+int rfp_plan_create(std::unique_ptr<Plan> &out, int kind, int r2h, int forward, int dtype, size_t ndim,
+                    const size_t *shape, const ptrdiff_t *stride_in, const ptrdiff_t *stride_out, size_t naxes,
+                    const size_t *axes, Error &err);
+int rfp_execute(Plan &P, const void *in, void *out, double fct, hipStream_t s, Error &err);
+int rfp_num_launches(const Rfp &R);
+void rfp_destroy(Rfp *r);
 
 // FFT convolution (conv.hip)
 struct Conv2d;

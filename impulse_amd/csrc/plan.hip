@@ -50,6 +50,7 @@ Plan::~Plan()
 {
     for (auto e : prof_ev) (void)hipEventDestroy(e);
     r2r_destroy(r2r);
+    rfp_destroy(rfp);
 }
 
 size_t Plan::device_bytes() const
@@ -1054,6 +1055,14 @@ int plan_execute(Plan &P, const void *in, void *out, double fct, hipStream_t str
             return E_UNSUPPORTED;
         }
         return r2r_execute(P, in, out, fct, stream, err);
+    }
+    if (P.rfp) {
+        if (in == out && P.sin != P.sout) { err = {E_INVALID, "stride mismatch"}; return E_INVALID; }
+        if (((uintptr_t)in | (uintptr_t)out) % 8) {
+            err = {E_UNSUPPORTED, "data pointers must be aligned to the element size"};
+            return E_UNSUPPORTED;
+        }
+        return rfp_execute(P, in, out, fct, stream, err);
     }
     if (in == out && P.kind != J_C2C) {
         err = {E_INVALID, "in-place r2c/c2r is not supported"};

@@ -448,3 +448,62 @@ def _r2r(fn, x, axes, type, fct, ortho, out):
     check(fn(x.ndim, sizes(x.shape), strides(x.strides), strides(out.strides), len(axes), sizes(axes), int(type),
              _lib.F64, x.ctypes.data, out.ctypes.data, float(fct), 1 if ortho else 0, 1))
     return out
+
+
+# ---------------------------------------------------------------------------
+# r2r_fftpack and the Hartley transforms (pocketfft_hdronly.h:3392-3445)
+# ---------------------------------------------------------------------------
+
+def _rfp_host(fn, x, axes, extra, fct, out):
+    x = np.asarray(x)
+    if x.dtype != np.float64:
+        raise TypeError("real-to-real transforms need float64")
+    if out is None:
+        out = np.empty(x.shape, np.float64)
+    check(fn(x.ndim, sizes(x.shape), strides(x.strides), strides(out.strides), len(axes), sizes(axes), *extra,
+             _lib.F64, x.ctypes.data, out.ctypes.data, float(fct), 1))
+    return out
+
+
+def r2r_fftpack(x, axes, real2hermitian, forward, fct=1.0, out=None):
+    """pocketfft::r2r_fftpack on a float64 numpy array: along each axis, `forward` transforms real data to
+    FFTPACK halfcomplex order and backward transforms halfcomplex data back; `real2hermitian` adds the
+    reference's sign flips of the imaginary slots (input of a forward call when false, output of a backward
+    call when true)."""
+    return _rfp_host(lib().impulse_fft_r2r_fftpack, x, axes, [int(bool(real2hermitian)), int(bool(forward))], fct,
+                     out)
+
+
+def r2r_separable_hartley(x, axes, fct=1.0, out=None):
+    """pocketfft::r2r_separable_hartley: H = Re X + Im X along each axis in turn."""
+    return _rfp_host(lib().impulse_fft_r2r_separable_hartley, x, axes, [], fct, out)
+
+
+def r2r_genuine_hartley(x, axes, fct=1.0, out=None):
+    """pocketfft::r2r_genuine_hartley: H = Re X + Im X of the DFT over all axes at once."""
+    return _rfp_host(lib().impulse_fft_r2r_genuine_hartley, x, axes, [], fct, out)
+
+
+class FftpackPlan:
+    """A device plan for r2r_fftpack (kind FFTPACK) or the Hartley transforms (HARTLEY, HARTLEY_GENUINE)
+    over torch device tensors, with byte strides like Plan."""
+
+    def __init__(self, kind, shape, stride_in, stride_out, axes, real2hermitian=True, forward=True):
+        h = ctypes.c_void_p()
+        check(lib().impulse_fft_fftpack_plan_create(ctypes.byref(h), int(kind), int(bool(real2hermitian)),
+                                                    int(bool(forward)), _lib.F64, len(shape), sizes(shape),
+                                                    strides(stride_in), strides(stride_out), len(axes), sizes(axes)))
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            lib().impulse_fft_plan_destroy(h)
+            self.handle = None
+
+    def execute(self, in_ptr, out_ptr, fct=1.0, stream=0):
+        check(lib().impulse_fft_execute(self.handle, in_ptr, out_ptr, float(fct), stream or None))
+
+    @property
+    def num_launches(self):
+        return lib().impulse_fft_plan_num_launches(self.handle)

@@ -58,7 +58,12 @@ typedef enum impulse_fft_kind {
     IMPULSE_FFT_C2R_ADJOINT = 4,
     /* Real-to-real transforms (impulse_fft_r2r_plan_create, impulse_fft_dct/dst). */
     IMPULSE_FFT_DCT = 5,
-    IMPULSE_FFT_DST = 6
+    IMPULSE_FFT_DST = 6,
+    /* FFTPACK-order and Hartley real-to-real transforms (impulse_fft_fftpack_plan_create,
+     * impulse_fft_r2r_fftpack, impulse_fft_r2r_separable_hartley, impulse_fft_r2r_genuine_hartley). */
+    IMPULSE_FFT_FFTPACK = 7,
+    IMPULSE_FFT_HARTLEY = 8,
+    IMPULSE_FFT_HARTLEY_GENUINE = 9
 } impulse_fft_kind;
 
 typedef enum impulse_fft_dtype {
@@ -162,6 +167,41 @@ int impulse_fft_dst(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in
 int impulse_fft_r2r_plan_create(impulse_fft_plan *plan, int kind, int type, int ortho, int dtype,
                                 size_t ndim, const size_t *shape, const ptrdiff_t *stride_in,
                                 const ptrdiff_t *stride_out, size_t naxes, const size_t *axes);
+
+/* pocketfft::r2r_fftpack (pocketfft_hdronly.h:3392-3403), which impulse declares
+ * but does not export (cpp_pocketfft/pocketfft.nim:71-106).  Along each axis in
+ * turn, between real data and FFTPACK halfcomplex [r0, r1, i1, r2, i2, ...
+ * (, r_{n/2})].  As in the reference (ExecR2R, :3252-3270), `forward` picks
+ * the operation: forward != 0 writes the halfcomplex X[k] = sum_n x[n]
+ * e^{-2 pi i k n / n_axis}; forward == 0 reads halfcomplex data and writes the
+ * unnormalised inverse of its Hermitian extension.  `real2hermitian` only
+ * adds sign flips of positions 2, 4, ... (the imaginary slots): on the input
+ * of a forward call when it is 0, on the output of a backward call when it is
+ * 1.  Real doubles with byte strides on HOST memory, in == out allowed with
+ * equal strides; fct scales the first axis. */
+int impulse_fft_r2r_fftpack(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in,
+                            const ptrdiff_t *stride_out, size_t naxes, const size_t *axes,
+                            int real2hermitian, int forward, int dtype, const void *data_in,
+                            void *data_out, double fct, size_t nthreads);
+/* pocketfft::r2r_separable_hartley (pocketfft_hdronly.h:3405-3415): along each
+ * axis H[k] = Re X[k] + Im X[k] with X the forward DFT along that axis. */
+int impulse_fft_r2r_separable_hartley(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in,
+                                      const ptrdiff_t *stride_out, size_t naxes, const size_t *axes,
+                                      int dtype, const void *data_in, void *data_out, double fct,
+                                      size_t nthreads);
+/* pocketfft::r2r_genuine_hartley (pocketfft_hdronly.h:3417-3445): H = Re X + Im X
+ * with X the forward DFT over all `axes` at once (one axis: separable). */
+int impulse_fft_r2r_genuine_hartley(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in,
+                                    const ptrdiff_t *stride_out, size_t naxes, const size_t *axes,
+                                    int dtype, const void *data_in, void *data_out, double fct,
+                                    size_t nthreads);
+/* The same three as plans: kind IMPULSE_FFT_FFTPACK (with real2hermitian and
+ * forward), IMPULSE_FFT_HARTLEY or IMPULSE_FFT_HARTLEY_GENUINE (flags ignored),
+ * run by impulse_fft_execute (device pointers) or impulse_fft_execute_host. */
+int impulse_fft_fftpack_plan_create(impulse_fft_plan *plan, int kind, int real2hermitian, int forward,
+                                    int dtype, size_t ndim, const size_t *shape,
+                                    const ptrdiff_t *stride_in, const ptrdiff_t *stride_out,
+                                    size_t naxes, const size_t *axes);
 
 /* Plan cache control (PocketFFT's is a fixed 16-entry LRU, pocketfft_hdronly.h:50-52). */
 int impulse_fft_cache_set_capacity(size_t entries);

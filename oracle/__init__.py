@@ -179,3 +179,34 @@ def upfirdn_length(nt, nh, up=1, down=1):
     L.oracle_upfirdn_length.argtypes = [ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]
     L.oracle_upfirdn_length.restype = ctypes.c_size_t
     return int(L.oracle_upfirdn_length(int(nt), int(nh), int(up), int(down)))
+
+
+def _rfp(fname, x, axes, extra, fct, out):
+    x = np.asarray(x)
+    assert x.dtype == np.float64
+    if out is None:
+        out = np.empty(x.shape, np.float64)
+    f = getattr(lib(), fname)
+    f.argtypes = [ctypes.c_size_t, _szp, _pdp, _pdp, ctypes.c_size_t, _szp] + [ctypes.c_int] * len(extra) + \
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double]
+    f.restype = ctypes.c_int
+    rc = f(x.ndim, _arr(list(x.shape), ctypes.c_size_t), _arr(list(x.strides), ctypes.c_ssize_t),
+           _arr(list(out.strides), ctypes.c_ssize_t), len(axes), _arr(list(axes), ctypes.c_size_t),
+           *[int(v) for v in extra], x.ctypes.data, out.ctypes.data, float(fct))
+    _check(rc, fname)
+    return out
+
+
+def r2r_fftpack(x, axes, real2hermitian, forward, fct=1.0, out=None):
+    """pocketfft::r2r_fftpack semantics (naive long-double sums)."""
+    return _rfp("oracle_r2r_fftpack", x, axes, [bool(real2hermitian), bool(forward)], fct, out)
+
+
+def r2r_separable_hartley(x, axes, fct=1.0, out=None):
+    """pocketfft::r2r_separable_hartley semantics (naive long-double sums)."""
+    return _rfp("oracle_r2r_separable_hartley", x, axes, [], fct, out)
+
+
+def r2r_genuine_hartley(x, axes, fct=1.0, out=None):
+    """pocketfft::r2r_genuine_hartley semantics (naive long-double sums)."""
+    return _rfp("oracle_r2r_genuine_hartley", x, axes, [], fct, out)

@@ -645,3 +645,156 @@ int oracle_upfirdn(size_t nt, const double *t, size_t nh, const double *h, int u
     }
     return 0;
 }
+
+/* ---- r2r_fftpack and the Hartley transforms (pocketfft_hdronly.h:3252-3270,
+ * 3066-3110, 3392-3445), naive long-double sums per line ---- */
+typedef struct {
+    const char *in; char *out; ptrdiff_t is, os; size_t n; int hartley, r2h, forward; long double fct;
+    long double *x, *y;
+} rfp_ctx;
+
+static int rfp_line(void *vctx, ptrdiff_t io, ptrdiff_t oo)
+{
+    rfp_ctx *c = (rfp_ctx *)vctx;
+    const size_t N = c->n;
+    long double *x = c->x, *y = c->y;
+    for (size_t j = 0; j < N; ++j) x[j] = *(const double *)(c->in + io + (ptrdiff_t)j * c->is);
+    if (c->hartley) {
+        /* H[k] = Re X[k] + Im X[k], X[k] = sum x[n] e^{-2 pi i k n / N} */
+        for (size_t k = 0; k < N; ++k) {
+            long double acc = 0.0L;
+            for (size_t n = 0; n < N; ++n)
+                acc += x[n] * (cospi_q(2 * (uint64_t)((k * n) % N), N) - sinpi_q(2 * (uint64_t)((k * n) % N), N));
+            y[k] = acc;
+        }
+    } else {
+        /* ExecR2R: the direction picks the operation (forward: real -> halfcomplex, else
+         * halfcomplex -> real); real2hermitian only adds the sign flips of the positions
+         * 2, 4, ... (the imaginary slots): before a forward call without it, after a
+         * backward call with it */
+        if (!c->r2h && c->forward)
+            for (size_t i = 2; i < N; i += 2) x[i] = -x[i];
+        if (c->forward) {
+            for (size_t k = 0; 2 * k <= N; ++k) {
+                long double re = 0.0L, im = 0.0L;
+                for (size_t n = 0; n < N; ++n) {
+                    const uint64_t a = 2 * (uint64_t)((k * n) % N);
+                    re += x[n] * cospi_q(a, N);
+                    im -= x[n] * sinpi_q(a, N);
+                }
+                if (k == 0) y[0] = re;
+                else if (2 * k == N) y[N - 1] = re;
+                else { y[2 * k - 1] = re; y[2 * k] = im; }
+            }
+        } else {
+            for (size_t n = 0; n < N; ++n) {
+                long double acc = x[0];
+                for (size_t k = 1; 2 * k < N; ++k) {
+                    const uint64_t a = 2 * (uint64_t)((k * n) % N);
+                    acc += 2.0L * (x[2 * k - 1] * cospi_q(a, N) - x[2 * k] * sinpi_q(a, N));
+                }
+                if (N % 2 == 0 && N > 1) acc += x[N - 1] * ((n & 1) ? -1.0L : 1.0L);
+                y[n] = acc;
+            }
+        }
+        if (c->r2h && !c->forward)
+            for (size_t i = 2; i < N; i += 2) y[i] = -y[i];
+    }
+    for (size_t k = 0; k < N; ++k) *(double *)(c->out + oo + (ptrdiff_t)k * c->os) = (double)(y[k] * c->fct);
+    return 0;
+}
+
+static int rfp_nd(size_t ndim, const size_t *shape, const ptrdiff_t *sin, const ptrdiff_t *sout, size_t naxes,
+                  const size_t *axes, int hartley, int r2h, int forward, const double *in, double *out, double fct)
+{
+    if (check_args(ndim, shape, naxes, axes)) return 1;
+    if (prod(ndim, shape) == 0) return 0;
+    if (in == out)
+        for (size_t d = 0; d < ndim; ++d) if (sin[d] != sout[d]) return 1;
+    int rc = 0;
+    for (size_t a = 0; a < naxes && !rc; ++a) {
+        rfp_ctx c;
+        const size_t ax = axes[a];
+        c.in = a ? (const char *)out : (const char *)in;
+        c.out = (char *)out;
+        c.is = a ? sout[ax] : sin[ax];
+        c.os = sout[ax];
+        c.n = shape[ax];
+        c.hartley = hartley; c.r2h = r2h; c.forward = forward;
+        c.fct = a ? 1.0L : (long double)fct;
+        c.x = (long double *)malloc(c.n * sizeof(long double));
+        c.y = (long double *)malloc(c.n * sizeof(long double));
+        if (!c.x || !c.y) { free(c.x); free(c.y); return 2; }
+        rc = for_each_line(ndim, shape, a ? sout : sin, sout, ax, rfp_line, &c);
+        free(c.x); free(c.y);
+    }
+    return rc;
+}
+
+int oracle_r2r_fftpack(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in, const ptrdiff_t *stride_out,
+                       size_t naxes, const size_t *axes, int real2hermitian, int forward, const double *data_in,
+                       double *data_out, double fct)
+{
+    return rfp_nd(ndim, shape, stride_in, stride_out, naxes, axes, 0, real2hermitian != 0, forward != 0, data_in,
+                  data_out, fct);
+}
+
+int oracle_r2r_separable_hartley(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in,
+                                 const ptrdiff_t *stride_out, size_t naxes, const size_t *axes,
+                                 const double *data_in, double *data_out, double fct)
+{
+    return rfp_nd(ndim, shape, stride_in, stride_out, naxes, axes, 1, 1, 1, data_in, data_out, fct);
+}
+
+int oracle_r2r_genuine_hartley(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in,
+                               const ptrdiff_t *stride_out, size_t naxes, const size_t *axes,
+                               const double *data_in, double *data_out, double fct)
+{
+    if (naxes == 1)
+        return oracle_r2r_separable_hartley(ndim, shape, stride_in, stride_out, naxes, axes, data_in, data_out, fct);
+    if (check_args(ndim, shape, naxes, axes)) return 1;
+    if (prod(ndim, shape) == 0) return 0;
+    if (ndim > 64) return 1;
+    /* H[k] = sum_n x[n] (cos - sin)(2 pi sum_d k_d n_d / N_d) over the transformed dims, the
+     * other indices shared between input and output; computed into a buffer first (in == out) */
+    size_t total = prod(ndim, shape);
+    long double *res = (long double *)malloc(total * sizeof(long double));
+    if (!res) return 2;
+    int isax[64] = {0};
+    for (size_t a = 0; a < naxes; ++a) isax[axes[a]] = 1;
+    size_t ko[64], ni[64];
+    for (size_t e = 0; e < total; ++e) {
+        size_t rem = e;
+        for (size_t d = ndim; d-- > 0;) { ko[d] = rem % shape[d]; rem /= shape[d]; }
+        /* sum over the input points sharing the non-transformed indices */
+        size_t span = 1;
+        for (size_t d = 0; d < ndim; ++d) if (isax[d]) span *= shape[d];
+        long double acc = 0.0L;
+        for (size_t f = 0; f < span; ++f) {
+            size_t r2 = f;
+            for (size_t d = ndim; d-- > 0;) {
+                if (isax[d]) { ni[d] = r2 % shape[d]; r2 /= shape[d]; }
+                else ni[d] = ko[d];
+            }
+            /* phase as a fraction of a turn, exact per axis */
+            long double turn = 0.0L;
+            ptrdiff_t off = 0;
+            for (size_t d = 0; d < ndim; ++d) {
+                off += (ptrdiff_t)ni[d] * stride_in[d];
+                if (isax[d]) turn += (long double)((ko[d] * ni[d]) % shape[d]) / (long double)shape[d];
+            }
+            turn -= floorl(turn);
+            const long double ang = 6.283185307179586476925286766559005768L * turn;
+            acc += (long double)*(const double *)((const char *)data_in + off) * (cosl(ang) - sinl(ang));
+        }
+        res[e] = acc * (long double)fct;
+    }
+    for (size_t e = 0; e < total; ++e) {
+        size_t rem = e;
+        ptrdiff_t off = 0;
+        for (size_t d = ndim; d-- > 0;) { off += (ptrdiff_t)(rem % shape[d]) * stride_out[d]; rem /= shape[d]; }
+        *(double *)((char *)data_out + off) = (double)res[e];
+    }
+    free(res);
+    return 0;
+}

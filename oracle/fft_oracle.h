@@ -89,6 +89,28 @@ int oracle_dst(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in, con
  * at tests/test_signal.nim:67-85 fix), convolved in full with the nh taps h,
  * every down-th sample from the first kept; y holds
  * ceil(((nt-1) up + nh) / down) values, each a long-double direct sum. */
+/* pocketfft::r2r_fftpack (pocketfft_hdronly.h:3392-3403 with ExecR2R, :3252-3270):
+ * along each axis the direction picks the operation -- forward: the FFTPACK
+ * halfcomplex [r0, r1, i1, r2, i2, ... (, r_{n/2})] of X[k] = sum x[n]
+ * e^{-2 pi i k n/N}; backward: halfcomplex input summed back as x[n] = r0 +
+ * 2 sum_k (r_k cos - i_k sin)(2 pi k n/N) (+ r_{N/2} (-1)^n) -- and
+ * real2hermitian only decides the sign flips of positions 2, 4, ... (the
+ * imaginary slots): of the input before a forward call when it is 0, of the
+ * output after a backward call when it is 1 (the reference passes `forward`,
+ * not `real2hermitian`, as the plan's direction, :3264).
+ * Separable Hartley (:3405-3415, copy_hartley :3066-3095): H[k] = Re X[k] +
+ * Im X[k] along each axis.  Genuine Hartley (:3417-3445): the same with X the
+ * DFT over all axes at once.  fct scales once.  Naive long-double sums. */
+int oracle_r2r_fftpack(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in, const ptrdiff_t *stride_out,
+                       size_t naxes, const size_t *axes, int real2hermitian, int forward, const double *data_in,
+                       double *data_out, double fct);
+int oracle_r2r_separable_hartley(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in,
+                                 const ptrdiff_t *stride_out, size_t naxes, const size_t *axes,
+                                 const double *data_in, double *data_out, double fct);
+int oracle_r2r_genuine_hartley(size_t ndim, const size_t *shape, const ptrdiff_t *stride_in,
+                               const ptrdiff_t *stride_out, size_t naxes, const size_t *axes,
+                               const double *data_in, double *data_out, double fct);
+
 size_t oracle_upfirdn_length(size_t nt, size_t nh, int up, int down);
 int oracle_upfirdn(size_t nt, const double *t, size_t nh, const double *h, int up, int down, double *y);
 

@@ -152,3 +152,38 @@ def dct(x, axes, type=2, fct=1.0, ortho=False, nthreads=1, out=None):
 def dst(x, axes, type=2, fct=1.0, ortho=False, nthreads=1, out=None):
     """pocketfft::dst<double> (pocketfft_hdronly.h:3302-3318)."""
     return _r2r("ref_dst_f64", x, axes, type, fct, ortho, nthreads, out)
+
+
+def _rfp(name, x, axes, extra, fct, nthreads, out):
+    x = np.asarray(x)
+    assert x.dtype == np.float64
+    if out is None:
+        out = np.empty(x.shape, np.float64)
+    L = lib()
+    f = getattr(L, name)
+    szp, pdp = ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_ssize_t)
+    head = [ctypes.c_size_t, szp, pdp, pdp, ctypes.c_size_t, szp]
+    f.argtypes = head + [ctypes.c_int] * len(extra) + [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                                       ctypes.c_size_t]
+    f.restype = ctypes.c_int
+    rc = f(x.ndim, _a(x.shape, ctypes.c_size_t), _a(x.strides, ctypes.c_ssize_t), _a(out.strides, ctypes.c_ssize_t),
+           len(axes), _a(list(axes), ctypes.c_size_t), *[int(v) for v in extra], x.ctypes.data, out.ctypes.data,
+           float(fct), nthreads)
+    if rc != 0:
+        raise ValueError(f"reference pocketfft raised (code {rc})")
+    return out
+
+
+def r2r_fftpack(x, axes, real2hermitian, forward, fct=1.0, nthreads=1, out=None):
+    """pocketfft::r2r_fftpack<double> (pocketfft_hdronly.h:3392-3403)."""
+    return _rfp("ref_r2r_fftpack_f64", x, axes, [bool(real2hermitian), bool(forward)], fct, nthreads, out)
+
+
+def r2r_separable_hartley(x, axes, fct=1.0, nthreads=1, out=None):
+    """pocketfft::r2r_separable_hartley<double> (pocketfft_hdronly.h:3405-3415)."""
+    return _rfp("ref_r2r_separable_hartley_f64", x, axes, [], fct, nthreads, out)
+
+
+def r2r_genuine_hartley(x, axes, fct=1.0, nthreads=1, out=None):
+    """pocketfft::r2r_genuine_hartley<double> (pocketfft_hdronly.h:3417-3445)."""
+    return _rfp("ref_r2r_genuine_hartley_f64", x, axes, [], fct, nthreads, out)

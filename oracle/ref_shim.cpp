@@ -101,4 +101,37 @@ int ref_dst_f64(size_t ndim, const size_t *shape, const ptrdiff_t *sin, const pt
     return 0;
 }
 
+int ref_r2r_fftpack_f64(size_t ndim, const size_t *shape, const ptrdiff_t *sin, const ptrdiff_t *sout,
+                        size_t naxes, const size_t *axes, int r2h, int forward, const double *in, double *out,
+                        double fct, size_t nthreads)
+{
+    try {
+        pocketfft::r2r_fftpack<double>(mk_shape(ndim, shape), mk_stride(ndim, sin), mk_stride(ndim, sout),
+                                       mk_shape(naxes, axes), r2h != 0, forward != 0, in, out, fct, nthreads);
+    } catch (...) { return 1; }
+    return 0;
+}
+
+int ref_r2r_separable_hartley_f64(size_t ndim, const size_t *shape, const ptrdiff_t *sin, const ptrdiff_t *sout,
+                                  size_t naxes, const size_t *axes, const double *in, double *out, double fct,
+                                  size_t nthreads)
+{
+    try {
+        pocketfft::r2r_separable_hartley<double>(mk_shape(ndim, shape), mk_stride(ndim, sin), mk_stride(ndim, sout),
+                                                 mk_shape(naxes, axes), in, out, fct, nthreads);
+    } catch (...) { return 1; }
+    return 0;
+}
+
+int ref_r2r_genuine_hartley_f64(size_t ndim, const size_t *shape, const ptrdiff_t *sin, const ptrdiff_t *sout,
+                                size_t naxes, const size_t *axes, const double *in, double *out, double fct,
+                                size_t nthreads)
+{
+    try {
+        pocketfft::r2r_genuine_hartley<double>(mk_shape(ndim, shape), mk_stride(ndim, sin), mk_stride(ndim, sout),
+                                               mk_shape(naxes, axes), in, out, fct, nthreads);
+    } catch (...) { return 1; }
+    return 0;
+}
+
 }  // extern "C"

@@ -75,10 +75,12 @@ def main():
 
 R2R_LENGTHS = [1, 2, 3, 4, 5, 6, 7, 8, 13, 16, 17, 31, 32, 33, 64, 97, 100, 128, 255, 256, 1009, 2048]
 R2R_ORTHO = [2, 3, 5, 8, 17, 100]
+FP_LENGTHS = [1, 2, 3, 4, 5, 6, 7, 8, 13, 16, 17, 31, 64, 97, 100, 255, 256, 1000]
 
 
 def r2r():
-    """pocketfft::dct / dst of every type on 1-D inputs (plain and ortho) and n-D views."""
+    """pocketfft::dct / dst of every type on 1-D inputs (plain and ortho) and n-D views; r2r_fftpack (all
+    flag pairs) and the separable / genuine Hartley transforms."""
     g = {}
     for n in R2R_LENGTHS:
         x = synth.uniform((n,), -1.0, 1.0, seed=4000 + n)
@@ -90,8 +92,22 @@ def r2r():
                 g[f"{name}{t}_{n}"] = fn(x, [0], t)
                 if n in R2R_ORTHO:
                     g[f"{name}{t}_{n}_ortho"] = fn(x, [0], t, 0.5, True)
+    # r2r_fftpack (all four flag pairs) and the separable Hartley transform, 1-D
+    for n in FP_LENGTHS:
+        x = synth.uniform((n,), -1.0, 1.0, seed=5000 + n)
+        g[f"fp_{n}_in"] = x
+        for r2h in (0, 1):
+            for fw in (0, 1):
+                g[f"fp{r2h}{fw}_{n}"] = ref.r2r_fftpack(x, [0], bool(r2h), bool(fw), 1.0 if fw else 0.5)
+        g[f"hartley_{n}"] = ref.r2r_separable_hartley(x, [0])
     v = synth.uniform((6, 9, 10), -1.0, 1.0, seed=11)
     g["nd_r2r_in"] = v
+    g["nd_fp11_20"] = ref.r2r_fftpack(v[:, ::2, :], [2, 0], True, True, 0.25)
+    g["nd_fp00_1"] = ref.r2r_fftpack(v[::-1], [1], False, False)
+    g["nd_fp01_02"] = ref.r2r_fftpack(v, [0, 2], False, True)
+    g["nd_hartley_012"] = ref.r2r_separable_hartley(v, [0, 1, 2], 0.5)
+    g["nd_genuine_12"] = ref.r2r_genuine_hartley(v, [1, 2])
+    g["nd_genuine_201"] = ref.r2r_genuine_hartley(v[:, ::-1, :], [2, 0, 1], 2.0)
     for t in (1, 2, 3, 4):
         g[f"nd_dct{t}_20"] = ref.dct(v[:, ::2, :], [2, 0], t, 0.25, t == 2)
         g[f"nd_dst{t}_012"] = ref.dst(v[::-1], [0, 1, 2], t, 1.0, t == 3)

@@ -260,3 +260,78 @@ def test_r2r_against_reference_library():
                 if not (t == 1 and n == 1):
                     assert rel_l2(oracle.dct(x, [0], t, 0.7, ortho), ref.dct(x, [0], t, 0.7, ortho)) < TOL, (n, t)
                 assert rel_l2(oracle.dst(x, [0], t, 0.7, ortho), ref.dst(x, [0], t, 0.7, ortho)) < TOL, (n, t)
+
+
+# ---------------------------------------------------------------------------
+# r2r_fftpack and the Hartley transforms (pocketfft_hdronly.h:3392-3445)
+# ---------------------------------------------------------------------------
+
+def test_fftpack_hartley_golden(golden_r2r):
+    g = golden_r2r
+    n_checked = 0
+    for key in g.files:
+        if key.startswith("fp") and key[2:4].isdigit() and not key.startswith("fp_"):
+            r2h, fw, n = int(key[2]), int(key[3]), int(key.split("_")[1])
+            y = oracle.r2r_fftpack(g[f"fp_{n}_in"], [0], r2h, fw, 1.0 if fw else 0.5)
+            assert rel_l2(y, g[key]) < TOL, key
+            n_checked += 1
+        elif key.startswith("hartley_"):
+            n = int(key.split("_")[1])
+            assert rel_l2(oracle.r2r_separable_hartley(g[f"fp_{n}_in"], [0]), g[key]) < TOL, key
+            n_checked += 1
+    assert n_checked == 5 * 18
+    v = g["nd_r2r_in"]
+    assert rel_l2(oracle.r2r_fftpack(v[:, ::2, :], [2, 0], True, True, 0.25), g["nd_fp11_20"]) < TOL
+    assert rel_l2(oracle.r2r_fftpack(v[::-1], [1], False, False), g["nd_fp00_1"]) < TOL
+    assert rel_l2(oracle.r2r_fftpack(v, [0, 2], False, True), g["nd_fp01_02"]) < TOL
+    assert rel_l2(oracle.r2r_separable_hartley(v, [0, 1, 2], 0.5), g["nd_hartley_012"]) < TOL
+    assert rel_l2(oracle.r2r_genuine_hartley(v, [1, 2]), g["nd_genuine_12"]) < TOL
+    assert rel_l2(oracle.r2r_genuine_hartley(v[:, ::-1, :], [2, 0, 1], 2.0), g["nd_genuine_201"]) < TOL
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 8, 9, 30, 31])
+def test_fftpack_hartley_closed_forms(n):
+    """Independent statements: the forward halfcomplex equals the C path's rfft_packed; backward after
+    forward returns N x; the separable Hartley equals Re + Im of the oracle's r2c spectrum extended by
+    Hermitian symmetry and is its own inverse up to N."""
+    x = np.random.default_rng(n).uniform(-1, 1, n)
+    hc = oracle.r2r_fftpack(x, [0], True, True)
+    assert rel_l2(hc, oracle.rfft_packed(x, True)) < TOL
+    assert rel_l2(oracle.r2r_fftpack(hc, [0], False, False, 1.0 / n), x) < TOL
+    X = oracle.c2c(x.astype(complex), [0])
+    assert rel_l2(oracle.r2r_separable_hartley(x, [0]), X.real + X.imag) < TOL
+    assert rel_l2(oracle.r2r_separable_hartley(oracle.r2r_separable_hartley(x, [0]), [0], 1.0 / n), x) < TOL
+    # the mixed flag pairs: sign flips of the positions 2, 4, ... around the plain operations
+    flip = np.ones(n)
+    flip[2::2] = -1.0
+    assert rel_l2(oracle.r2r_fftpack(x, [0], False, True), oracle.r2r_fftpack(x * flip, [0], True, True)) < TOL
+    assert rel_l2(oracle.r2r_fftpack(x, [0], True, False), oracle.r2r_fftpack(x, [0], False, False) * flip) < TOL
+
+
+def test_genuine_hartley_nd_closed_forms():
+    v = np.random.default_rng(13).uniform(-1, 1, (4, 6, 5))
+    X = oracle.c2c(v.astype(complex), [0, 2])
+    assert rel_l2(oracle.r2r_genuine_hartley(v, [0, 2]), X.real + X.imag) < TOL
+    # the genuine transform is an involution up to the number of points
+    h = oracle.r2r_genuine_hartley(v, [0, 1, 2])
+    assert rel_l2(oracle.r2r_genuine_hartley(h, [0, 1, 2], 1.0 / v.size), v) < TOL
+    # one axis: the separable transform
+    assert rel_l2(oracle.r2r_genuine_hartley(v, [1]), oracle.r2r_separable_hartley(v, [1])) < TOL
+
+
+@pytest.mark.skipif(not ref.available(), reason="reference PocketFFT not built here")
+def test_fftpack_hartley_against_reference_library():
+    rng = np.random.default_rng(14)
+    for n in list(range(1, 50)) + [64, 97, 128, 243, 1000]:
+        x = rng.uniform(-1, 1, n)
+        for r2h in (True, False):
+            for fw in (True, False):
+                assert rel_l2(oracle.r2r_fftpack(x, [0], r2h, fw, 0.7), ref.r2r_fftpack(x, [0], r2h, fw, 0.7)) < TOL
+        assert rel_l2(oracle.r2r_separable_hartley(x, [0], 1.3), ref.r2r_separable_hartley(x, [0], 1.3)) < TOL
+    v = rng.uniform(-1, 1, (5, 6, 7))
+    for axes in ([0, 1], [2, 0], [0, 1, 2]):
+        assert rel_l2(oracle.r2r_genuine_hartley(v, axes, 0.5), ref.r2r_genuine_hartley(v, axes, 0.5)) < TOL
+        for r2h in (True, False):
+            for fw in (True, False):
+                assert rel_l2(oracle.r2r_fftpack(v[::-1, :, ::2], axes, r2h, fw),
+                              ref.r2r_fftpack(v[::-1, :, ::2], axes, r2h, fw)) < TOL

@@ -40,9 +40,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-forms", action="store_true",
-                    help="skip the forms and widened paths (ragged, MFMA, convolution, DCT, upfirdn)")
+                    help="skip the forms and widened paths (ragged, MFMA, convolution, DCT, upfirdn, fftpack)")
     ap.add_argument("--only-forms", default="",
-                    help="comma list of forms to run alone (ragged,mfma,conv,dct,upfirdn), no drop-in step")
+                    help="comma list of forms to run alone (ragged,mfma,conv,dct,upfirdn,fftpack), no drop-in step")
     ap.add_argument("--form-reps", type=int, default=10)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (the reference "
@@ -348,6 +348,31 @@ def dct_form(torch, reps):
     return out
 
 
+def fftpack_form(torch, reps):
+    """r2r_fftpack (forward, halfcomplex out) over the rows of a float64 4096 x 4096 array and the genuine
+    2-D Hartley transform of it (pocketfft_hdronly.h:3392-3445), uniform[-1, 1].  Compulsory bytes: the
+    array read once and written once per call."""
+    from impulse_amd import _lib, synth
+    from impulse_amd import fft as F
+
+    n = 4096
+    x = synth.uniform_device(n * n, -1.0, 1.0, seed=41, dtype=torch.float64).view(n, n)
+    y = torch.empty_like(x)
+    sp = torch.cuda.current_stream().cuda_stream
+    st = [n * 8, 8]
+    out = {"form": "fftpack", "workload": "f64 4096 x 4096", "cases": []}
+    for name, kind, axes in (("r2r_fftpack rows", _lib.FFTPACK, [1]),
+                             ("genuine Hartley axes [0, 1]", _lib.HARTLEY_GENUINE, [0, 1])):
+        p = F.FftpackPlan(kind, [n, n], st, st, axes, True, True)
+        ms = _timed(torch, lambda: p.execute(x.data_ptr(), y.data_ptr(), 1.0, sp), reps)
+        comp = 2 * n * n * 8
+        out["cases"].append({"case": name, "ms": round(ms, 3), "launches": p.num_launches,
+                             "compulsory_GBs": round(comp / (ms * 1e-3) / 1e9, 1),
+                             "hbm_roofline_frac": round(comp / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)})
+        del p
+    return out
+
+
 def upfirdn_form(torch, reps):
     """upfirdn / resample (impulse/signal.nim:545-809) on float64 signals, uniform[-1, 1]:
     resample by 3/2 of 2^24 samples with the generated 61-tap filter (direct algorithm), and a
@@ -403,7 +428,7 @@ def main():
     L = _lib.lib()
     _lib.check(L.impulse_fft_device_check())
     form_fns = {"ragged": ragged_form, "mfma": mfma_form, "conv": conv_form, "dct": dct_form,
-                "upfirdn": upfirdn_form}
+                "upfirdn": upfirdn_form, "fftpack": fftpack_form}
     if args.only_forms:
         print(json.dumps({"forms": [form_fns[f](torch, args.form_reps) for f in args.only_forms.split(",")]}))
         return
@@ -490,7 +515,7 @@ def main():
 
     forms = [] if args.no_forms else [ragged_form(torch, args.form_reps), mfma_form(torch, args.form_reps),
                                        conv_form(torch, args.form_reps), dct_form(torch, args.form_reps),
-                                       upfirdn_form(torch, args.form_reps)]
+                                       upfirdn_form(torch, args.form_reps), fftpack_form(torch, args.form_reps)]
     cpu = None if args.no_cpu else cpu_baseline(plist, args.cpu_seconds)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,

@@ -44,6 +44,7 @@
 
 #include "butterflies.h"
 #include "engine.h"
+#include "lds_fft.h"
 
 namespace ifft {
 
@@ -108,14 +109,38 @@ __device__ __forceinline__ double2 expipi(double a, double b)
 constexpr double kSqrt2 = 1.4142135623730951;
 
 // input point n as the half-length DCT sees it (DST sign / reversal, type-3 ortho)
+template <int TYPE>
 __device__ __forceinline__ double r2r_xp(const R2rParams &P, const double *x, long long n)
 {
     const long long N = P.N;
-    const long long orig = (!P.cosine && P.type != 2) ? N - 1 - n : n;
+    const long long orig = (!P.cosine && TYPE != 2) ? N - 1 - n : n;
     double v = x[orig * P.sa_in];
-    if (P.type == 3 && P.ortho && orig == 0) v *= kSqrt2;
-    if (!P.cosine && P.type == 2 && (n & 1)) v = -v;
+    if (TYPE == 3 && P.ortho && orig == 0) v *= kSqrt2;
+    if (!P.cosine && TYPE == 2 && (n & 1)) v = -v;
     return v;
+}
+
+// point m of the half-length complex sequence of one line (types 2-4, even N)
+template <int TYPE>
+__device__ __forceinline__ double2 half_pre(const R2rParams &P, const double *x, long long m)
+{
+    const long long N = P.N, M = P.L;
+    if constexpr (TYPE == 2) {
+        auto v = [&](long long q) {
+            return q < M ? r2r_xp<2>(P, x, 2 * q) : r2r_xp<2>(P, x, 2 * N - 2 * q - 1);
+        };
+        return make_double2(v(2 * m), v(2 * m + 1));
+    } else if constexpr (TYPE == 3) {
+        auto xe = [&](long long k) { return k < N ? r2r_xp<3>(P, x, k) : 0.0; };
+        auto V = [&](long long k) { return cmul(expipi((double)k, 2.0 * N), make_double2(xe(k), -xe(N - k))); };
+        const double2 vk = V(m), vm = cconj(V(M - m));
+        const double2 sum = cadd(vk, vm);
+        const double2 dif = cmul(expipi(2.0 * m, (double)N), csub(vk, vm));
+        return make_double2(sum.x - dif.y, sum.y + dif.x);
+    } else {
+        return cmul(make_double2(r2r_xp<4>(P, x, 2 * m), r2r_xp<4>(P, x, N - 1 - 2 * m)),
+                    cconj(expipi((double)m, (double)N)));
+    }
 }
 
 __global__ void k_r2r_pre(const R2rParams P)
@@ -129,23 +154,7 @@ __global__ void k_r2r_pre(const R2rParams P)
         auto X = [&](long long j) { return x[j * P.sa_in]; };
         double2 z = make_double2(0.0, 0.0);
         if (P.half) {
-            const long long M = L;
-            if (P.type == 2) {
-                auto v = [&](long long q) { return q < M ? r2r_xp(P, x, 2 * q) : r2r_xp(P, x, 2 * N - 2 * q - 1); };
-                z = make_double2(v(2 * m), v(2 * m + 1));
-            } else if (P.type == 3) {
-                auto xe = [&](long long k) { return k < N ? r2r_xp(P, x, k) : 0.0; };
-                auto V = [&](long long k) {
-                    return cmul(expipi((double)k, 2.0 * N), make_double2(xe(k), -xe(N - k)));
-                };
-                const double2 vk = V(m), vm = cconj(V(M - m));
-                const double2 sum = cadd(vk, vm);
-                const double2 dif = cmul(expipi(2.0 * m, (double)N), csub(vk, vm));
-                z = make_double2(sum.x - dif.y, sum.y + dif.x);
-            } else {
-                z = cmul(make_double2(r2r_xp(P, x, 2 * m), r2r_xp(P, x, N - 1 - 2 * m)),
-                         cconj(expipi((double)m, (double)N)));
-            }
+            z = P.type == 2 ? half_pre<2>(P, x, m) : P.type == 3 ? half_pre<3>(P, x, m) : half_pre<4>(P, x, m);
         } else if (P.cosine) {
             switch (P.type) {
                 case 1: {
@@ -192,12 +201,13 @@ __global__ void k_r2r_pre(const R2rParams P)
 }
 
 // output i (as the DCT forms it) of a half-length line: DST reversal / signs, scale
+template <int TYPE>
 __device__ __forceinline__ void r2r_put(const R2rParams &P, double *y, long long i, double v)
 {
     const long long N = P.N;
     long long o = i;
     if (!P.cosine) {
-        if (P.type == 2) o = N - 1 - i;
+        if (TYPE == 2) o = N - 1 - i;
         else if (i & 1) v = -v;
     }
     y[o * P.sa_out] = v * P.fct;
@@ -209,9 +219,42 @@ __host__ __device__ inline long long r2r_post_count(int half, int type, long lon
     return half ? (type == 2 ? N / 2 + 1 : N / 2) : N;
 }
 
+// outputs of post item k of one half-length line from its transformed points Zat(m)
+template <int TYPE, typename ZAT>
+__device__ __forceinline__ void half_post(const R2rParams &P, double *yl, long long k, ZAT Zat)
+{
+    const long long N = P.N, M = P.L;
+    if constexpr (TYPE == 2) {
+        const double2 zk = Zat(k == M ? 0 : k), zm = cconj(Zat(k == 0 ? 0 : M - k));
+        const double2 ev = cscale(cadd(zk, zm), 0.5);
+        const double2 d = csub(zk, zm);
+        const double2 od = make_double2(0.5 * d.y, -0.5 * d.x);
+        const double2 V = cadd(ev, cmul(cconj(expipi(2.0 * k, (double)N)), od));
+        const double2 W = cmul(cconj(expipi((double)k, 2.0 * N)), V);
+        const long long oi = P.cosine ? 0 : N - 1;   // the ortho end point as the DCT indexes it
+        double lo = 2.0 * W.x;
+        if (P.ortho && k == oi) lo *= 0.5 * kSqrt2;
+        r2r_put<2>(P, yl, k, lo);
+        if (k > 0 && k < M) {
+            double hi = -2.0 * W.y;
+            if (P.ortho && N - k == oi) hi *= 0.5 * kSqrt2;
+            r2r_put<2>(P, yl, N - k, hi);
+        }
+    } else if constexpr (TYPE == 3) {
+        const double2 z = Zat(k);
+        auto sigma = [&](long long q) { return q < M ? 2 * q : 2 * N - 2 * q - 1; };
+        r2r_put<3>(P, yl, sigma(2 * k), z.x);
+        r2r_put<3>(P, yl, sigma(2 * k + 1), z.y);
+    } else {
+        const double2 S = cmul(Zat(k), cconj(expipi((double)(4 * k + 1), 4.0 * N)));
+        r2r_put<4>(P, yl, 2 * k, 2.0 * S.x);
+        r2r_put<4>(P, yl, N - 1 - 2 * k, -2.0 * S.y);
+    }
+}
+
 __global__ void k_r2r_post(const R2rParams P)
 {
-    const long long N = P.N, L = P.L;
+    const long long N = P.N;
     const long long cnt = r2r_post_count(P.half, P.type, N), total = P.lines * cnt;
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (long long)gridDim.x * blockDim.x) {
@@ -220,33 +263,9 @@ __global__ void k_r2r_post(const R2rParams P)
         double *yl = P.dst + line_off(P.out, line);
         auto Zat = [&](long long m) { return P.work[r2r_widx(P, line, m)]; };
         if (P.half) {
-            const long long M = L;
-            if (P.type == 2) {
-                const double2 zk = Zat(k == M ? 0 : k), zm = cconj(Zat(k == 0 ? 0 : M - k));
-                const double2 ev = cscale(cadd(zk, zm), 0.5);
-                const double2 d = csub(zk, zm);
-                const double2 od = make_double2(0.5 * d.y, -0.5 * d.x);
-                const double2 V = cadd(ev, cmul(cconj(expipi(2.0 * k, (double)N)), od));
-                const double2 W = cmul(cconj(expipi((double)k, 2.0 * N)), V);
-                const long long oi = P.cosine ? 0 : N - 1;   // the ortho end point as the DCT indexes it
-                double lo = 2.0 * W.x;
-                if (P.ortho && k == oi) lo *= 0.5 * kSqrt2;
-                r2r_put(P, yl, k, lo);
-                if (k > 0 && k < M) {
-                    double hi = -2.0 * W.y;
-                    if (P.ortho && N - k == oi) hi *= 0.5 * kSqrt2;
-                    r2r_put(P, yl, N - k, hi);
-                }
-            } else if (P.type == 3) {
-                const double2 z = Zat(k);
-                auto sigma = [&](long long q) { return q < M ? 2 * q : 2 * N - 2 * q - 1; };
-                r2r_put(P, yl, sigma(2 * k), z.x);
-                r2r_put(P, yl, sigma(2 * k + 1), z.y);
-            } else {
-                const double2 S = cmul(Zat(k), cconj(expipi((double)(4 * k + 1), 4.0 * N)));
-                r2r_put(P, yl, 2 * k, 2.0 * S.x);
-                r2r_put(P, yl, N - 1 - 2 * k, -2.0 * S.y);
-            }
+            if (P.type == 2) half_post<2>(P, yl, k, Zat);
+            else if (P.type == 3) half_post<3>(P, yl, k, Zat);
+            else half_post<4>(P, yl, k, Zat);
             continue;
         }
         double y;
@@ -286,12 +305,89 @@ __global__ void k_r2r_post(const R2rParams P)
     }
 }
 
+// Types 2-4 at even N with the N/2-point transform in LDS: `count` lines per
+// workgroup (count M <= 8 THREADS points), each loaded, transformed and stored
+// by THREADS / count threads, so HBM sees each line read once and written once.
+// One instance per type keeps only that type's pre- and post-processing in
+// the kernel; three waves per SIMD (168 registers) let three 256-thread
+// workgroups share a CU.
+constexpr int kR2rFusedMax = 4096;   // longest N/2 the fused kernel takes (512 threads)
+#ifndef DCT_FUSED_WPE
+#define DCT_FUSED_WPE 3
+#endif
+
+template <int TYPE, int THREADS>
+__global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(DCT_FUSED_WPE)))
+k_r2r_fused(const R2rParams P, int count)
+{
+    extern __shared__ double2 zl[];
+    const int M = (int)P.L;
+    const int NT = THREADS / count;
+    const int t = threadIdx.x / NT, i = threadIdx.x % NT;
+    const long long cnt = r2r_post_count(1, TYPE, P.N);
+    const double s = TYPE == 3 ? 1.0 : -1.0;
+    for (long long l0 = (long long)blockIdx.x * count; l0 < P.lines; l0 += (long long)gridDim.x * count) {
+        const long long line = l0 + t;
+        const bool valid = line < P.lines;
+        if (valid) {
+            const double *x = P.src + line_off(P.in, line);
+            for (int m = i; m < M; m += NT) zl[lpad(t * M + m)] = half_pre<TYPE>(P, x, m);
+        }
+        __syncthreads();
+        lds_fft(zl, M, count, s, 0);   // ends with a barrier
+        if (valid) {
+            double *yl = P.dst + line_off(P.out, line);
+            auto Zat = [&](long long m) { return zl[lpad(t * M + (int)m)]; };
+            for (long long k = i; k < cnt; k += NT) half_post<TYPE>(P, yl, k, Zat);
+        }
+        __syncthreads();
+    }
+}
+
+static bool smooth13_r2r(long long n)
+{
+    for (int p : {2, 3, 5, 7, 11, 13})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+// workgroup size of the fused kernel for N/2 = M, and lines per workgroup: the
+// largest power of two <= 64 with count M <= 8 threads
+static int fused_threads(long long M) { return M <= 2048 ? 256 : 512; }
+static int fused_count(long long M)
+{
+    const long long budget = 8LL * fused_threads(M);
+    int c = 1;
+    while (2 * c <= 64 && 2 * c * M <= budget) c *= 2;
+    return c;
+}
+
+template <int THREADS>
+static hipError_t launch_fused_t(int type, const R2rParams &Q, unsigned grid, size_t lds, int count, hipStream_t s)
+{
+    static bool attr_done = false;
+    if (!attr_done) {
+        for (const void *f : {(const void *)k_r2r_fused<2, THREADS>, (const void *)k_r2r_fused<3, THREADS>,
+                              (const void *)k_r2r_fused<4, THREADS>})
+            if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024) != hipSuccess)
+                (void)hipGetLastError();
+        attr_done = true;
+    }
+    switch (type) {
+        case 2: hipLaunchKernelGGL((k_r2r_fused<2, THREADS>), dim3(grid), dim3(THREADS), lds, s, Q, count); break;
+        case 3: hipLaunchKernelGGL((k_r2r_fused<3, THREADS>), dim3(grid), dim3(THREADS), lds, s, Q, count); break;
+        default: hipLaunchKernelGGL((k_r2r_fused<4, THREADS>), dim3(grid), dim3(THREADS), lds, s, Q, count); break;
+    }
+    return hipGetLastError();
+}
+
 struct R2r {
     int cosine = 1, type = 2, ortho = 0;
     struct Axis {
         int axis;
         long long N, L, lines;
         int forward, half, col;
+        int fused;                        // one LDS kernel per line batch (no sub-plan)
         LineMap in_first, in_next, out;
         long long sa_in_first, sa_in_next, sa_out;
         std::unique_ptr<Plan> fft;
@@ -381,6 +477,12 @@ int r2r_plan_create(std::unique_ptr<Plan> &out, int kind, int type, int ortho, i
         A.sa_in_next = A.sa_out = P.sout[A.axis] / 8;
         // threads walk along lines when the axis is the input's unit-stride dimension, else across lines
         A.col = A.lines > 1 && std::llabs(a == 0 ? A.sa_in_first : A.sa_in_next) != 1;
+        // lines along the unit-stride axis, N/2 13-smooth and within the LDS budget: one fused kernel
+        A.fused = A.half && !A.col && A.L <= kR2rFusedMax && smooth13_r2r(A.L);
+        if (A.fused) {
+            R->ax.push_back(std::move(A));
+            continue;
+        }
         // c2c over the workspace, in place: [lines, L] along axis 1, or [L, lines] along axis 0
         const size_t fshape_r[2] = {(size_t)A.lines, (size_t)A.L}, fshape_c[2] = {(size_t)A.L, (size_t)A.lines};
         const ptrdiff_t fstr_r[2] = {(ptrdiff_t)(A.L * 16), 16}, fstr_c[2] = {(ptrdiff_t)(A.lines * 16), 16};
@@ -391,7 +493,7 @@ int r2r_plan_create(std::unique_ptr<Plan> &out, int kind, int type, int ortho, i
         work = std::max(work, (size_t)(A.lines * A.L) * sizeof(double2));
         R->ax.push_back(std::move(A));
     }
-    if (R->work.alloc(work)) { err = {E_NOMEM, "DCT/DST workspace allocation failed"}; return E_NOMEM; }
+    if (work && R->work.alloc(work)) { err = {E_NOMEM, "DCT/DST workspace allocation failed"}; return E_NOMEM; }
     P.r2r = R.release();
     out = std::move(p);
     return OK;
@@ -400,7 +502,7 @@ int r2r_plan_create(std::unique_ptr<Plan> &out, int kind, int type, int ortho, i
 int r2r_num_launches(const R2r &R)
 {
     int n = 0;
-    for (const auto &A : R.ax) n += 2 + (int)A.fft->passes.size();
+    for (const auto &A : R.ax) n += A.fused ? 1 : 2 + (int)A.fft->passes.size();
     return n;
 }
 
@@ -426,6 +528,16 @@ int r2r_execute(Plan &P, const void *in, void *out, double fct, hipStream_t s, E
         Q.half = A.half;
         Q.col = A.col;
         Q.fct = a == 0 ? fct : 1.0;
+        if (A.fused) {
+            const int count = fused_count(A.L);
+            const long long groups = (A.lines + count - 1) / count;
+            const size_t lds = (size_t)lds_points_padded((int)(count * A.L)) * sizeof(double2);
+            const unsigned grid = (unsigned)std::min<long long>(groups, 1 << 20);
+            hipError_t e = fused_threads(A.L) == 256 ? launch_fused_t<256>(R.type, Q, grid, lds, count, s)
+                                                     : launch_fused_t<512>(R.type, Q, grid, lds, count, s);
+            if (e != hipSuccess) { err = {E_HIP, std::string("DCT/DST launch: ") + hipGetErrorString(e)}; return E_HIP; }
+            continue;
+        }
         const long long pre = A.lines * A.L, post = A.lines * r2r_post_count(A.half, R.type, A.N);
         const unsigned gpre = (unsigned)std::min<long long>((pre + 255) / 256, 1 << 20);
         const unsigned gpost = (unsigned)std::min<long long>((post + 255) / 256, 1 << 20);

@@ -1,0 +1,132 @@
+// A mixed-radix Stockham FFT of runtime length on data in LDS, run by the
+// threads of a workgroup: radices 2, 3, 4, 5, 7, 8, 11, 13, 16, FP64, twiddles
+// from sincospi.  Shared by the ragged-batch kernel and the fused DCT kernel.
+#pragma once
+
+#include "butterflies.h"
+
+namespace ifft {
+
+// Every configuration keeps lds_points <= 8 threads, so a thread owns at most
+// 8 points of any LDS batch and ceil(8 / R) butterflies of a radix-R stage.
+constexpr int kPointsPerThread = 8;
+__host__ __device__ constexpr int nb_rg(int R) { return (kPointsPerThread + R - 1) / R; }
+
+// exp(sign 2 pi i k / L), 0 <= k < L < 2^31
+__device__ __forceinline__ double2 twid32(int k, int L, double sign)
+{
+    double s, c;
+    sincospi(2.0 * (double)k / (double)L, &s, &c);
+    return make_double2(c, sign * s);
+}
+
+// LDS holds transform data at padded positions: one word skipped every 16, so
+// that the power-of-two strides of the stages (16 b, 16 p) spread over the banks.
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+constexpr int lds_points_padded(int n) { return n + n / 16 + 1; }
+
+// Stockham stage on one transform in LDS: butterflies b = i, i + NT, ...
+// The twiddle roots are generated before the values are loaded, so the
+// sincospi evaluation does not hold the R NB values live.
+template <int R, int NB>
+__device__ __forceinline__ void rstage_body(double2 *lds, int base, int N, int p, int i, int NT, double s)
+{
+    const int nb = N / R;
+    double2 w1[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        const int k = (i + c * NT) % p;
+        w1[c] = k ? twid32(k, p * R, s) : make_double2(1.0, 0.0);
+    }
+    double2 v[NB][R];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        const int b = i + c * NT;
+        if (b < nb)
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[c][q] = lds[lpad(base + b + q * nb)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        const int b = i + c * NT;
+        if (b < nb) {
+            const int k = b % p;
+            if (k) {
+                double2 w = w1[c];
+#pragma unroll
+                for (int q = 1; q < R; ++q) {
+                    v[c][q] = cmul(v[c][q], w);
+                    if (q + 1 < R) w = cmul(w, w1[c]);
+                }
+            }
+            dft<R>(v[c], s);
+            const int j = (b - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) lds[lpad(base + j + q * p)] = v[c][q];
+        }
+    }
+    __syncthreads();
+}
+
+// the stage as a call of its own (one copy per radix in the ragged kernel, which keeps its
+// register budget to the largest stage)
+template <int R, int NB>
+__device__ __noinline__ void rstage(double2 *lds, int base, int N, int p, int i, int NT, double s)
+{
+    rstage_body<R, NB>(lds, base, N, p, i, NT, s);
+}
+
+// Radix sequence of a 13-smooth L as 4-bit codes (stage 0 in the low bits),
+// in registers: powers of two as 16s and one 2/4/8, odd radices ascending,
+// before (order 1, descending) or after (order 0) the powers of two.
+__device__ __forceinline__ unsigned long long radix_codes(int L, int order, int &ns)
+{
+    // code -> radix: 1:2 2:3 3:4 4:5 5:7 6:8 7:11 8:13 9:16
+    unsigned long long pw = 0, od = 0;
+    int np = 0, no = 0;
+    int n = L;
+    int twos = __builtin_ctz((unsigned)n);
+    n >>= twos;
+    for (; twos >= 4; twos -= 4) pw |= 9ull << (4 * np++);
+    if (twos) pw |= (unsigned long long)(twos == 3 ? 6 : twos == 2 ? 3 : 1) << (4 * np++);
+    const int ps[5] = {3, 5, 7, 11, 13}, cs[5] = {2, 4, 5, 7, 8};
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+        while (n % ps[q] == 0) {
+            n /= ps[q];
+            if (order == 1) od = (od << 4) | (unsigned long long)cs[q];   // descending: latest first
+            else od |= (unsigned long long)cs[q] << (4 * no);
+            ++no;
+        }
+    ns = np + no;
+    return order == 1 ? (od | (pw << (4 * no))) : (pw | (od << (4 * np)));
+}
+
+// `count` transforms of length L at lds + t L, blockDim.x / count threads each
+// (count divides blockDim.x).  Ends with a barrier.
+__device__ __noinline__ void lds_fft(double2 *lds, int L, int count, double s, int order)
+{
+    int ns;
+    unsigned long long codes = radix_codes(L, order, ns);
+    const int NT = blockDim.x / count;
+    const int t = threadIdx.x / NT, i = threadIdx.x % NT;
+    const int base = t * L;
+    int p = 1;
+    for (int st = 0; st < ns; ++st, codes >>= 4) {
+        switch ((int)(codes & 15)) {
+            case 1: rstage<2, nb_rg(2)>(lds, base, L, p, i, NT, s); p *= 2; break;
+            case 2: rstage<3, nb_rg(3)>(lds, base, L, p, i, NT, s); p *= 3; break;
+            case 3: rstage<4, nb_rg(4)>(lds, base, L, p, i, NT, s); p *= 4; break;
+            case 4: rstage<5, nb_rg(5)>(lds, base, L, p, i, NT, s); p *= 5; break;
+            case 5: rstage<7, nb_rg(7)>(lds, base, L, p, i, NT, s); p *= 7; break;
+            case 6: rstage<8, nb_rg(8)>(lds, base, L, p, i, NT, s); p *= 8; break;
+            case 7: rstage<11, nb_rg(11)>(lds, base, L, p, i, NT, s); p *= 11; break;
+            case 8: rstage<13, nb_rg(13)>(lds, base, L, p, i, NT, s); p *= 13; break;
+            default: rstage<16, nb_rg(16)>(lds, base, L, p, i, NT, s); p *= 16; break;
+        }
+    }
+    if (ns == 0) __syncthreads();
+}
+
+}  // namespace ifft

@@ -177,7 +177,7 @@ def test_c_api_device_plan_reuse():
     shape = [48, 250]
     st = [250 * 8, 8]
     p = F.Plan(_lib.DST, shape, st, st, [1], True, r2r_type=4)
-    assert p.num_launches >= 3
+    assert p.num_launches >= 1
     x = np.random.default_rng(9).uniform(-1, 1, shape)
     xd = torch.from_numpy(x).cuda()
     for fct in (1.0, 0.25, -3.0):

@@ -26,6 +26,7 @@
 
 #include "butterflies.h"
 #include "engine.h"
+#include "lds_fft.h"
 
 namespace ifft {
 
@@ -174,6 +175,107 @@ __global__ void k_rfp_genuine(const GenParams G)
     }
 }
 
+// Forward r2r_fftpack (MODE 0) and the separable Hartley transform (MODE 1) of
+// even-length lines along the input's unit-stride axis in one kernel: a line's
+// z[m] = x[2m] + i x[2m+1] (M = N/2 points, the sign flips of positions 2, 4,
+// ... fused into the load when `flip`) is transformed in LDS, split into
+// X[k], X[M-k] = (Z[k] + conj Z[M-k]) / 2 -+ i/2 W_N^{+-k} (Z[k] - conj Z[M-k]),
+// and written as halfcomplex or Hartley values; `count` lines per workgroup.
+constexpr int kRfpFusedMax = 4096;
+template <int MODE, int THREADS>
+__global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(3)))
+k_rfp_fused(const RfpParams P, int count, int flip, double fct)
+{
+    extern __shared__ double2 zl[];
+    const int M = (int)(P.N / 2);
+    const long long N = P.N;
+    const int NT = THREADS / count;
+    const int t = threadIdx.x / NT, i = threadIdx.x % NT;
+    for (long long l0 = (long long)blockIdx.x * count; l0 < P.lines; l0 += (long long)gridDim.x * count) {
+        const long long line = l0 + t;
+        const bool valid = line < P.lines;
+        if (valid) {
+            const double *x = P.rsrc + rfp_line_off(P.lin, line);
+            for (int m = i; m < M; m += NT) {
+                double re = x[(2LL * m) * P.sa_in];
+                const double im = x[(2LL * m + 1) * P.sa_in];
+                if (flip && m > 0) re = -re;
+                zl[lpad(t * M + m)] = make_double2(re, im);
+            }
+        }
+        __syncthreads();
+        lds_fft(zl, M, count, -1.0, 0);   // ends with a barrier
+        if (valid) {
+            double *y = P.rdst + rfp_line_off(P.lout, line);
+            const long long sa = P.sa_out;
+            for (int k = i; 2 * k <= M; k += NT) {
+                const double2 zk = zl[lpad(t * M + k)], zm = zl[lpad(t * M + (k == 0 ? 0 : M - k))];
+                double sn, cs;
+                sincospi(-(double)k / (double)M, &sn, &cs);   // W_N^k
+                const double2 w = make_double2(cs, sn);
+                const double2 e1 = cscale(cadd(zk, cconj(zm)), 0.5), d1 = csub(zk, cconj(zm));
+                const double2 o1 = make_double2(0.5 * d1.y, -0.5 * d1.x);
+                const double2 e2 = cscale(cadd(zm, cconj(zk)), 0.5), d2 = csub(zm, cconj(zk));
+                const double2 o2 = make_double2(0.5 * d2.y, -0.5 * d2.x);
+                const double2 Xk = cscale(cadd(e1, cmul(o1, w)), fct);
+                const double2 Xm = cscale(cadd(e2, cmul(o2, make_double2(-w.x, w.y))), fct);
+                auto put = [&](int q, double2 X) {
+                    if (MODE == 0) {
+                        if (q == 0) y[0] = X.x;
+                        else if (q == M) y[(N - 1) * sa] = X.x;
+                        else {
+                            y[(2LL * q - 1) * sa] = X.x;
+                            y[(2LL * q) * sa] = X.y;
+                        }
+                    } else {
+                        if (q == 0 || q == M) y[(long long)q * sa] = X.x;
+                        else {
+                            y[(long long)q * sa] = X.x + X.y;
+                            y[(N - q) * sa] = X.x - X.y;
+                        }
+                    }
+                };
+                put(k, Xk);
+                if (M - k != k) put(M - k, Xm);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+static bool rfp_smooth13(long long n)
+{
+    for (int p : {2, 3, 5, 7, 11, 13})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+static int rfp_fused_threads(long long M) { return M <= 2048 ? 256 : 512; }
+static int rfp_fused_count(long long M)
+{
+    const long long budget = 8LL * rfp_fused_threads(M);
+    int c = 1;
+    while (2 * c <= 64 && 2 * c * M <= budget) c *= 2;
+    return c;
+}
+
+template <int THREADS>
+static hipError_t launch_rfp_fused(int mode, const RfpParams &Q, unsigned grid, size_t lds, int count, int flip,
+                                   double fct, hipStream_t s)
+{
+    static bool attr_done = false;
+    if (!attr_done) {
+        for (const void *f : {(const void *)k_rfp_fused<0, THREADS>, (const void *)k_rfp_fused<1, THREADS>})
+            if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024) != hipSuccess)
+                (void)hipGetLastError();
+        attr_done = true;
+    }
+    if (mode == 0)
+        hipLaunchKernelGGL((k_rfp_fused<0, THREADS>), dim3(grid), dim3(THREADS), lds, s, Q, count, flip, fct);
+    else
+        hipLaunchKernelGGL((k_rfp_fused<1, THREADS>), dim3(grid), dim3(THREADS), lds, s, Q, count, flip, fct);
+    return hipGetLastError();
+}
+
 struct Rfp {
     int kind = 7;                 // 7 r2r_fftpack, 8 separable Hartley, 9 genuine Hartley
     int r2h = 1, forward = 1;
@@ -184,6 +286,7 @@ struct Rfp {
         long long sa_in_first, sa_in_next, sa_out;
         std::unique_ptr<Plan> plan;        // r2c (to the workspace) or c2r (from it)
         bool negcopy = false;              // forward r2r_fftpack without real2hermitian
+        bool fused = false;                // one LDS kernel per line batch (no sub-plan)
     };
     std::vector<Axis> ax;
     std::unique_ptr<Plan> gen;             // genuine: n-D r2c into the workspace
@@ -307,6 +410,15 @@ int rfp_plan_create(std::unique_ptr<Plan> &out, int kind, int r2h, int forward, 
             const std::vector<ptrdiff_t> wst = work_strides(P.shape, A.axis, A.H);
             const size_t ax1[1] = {(size_t)A.axis};
             const bool to_spec = !(kind == 7 && !forward);   // forward and Hartley run r2c; backward runs c2r
+            // even lines along the input's unit-stride axis, N/2 13-smooth within the LDS budget: fused
+            const long long sa_src = a == 0 ? A.sa_in_first : A.sa_in_next;
+            A.fused = to_spec && A.N % 2 == 0 && A.N / 2 <= kRfpFusedMax && rfp_smooth13(A.N / 2) &&
+                      (sa_src == 1 || A.lines == 1);
+            if (A.fused) {
+                A.negcopy = false;
+                R->ax.push_back(std::move(A));
+                continue;
+            }
             A.negcopy = kind == 7 && forward && !r2h;
             if (A.negcopy) {
                 // r2c from the sign-flipped copy [lines, N]
@@ -340,7 +452,7 @@ int rfp_num_launches(const Rfp &R)
 {
     if (R.gen) return (int)R.gen->passes.size() + 1;
     int n = 0;
-    for (const auto &A : R.ax) n += 1 + (int)A.plan->passes.size();
+    for (const auto &A : R.ax) n += A.fused ? 1 : 1 + (int)A.plan->passes.size() + (A.negcopy ? 1 : 0);
     return n;
 }
 
@@ -397,6 +509,23 @@ int rfp_execute(Plan &P, const void *in, void *out, double fct, hipStream_t s, E
                 hipLaunchKernelGGL(k_rfp_negate, dim3(rfp_grid(A.lines * (A.N / 2 + 1))), dim3(256), 0, s, Q);
                 RFP_TRY_HIP(hipGetLastError());
             }
+            continue;
+        }
+        if (A.fused) {
+            const long long M = A.N / 2;
+            const int count = rfp_fused_count(M);
+            const long long groups = (A.lines + count - 1) / count;
+            const size_t lds = (size_t)lds_points_padded((int)(count * M)) * sizeof(double2);
+            const unsigned grid = (unsigned)std::min<long long>(groups, 1 << 20);
+            Q.rsrc = (const double *)src;
+            Q.lin = a == 0 ? A.lin_first : A.lin_next;
+            Q.sa_in = a == 0 ? A.sa_in_first : A.sa_in_next;
+            Q.rdst = (double *)out;
+            const int mode = R.kind == 7 ? 0 : 1;
+            const int flip = R.kind == 7 && R.forward && !R.r2h;
+            hipError_t e = rfp_fused_threads(M) == 256 ? launch_rfp_fused<256>(mode, Q, grid, lds, count, flip, f, s)
+                                                       : launch_rfp_fused<512>(mode, Q, grid, lds, count, flip, f, s);
+            RFP_TRY_HIP(e);
             continue;
         }
         const void *rsrc = src;

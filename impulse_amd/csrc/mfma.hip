@@ -266,7 +266,7 @@ __global__ void __launch_bounds__(16 * tile_of<VAR>()) k_mfma256(const MfmaParam
     __shared__ double2 tws[512];   // W_256^m and W_65536^m (m < 256), forward sign
     const int tid = threadIdx.x;
     // the tiles of one signal run on one XCD, so its 2 KB rows are fetched into one L2
-    const long long tile = xcd_block(blockIdx.x, gridDim.x);
+    const long long tile = xcd_span_block(blockIdx.x, gridDim.x);
     const long long L = P.L;
     long long base = 0;   // first element of the signal (COL/ROW) or of transform 0 of the tile (SINGLE)
     int t0 = 0, ntr = TILE;

@@ -44,8 +44,11 @@ enum StoreMode : int {
     ST_C2R_HALF = 4,  // x[2 pos] = Re z, x[2 pos + 1] = Im z
     ST_C2R_PAIR = 5,  // xa[pos] = Re z, xb[pos] = Im z
     ST_MUL_TABLE = 6, // y[pos] = z[n] tab[pos]                      (Bluestein spectrum product)
-    ST_R2C_ROWPAIR = 7 // even r2c, last four-step pass: rows j and L2-j of one workgroup hold
-                       // Z[k] and Z[M-k], stored as the half spectrum X[k], X[M-k]
+    ST_R2C_ROWPAIR = 7, // even r2c, last four-step pass: rows j and L2-j of one workgroup hold
+                        // Z[k] and Z[M-k], stored as the half spectrum X[k], X[M-k]
+    ST_CONV_TWIDDLE = 8 // four-step Bluestein row pass: row j is transformed, multiplied by
+                        // row j of the transposed kernel spectrum (mtab), transformed back with
+                        // the opposite sign and stored as y[pos] = z[n] conj(W_L^{j n})
 };
 
 enum SideKind : int { SIDE_DIMS = 0, SIDE_DIMS_PAIR = 1, SIDE_LINEAR = 2 };

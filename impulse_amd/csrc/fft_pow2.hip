@@ -63,7 +63,7 @@ template <int N, int MAP> constexpr int pow2_ld() { return N + N / 16 + (MAP == 
 // W^m, W^2m, W^4m, W^8m come from the table, every other power is one to three
 // products of those, and each twiddle is used as soon as it is formed so that
 // at most five are live at a time.
-enum Pow2Io { IO_GENERIC = 0, IO_PLAIN = 1, IO_TWIDDLE = 2, IO_R2C_ROWPAIR = 3 };
+enum Pow2Io { IO_GENERIC = 0, IO_PLAIN = 1, IO_TWIDDLE = 2, IO_R2C_ROWPAIR = 3, IO_CONV = 4 };
 
 // Row assignment of the fused real post-process (ST_R2C_ROWPAIR, rows of L1 = N
 // points, L2 rows per signal, Z[j + L2 n] in row j).  Z[M - k] for k = j + L2 n
@@ -219,7 +219,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
     }
     dft<16>(v, s);
 
-    // ---- exchanges and the remaining stages
+    // ---- exchanges and the remaining stages (sign sg)
+    auto stages = [&](double sg) {
     int p = 16;   // product of the radices applied so far
 #pragma unroll
     for (int st = 1; st < NST; ++st) {
@@ -251,24 +252,61 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
             const int b = i + c * NT;
             const int k = b % p;
             switch (R) {
-                case 16: apply_twiddles<16>(v + c * R, rtw, k * (N / (p * 16)), s); break;
-                case 8: apply_twiddles<8>(v + c * R, rtw, k * (N / (p * 8)), s); break;
-                case 4: apply_twiddles<4>(v + c * R, rtw, k * (N / (p * 4)), s); break;
-                default: apply_twiddles<2>(v + c * R, rtw, k * (N / (p * 2)), s); break;
+                case 16: apply_twiddles<16>(v + c * R, rtw, k * (N / (p * 16)), sg); break;
+                case 8: apply_twiddles<8>(v + c * R, rtw, k * (N / (p * 8)), sg); break;
+                case 4: apply_twiddles<4>(v + c * R, rtw, k * (N / (p * 4)), sg); break;
+                default: apply_twiddles<2>(v + c * R, rtw, k * (N / (p * 2)), sg); break;
             }
         }
         switch (R) {
-            case 16: dft_groups<16>(v, s); break;
-            case 8: dft_groups<8>(v, s); break;
-            case 4: dft_groups<4>(v, s); break;
-            default: dft_groups<2>(v, s); break;
+            case 16: dft_groups<16>(v, sg); break;
+            case 8: dft_groups<8>(v, sg); break;
+            case 4: dft_groups<4>(v, sg); break;
+            default: dft_groups<2>(v, sg); break;
         }
         p *= R;
     }
+    };
+    stages(s);
     // the last stage had p_last = N / R_L, so butterfly b = i + c NT (k = b)
     // holds outputs y[b + q N / R_L]
     constexpr int RL = RS::r[NST - 1];
     constexpr int NB = 16 / RL;
+
+    if constexpr (IO == IO_CONV) {
+        // Bluestein row pass: y[n] times row j of the transposed kernel spectrum
+        // (one table of L = P.tw_L points, 32-bit byte offsets), then back to the
+        // stage-0 layout through LDS (real parts, then imaginary parts; the last
+        // exchange ended with a barrier) and the transform with the opposite sign
+        const __amdgpu_buffer_rsrc_t rsp = make_rsrc(P.mtab, P.tw_L * 16);
+        const int sbase = (int)T.j * N;
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            const int b = i + c * NT;
+#pragma unroll
+            for (int q = 0; q < RL; ++q)
+                v[c * RL + q] = cmul(v[c * RL + q], bload2(rsp, (sbase + b + q * (N / RL)) * 16));
+        }
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                const int b = i + c * NT;
+#pragma unroll
+                for (int q = 0; q < RL; ++q) at(b + q * (N / RL)) = half ? v[c * RL + q].y : v[c * RL + q].x;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const double x = at(i + q * (N / 16));
+                if (half) v[q].y = x;
+                else v[q].x = x;
+            }
+            __syncthreads();
+        }
+        dft<16>(v, -s);
+        stages(-s);
+    }
 
     if constexpr (MAP != MAP_ROWCOL) {
         if (!valid) return;
@@ -276,7 +314,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
             const __amdgpu_buffer_rsrc_t rout = make_rsrc((char *)P.dst + P.dst_lo, P.dst_span);
             const int ostep = (int)(P.out.pst * P.out.sax * 16);
             const int obase = (int)((T.oa + T.op0 * P.out.sax) * 16 - P.dst_lo);
-            constexpr bool twid = IO == IO_TWIDDLE;
+            constexpr bool twid = IO == IO_TWIDDLE || IO == IO_CONV;
+            const double so = IO == IO_CONV ? -s : s;   // the row pass stores conj(W^{j n})
             if constexpr (twid) {
                 // W_L^{j n}, n = b + q N/RL: W^{j b} times powers of W^{j N/RL}
                 // (binary powers, at most four products per twiddle)
@@ -299,7 +338,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
 #pragma unroll
                         for (int e = 0; e < 4; ++e)
                             if (q & (1 << e)) w = cmul(w, pw[e]);
-                        bstore2(rout, obase + n * ostep, cmul(v[c * RL + q], tw_sign(w, s)));
+                        bstore2(rout, obase + n * ostep, cmul(v[c * RL + q], tw_sign(w, so)));
                     }
                 }
             } else {
@@ -471,11 +510,28 @@ static hipError_t launch_rowpair(const FftParams &P, unsigned grid, unsigned blo
     }
 }
 
+// the four-step Bluestein row pass (transform, spectrum product, inverse transform)
+// on the row mapping
+template <int N>
+static hipError_t launch_conv(const FftParams &P, unsigned grid, unsigned block, size_t lds, hipStream_t s)
+{
+    static bool attr_done = false;
+    if (!attr_done) {
+        if (hipFuncSetAttribute((const void *)k_fft_pow2<N, MAP_ROW, IO_CONV>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 8192) != hipSuccess)
+            (void)hipGetLastError();
+        attr_done = true;
+    }
+    hipLaunchKernelGGL((k_fft_pow2<N, MAP_ROW, IO_CONV>), dim3(grid), dim3(block), lds, s, P);
+    return hipGetLastError();
+}
+
 template <int N>
 static hipError_t launch_n(const FftParams &P, int map, int io, unsigned grid, unsigned block, size_t lds,
                            hipStream_t s)
 {
     if (io == IO_R2C_ROWPAIR) return map == MAP_ROWCOL ? launch_rowpair<N>(P, grid, block, lds, s) : hipErrorInvalidValue;
+    if (io == IO_CONV) return map == MAP_ROW ? launch_conv<N>(P, grid, block, lds, s) : hipErrorInvalidValue;
     switch (io) {
         case IO_PLAIN: return launch_io<N, IO_PLAIN>(P, map, grid, block, lds, s);
         case IO_TWIDDLE: return launch_io<N, IO_TWIDDLE>(P, map, grid, block, lds, s);
@@ -513,6 +569,7 @@ bool pow2_fast_load(const FftParams &P)
 int pow2_io(const FftParams &P)
 {
     if (P.store_mode == ST_R2C_ROWPAIR) return IO_R2C_ROWPAIR;
+    if (P.store_mode == ST_CONV_TWIDDLE) return IO_CONV;
     return pow2_fast(P) ? (P.store_mode == ST_TWIDDLE ? IO_TWIDDLE : IO_PLAIN) : IO_GENERIC;
 }
 

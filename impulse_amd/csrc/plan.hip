@@ -611,7 +611,7 @@ struct Builder {
         if (real_half && !table_two_level(2 * Ls, rlo, rhi, rS)) return false;
         const double2 *twA = table_direct(L2), *twB = table_direct(L1);
         if (!twA || !twB) return false;
-        const double2 *chirp = nullptr, *spec = nullptr;
+        const double2 *chirp = nullptr, *spec = nullptr, *spec_rows = nullptr;
         if (blue && !bluestein_tables(Ls, n2, chirp, spec)) return false;
 
         // Optional batch chunks (IMPULSE_FFT_MALL_MB): the workspace of one chunk
@@ -720,6 +720,53 @@ struct Builder {
                 return add_fft(B, REF_S0, J.dst, J.fct, false, true);
             }
 
+            if (conv_rows_ok(L1, n2, ws)) {
+                // Bluestein in three passes: A1 (chirp in, forward columns, twiddle) ->
+                // B (each row in place: forward, times the spectrum, inverse, conjugate
+                // twiddle) -> A2 (inverse columns, chirp out, natural order).  With
+                // n = j + L1 n' and the spectrum index f = k + L2 k', the inverse
+                // transform splits the same way as the forward one, so the spectrum
+                // never has to be stored in natural order.
+                if (!spec_rows && !(spec_rows = spectrum_rows(spec, L1, L2))) return false;
+                FftParams A1 = passA(P0);
+                A1.load_mode = lmode;
+                A1.sign = -1;
+                A1.chirp = chirp;
+                A1.chirp_in = 1;
+                A1.conj_in = conj;
+                A1.ilim = Ls;
+                if (!add_fft(A1, J.src, REF_S0, false, true, true)) return false;
+                FftParams B = passB(P0);
+                B.sign = -1;
+                B.store_mode = ST_CONV_TWIDDLE;
+                B.mtab = spec_rows;
+                side_lin(B.out, n2, L1, 1);
+                B.tw_lo = lo; B.tw_hi = hi; B.tw_S = S; B.tw_L = n2; B.tw_mul = 1; B.tw_logS = ilog2(S);
+                if (!add_conv_rows(B, REF_S0, REF_S0)) return false;
+                FftParams P1 = P0;
+                side_lin(P1.in, n2, 1, L1);
+                FftParams A2 = passA(P1);
+                A2.load_mode = LD_C2C;
+                A2.sign = 1;
+                A2.chirp = chirp;
+                A2.chirp_out = 1;
+                A2.conj_out = conj;
+                A2.olim = Ls;
+                if (J.kind == J_R2C) {
+                    A2.store_mode = ST_C2C;   // natural order into the workspace, then the real post-process
+                    if (!add_fft(A2, REF_S0, REF_S1, false, true, true)) return false;
+                    return add_post(J, pair, Ls, n2, rlo, rhi, rS, u0, cn);
+                }
+                Side outA;
+                side_dims(outA, J, false, pair);
+                outA.ipos = 1;
+                outA.pst = L1;
+                outA.u0 = u0;
+                A2.out = outA;
+                A2.store_mode = (J.kind == J_C2C) ? ST_C2C : (pair ? ST_C2R_PAIR : ST_C2R_HALF);
+                return add_fft(A2, REF_S0, J.dst, J.fct, true, true);
+            }
+
             // Bluestein: A1 (chirp in) -> B1 (x spectrum) -> A2 (inverse) -> B2 (chirp out)
             FftParams A1 = passA(P0);
             A1.load_mode = lmode;
@@ -761,6 +808,67 @@ struct Builder {
             cn = std::min(chunk, nlog - u0);
             if (!emit(u0)) return false;
         }
+        return true;
+    }
+
+    // the three-pass Bluestein: the row pass runs on the power-of-two row kernel
+    // (rows of L1 points), whose buffer loads address the workspace and the
+    // spectrum table with 31-bit byte offsets
+    static bool conv_rows_ok(long long L1, long long n2, size_t ws)
+    {
+        if (env_ll("IMPULSE_FFT_NO_CONV_ROWS", 0)) return false;
+        return pow2_supported((int)L1) && L1 >= 64 && ws <= (size_t)0x7fffffff && n2 * 16 <= 0x7fffffffLL;
+    }
+
+    // the kernel spectrum in row order of the four-step split n2 = L1 L2:
+    // entry k L1 + k' holds spectrum bin k + L2 k' (k < L2, k' < L1)
+    const double2 *spectrum_rows(const double2 *spec, long long L1, long long L2)
+    {
+        const long long n2 = L1 * L2;
+        std::vector<double2> nat((size_t)n2), rows((size_t)n2);
+        if (hipMemcpy(nat.data(), spec, (size_t)n2 * sizeof(double2), hipMemcpyDeviceToHost) != hipSuccess) {
+            (void)hipGetLastError();
+            err = {E_HIP, "hipMemcpy of the Bluestein spectrum failed"};
+            return nullptr;
+        }
+        for (long long k = 0; k < L2; ++k)
+            for (long long kp = 0; kp < L1; ++kp) rows[(size_t)(k * L1 + kp)] = nat[(size_t)(k + L2 * kp)];
+        auto b = upload(rows);
+        return b ? (const double2 *)b->p : nullptr;
+    }
+
+    bool add_conv_rows(FftParams P, BufRef src, BufRef dst)
+    {
+        const int map = 0;
+        const long long tb = pow2_tb_host(P.N, map);
+        const long long per = (long long)pow2_lds_bytes(P.N, map);
+        const long long cap = max_lds_bytes() - (long long)sizeof(long long) * 8 * kMaxTB - 1024;
+        if (tb * per > cap) {
+            err = {E_UNSUPPORTED, "transform does not fit in LDS"};
+            return false;
+        }
+        P.TB = (int)tb;
+        P.NT = P.N / 16;
+        P.col_load = P.col_store = 0;
+        Pass ps;
+        ps.kind = Pass::FFT_POW2;
+        ps.map = map;
+        ps.fp = P;
+        ps.block = (unsigned)(tb * P.NT);
+        const long long grid = (P.ntf + tb - 1) / tb;
+        if (grid > INT_MAX) {
+            err = {E_UNSUPPORTED, "batch too large for one launch"};
+            return false;
+        }
+        ps.grid = (unsigned)std::max<long long>(1, grid);
+        ps.lds = (size_t)(per * tb);
+        ps.src = src;
+        ps.dst = dst;
+        ps.uses_fct = false;
+        ps.name = "k_fft_pow2";
+        ps.bytes_in = side_bytes(P, true) + (size_t)(P.ntf * P.N * 16);   // rows and the spectrum table
+        ps.bytes_out = side_bytes(P, false);
+        plan.passes.push_back(ps);
         return true;
     }
 

@@ -323,6 +323,52 @@ def test_r2c_fused_real_post(n):
     assert rel_l2(yf, yu) < 1e-15
 
 
+@pytest.mark.parametrize("n", [16381, 24573, 32749, 65521, 131071])
+def test_bluestein_three_pass(n):
+    """Four-step Bluestein lengths whose padded length splits into power-of-two rows (16381 -> 256 x 128,
+    24573 = 3 * 8191 -> 256 x 192, 32749, 65521, 131071): the row pass transforms, multiplies by the
+    kernel spectrum and transforms back in one launch.  c2c of both signs, r2c and c2r of odd (paired)
+    and even (half-length) signals against the oracle, and against the four-pass plan."""
+    import os
+    torch = _torch()
+    print(f"case three-pass Bluestein n={n}", flush=True)
+    sp = torch.cuda.current_stream().cuda_stream
+    batch = 3
+
+    def run(kind, shape, xin, out_shape, out_dtype, fwd, fct):
+        xd = _dev(xin)
+        yd = torch.empty(out_shape, dtype=out_dtype, device="cuda")
+        st_in = [s * xd.element_size() for s in xd.stride()]
+        st_out = [s * yd.element_size() for s in yd.stride()]
+        F.Plan(kind, shape, st_in, st_out, [1], fwd).execute(xd.data_ptr(), yd.data_ptr(), fct, sp)
+        torch.cuda.synchronize()
+        return yd.cpu().numpy()
+
+    def both(kind, shape, xin, out_shape, out_dtype, fwd, fct):
+        y3 = run(kind, shape, xin, out_shape, out_dtype, fwd, fct)
+        os.environ["IMPULSE_FFT_NO_CONV_ROWS"] = "1"
+        try:
+            y4 = run(kind, shape, xin, out_shape, out_dtype, fwd, fct)
+        finally:
+            del os.environ["IMPULSE_FFT_NO_CONV_ROWS"]
+        assert rel_l2(y3, y4) < 1e-13, (kind, n, fwd)
+        return y3
+
+    x = synth.complex_normal((batch, n), seed=n)
+    for fwd in (True, False):
+        y = both(_lib.C2C, [batch, n], x, (batch, n), torch.complex128, fwd, 0.5)
+        assert rel_l2(y, oracle.c2c(x, [1], fwd, 0.5)) < TOL, ("c2c", n, fwd)
+    for m in (n, 2 * n):                      # odd length (row pairs) and even length (half-length signal)
+        r = synth.uniform((batch, m), -1, 1, seed=m + 1)
+        for fwd in (True, False):
+            y = both(_lib.R2C, [batch, m], r, (batch, m // 2 + 1), torch.complex128, fwd, 1.0)
+            assert rel_l2(y, oracle.r2c(r, [1], fwd)) < TOL, ("r2c", m, fwd)
+        h = synth.complex_normal((batch, m // 2 + 1), seed=m + 2)
+        for fwd in (False, True):
+            y = both(_lib.C2R, [batch, m], h, (batch, m), torch.float64, fwd, 1.0 / m)
+            assert rel_l2(y, oracle.c2r(h, m, [1], fwd, 1.0 / m)) < TOL, ("c2r", m, fwd)
+
+
 # ---------------------------------------------------------------------------
 # error behaviour (pocketfft_hdronly.h:446-477 cases, forced from the host)
 # ---------------------------------------------------------------------------

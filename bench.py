@@ -13,7 +13,9 @@ with HIP events on the execution stream inside the timed region, and its
 algorithmic bytes per launch are the bytes each launch must read plus write
 (input read once, output written once).  cpu_baseline times the reference
 PocketFFT C++ (built from the reference sources into oracle/_ref) on a bounded
-sample of the same workload on this host's cores.
+sample of the same workload on this host's cores.  vendor_fft times torch.fft
+(hipFFT / rocFFT on ROCm) on the same device inputs as a black-box comparator;
+it is reported beside the result and never enters `value`.
 """
 import argparse
 import ctypes
@@ -44,6 +46,8 @@ def parse():
     ap.add_argument("--only-forms", default="",
                     help="comma list of forms to run alone (ragged,mfma,conv,dct,upfirdn,fftpack), no drop-in step")
     ap.add_argument("--form-reps", type=int, default=10)
+    ap.add_argument("--no-vendor", action="store_true",
+                    help="skip timing torch.fft (hipFFT/rocFFT) on the same inputs as a comparator")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (the reference "
                          "reading for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -261,6 +265,11 @@ def mfma_form(torch, reps):
                      "speedup_vs_fp64_dropin": round(ms64 / ms, 2)}
         del plan, y
         torch.cuda.empty_cache()
+    # torch.fft (hipFFT / rocFFT) on the same complex64 values, as a black-box comparator
+    xv = x.view(B, L)
+    msv = timed(lambda: torch.fft.fft(xv, dim=1))
+    out["vendor_fft_c64"] = {"library": "torch.fft.fft (hipFFT/rocFFT), black-box comparator", "ms": round(msv, 3),
+                             "f32_form_speedup": round(msv / out["f32"]["ms"], 2)}
     return out
 
 
@@ -409,6 +418,33 @@ def upfirdn_form(torch, reps):
     return out
 
 
+def vendor_fft(torch, plist, bufs, ms_per_step, reps):
+    """torch.fft (hipFFT / rocFFT on this ROCm build) on the same device inputs
+    and output shapes, timed with events: a black-box comparator, never part of
+    `value`."""
+    from impulse_amd import _lib
+    rows, total = [], 0.0
+    for p, (x, _) in zip(plist, bufs):
+        if p.kind == _lib.C2C:
+            fn = lambda x=x, p=p: torch.fft.fftn(x, dim=tuple(p.axes))
+        else:
+            fn = lambda x=x, p=p: torch.fft.rfft(x, dim=p.axes[0])
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        total += ms
+        rows.append({"part": p.name, "ms": round(ms, 4)})
+    return {"library": "torch.fft (hipFFT/rocFFT), black-box comparator on the same device inputs",
+            "parts": rows, "step_ms": round(total, 4), "this_engine_speedup": round(total / ms_per_step, 2)}
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -516,6 +552,7 @@ def main():
     forms = [] if args.no_forms else [ragged_form(torch, args.form_reps), mfma_form(torch, args.form_reps),
                                        conv_form(torch, args.form_reps), dct_form(torch, args.form_reps),
                                        upfirdn_form(torch, args.form_reps), fftpack_form(torch, args.form_reps)]
+    vendor = None if args.no_vendor else vendor_fft(torch, plist, bufs, ms_per_step, args.steps)
     cpu = None if args.no_cpu else cpu_baseline(plist, args.cpu_seconds)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
@@ -524,6 +561,7 @@ def main():
         "config": {"workload": "FP64 drop-in at scale: c2c c128 4096x4096 axes [0,1] + r2c f64 2^22 x16 + "
                                "r2c f64 1000003 x4", "parts": [p.name for p in plist]},
         "roofline": roofline, "cpu_baseline": cpu, "parts": part_rows, "launches": launches, "forms": forms,
+        "vendor_fft": vendor,
     }
     print(json.dumps(out))
 

@@ -54,6 +54,19 @@ constexpr int kRow = 273;           // LDS row stride (elements): 256 + 256/16 p
 // segments in the strided passes), 16 for FP64 data (LDS budget)
 template <int VAR>
 constexpr int tile_of() { return VAR == 0 ? 32 : 16; }
+// threads per workgroup: eight wavefronts; a wavefront computes TILE / 8 transforms
+// (four for FP32 data, two for FP64 data, so that the 78 KB FP64 tile still lets
+// two workgroups, four wavefronts per SIMD, share a CU)
+#ifndef IMPULSE_MFMA_BX_THREADS
+#define IMPULSE_MFMA_BX_THREADS 512
+#endif
+#ifndef IMPULSE_MFMA_BX_WPE
+#define IMPULSE_MFMA_BX_WPE 4
+#endif
+template <int VAR>
+constexpr int threads_of() { return VAR == 0 ? 512 : IMPULSE_MFMA_BX_THREADS; }
+template <int VAR>
+constexpr int wpe_of() { return VAR == 0 ? 4 : IMPULSE_MFMA_BX_WPE; }
 
 struct MfmaParams {
     const void *src;
@@ -258,9 +271,12 @@ __device__ __forceinline__ void bx_stage(double2 (&a)[4], const BxConsts &C)
 // ---------------------------------------------------------------------------
 
 template <int VAR, int PASS>
-__global__ void __launch_bounds__(16 * tile_of<VAR>()) k_mfma256(const MfmaParams P)
+__global__ void __launch_bounds__(threads_of<VAR>()) __attribute__((amdgpu_waves_per_eu(wpe_of<VAR>())))
+k_mfma256(const MfmaParams P)
 {
-    constexpr int TILE = tile_of<VAR>(), NT = 16 * TILE, TPS = 256 / TILE;   // threads, tiles per signal
+    constexpr int TILE = tile_of<VAR>(), NT = threads_of<VAR>(), TPS = 256 / TILE;   // threads, tiles per signal
+    constexpr int PER = TILE / (NT / 64);   // transforms per wavefront
+    static_assert(PER * (NT / 64) == TILE, "the tile splits evenly over the wavefronts");
     using E = typename Elem<VAR>::T;
     __shared__ E lds[TILE * kRow];
     __shared__ double2 tws[512];   // W_256^m and W_65536^m (m < 256), forward sign
@@ -315,23 +331,25 @@ __global__ void __launch_bounds__(16 * tile_of<VAR>()) k_mfma256(const MfmaParam
         }
     }
     __syncthreads();
-    // ---- compute: wave w owns transforms 4w .. 4w+3 ----
+    // ---- compute: wave w owns transforms PER w .. PER w + PER - 1 ----
     const int l = tid & 63, w = tid >> 6, g = l >> 4, j = l & 15;
     const double sign = P.sign;
     const bool last = PASS != MP_COL;
     // W_256^(b k1) for this lane's four (b, k1) pairs; the four-step twiddle
     // W_65536^(t k) = W_65536^(t k mod 256) W_256^(t k / 256), both factors from LDS, product in FP64
     double2 w1[4];
+    if constexpr (VAR == MF_F32) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) w1[r] = tw_sign(tws[(4 * g + r) * j], sign);
+        for (int r = 0; r < 4; ++r) w1[r] = tw_sign(tws[(4 * g + r) * j], sign);
+    }
     auto step_tw = [&](int t, int k) -> double2 {
         const int m = t * k;   // < 65536
         return tw_sign(cmul(tws[256 + (m & 255)], tws[m >> 8]), sign);
     };
     if constexpr (VAR == MF_F32) {
         const F32Consts C = f32_consts(sign);
-        for (int u = 0; u < 4; ++u) {
-            const int c = 4 * w + u;
+        for (int u = 0; u < PER; ++u) {
+            const int c = PER * w + u;
             float2 a[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) a[r] = lds[c * kRow + lidx(j + 16 * (4 * g + r))];
@@ -350,14 +368,15 @@ __global__ void __launch_bounds__(16 * tile_of<VAR>()) k_mfma256(const MfmaParam
         }
     } else {
         const BxConsts C = bx_consts(sign);
-        for (int u = 0; u < 4; ++u) {
-            const int c = 4 * w + u;
+        for (int u = 0; u < PER; ++u) {
+            const int c = PER * w + u;
             double2 a[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) a[r] = lds[c * kRow + lidx(j + 16 * (4 * g + r))];
             bx_stage(a, C);
+            // W_256^(b k1) from the LDS table at use (keeps 16 registers free)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) a[r] = cmul(a[r], w1[r]);
+            for (int r = 0; r < 4; ++r) a[r] = cmul(a[r], tw_sign(tws[(4 * g + r) * j], sign));
             bx_stage(a, C);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -466,7 +485,7 @@ static hipError_t launch_var(const MfmaParams &P, int pass, long long batch, hip
     constexpr int TILE = tile_of<VAR>();
     const long long grid = pass == MP_SINGLE ? (batch + TILE - 1) / TILE : batch * (256 / TILE);
     if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
-    const dim3 g((unsigned)grid), b(16 * TILE);
+    const dim3 g((unsigned)grid), b(threads_of<VAR>());
     if (pass == MP_COL) hipLaunchKernelGGL((k_mfma256<VAR, MP_COL>), g, b, 0, s, P);
     else if (pass == MP_ROW) hipLaunchKernelGGL((k_mfma256<VAR, MP_ROW>), g, b, 0, s, P);
     else hipLaunchKernelGGL((k_mfma256<VAR, MP_SINGLE>), g, b, 0, s, P);

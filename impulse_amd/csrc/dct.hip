@@ -32,7 +32,10 @@
 //   DCT-IV   z[n] = x[n] e^{-i pi (2n+1)/4N}      forward   y[k] = 2 Re(e^{-i pi k/2N} Z[k])
 //   DST-IV   as DCT-IV                            forward   y[k] = -2 Im(e^{-i pi k/2N} Z[k])
 // (each follows from writing the cosine / sine as the real / imaginary part of
-// one exponential and collecting the DFT kernel).  Twiddles come from FP64
+// one exponential and collecting the DFT kernel).  The half-length forms take
+// their pre / post twiddles, and the fused kernel its LDS-stage roots, from
+// per-axis tables written by the long-double host generator (one length per
+// axis, so one table serves every line); the full-length forms use FP64
 // sincospi.
 //
 // Workspace layout per axis: [lines, L] when the axis is the input's unit-
@@ -68,6 +71,11 @@ struct R2rParams {
     int half;                  // types 2-4 at even N: L = N/2 points per line
     int col;                   // workspace [L, lines] (line fastest) instead of [lines, L]
     double fct;
+    // half-length lines: the twiddles of the pre / post steps, L + 1 entries each
+    // (types 2, 3: e1[k] = exp(i pi 2k / N), e2[k] = exp(i pi k / 2N);
+    //  type 4: e1[k] = exp(i pi k / N), e2[k] = exp(i pi (4k + 1) / 4N)),
+    // and W_L^j (j < L) for the fused kernel's LDS stages
+    const double2 *e1, *e2, *ftw;
 };
 
 // (line, point) of work item e and the workspace index of that point
@@ -132,14 +140,13 @@ __device__ __forceinline__ double2 half_pre(const R2rParams &P, const double *x,
         return make_double2(v(2 * m), v(2 * m + 1));
     } else if constexpr (TYPE == 3) {
         auto xe = [&](long long k) { return k < N ? r2r_xp<3>(P, x, k) : 0.0; };
-        auto V = [&](long long k) { return cmul(expipi((double)k, 2.0 * N), make_double2(xe(k), -xe(N - k))); };
+        auto V = [&](long long k) { return cmul(P.e2[k], make_double2(xe(k), -xe(N - k))); };
         const double2 vk = V(m), vm = cconj(V(M - m));
         const double2 sum = cadd(vk, vm);
-        const double2 dif = cmul(expipi(2.0 * m, (double)N), csub(vk, vm));
+        const double2 dif = cmul(P.e1[m], csub(vk, vm));
         return make_double2(sum.x - dif.y, sum.y + dif.x);
     } else {
-        return cmul(make_double2(r2r_xp<4>(P, x, 2 * m), r2r_xp<4>(P, x, N - 1 - 2 * m)),
-                    cconj(expipi((double)m, (double)N)));
+        return cmul(make_double2(r2r_xp<4>(P, x, 2 * m), r2r_xp<4>(P, x, N - 1 - 2 * m)), cconj(P.e1[m]));
     }
 }
 
@@ -229,8 +236,8 @@ __device__ __forceinline__ void half_post(const R2rParams &P, double *yl, long l
         const double2 ev = cscale(cadd(zk, zm), 0.5);
         const double2 d = csub(zk, zm);
         const double2 od = make_double2(0.5 * d.y, -0.5 * d.x);
-        const double2 V = cadd(ev, cmul(cconj(expipi(2.0 * k, (double)N)), od));
-        const double2 W = cmul(cconj(expipi((double)k, 2.0 * N)), V);
+        const double2 V = cadd(ev, cmul(cconj(P.e1[k]), od));
+        const double2 W = cmul(cconj(P.e2[k]), V);
         const long long oi = P.cosine ? 0 : N - 1;   // the ortho end point as the DCT indexes it
         double lo = 2.0 * W.x;
         if (P.ortho && k == oi) lo *= 0.5 * kSqrt2;
@@ -246,7 +253,7 @@ __device__ __forceinline__ void half_post(const R2rParams &P, double *yl, long l
         r2r_put<3>(P, yl, sigma(2 * k), z.x);
         r2r_put<3>(P, yl, sigma(2 * k + 1), z.y);
     } else {
-        const double2 S = cmul(Zat(k), cconj(expipi((double)(4 * k + 1), 4.0 * N)));
+        const double2 S = cmul(Zat(k), cconj(P.e2[k]));
         r2r_put<4>(P, yl, 2 * k, 2.0 * S.x);
         r2r_put<4>(P, yl, N - 1 - 2 * k, -2.0 * S.y);
     }
@@ -334,7 +341,7 @@ k_r2r_fused(const R2rParams P, int count)
             for (int m = i; m < M; m += NT) zl[lpad(t * M + m)] = half_pre<TYPE>(P, x, m);
         }
         __syncthreads();
-        lds_fft(zl, M, count, s, 0);   // ends with a barrier
+        lds_fft(zl, M, count, s, 0, TwTable{P.ftw, M});   // ends with a barrier
         if (valid) {
             double *yl = P.dst + line_off(P.out, line);
             auto Zat = [&](long long m) { return zl[lpad(t * M + (int)m)]; };
@@ -388,6 +395,7 @@ struct R2r {
         long long N, L, lines;
         int forward, half, col;
         int fused;                        // one LDS kernel per line batch (no sub-plan)
+        std::shared_ptr<DevBuf> tab;      // half lines: e1 | e2 (L + 1 each) | W_L^j (fused, L)
         LineMap in_first, in_next, out;
         long long sa_in_first, sa_in_next, sa_out;
         std::unique_ptr<Plan> fft;
@@ -408,6 +416,33 @@ static LineMap line_map(const std::vector<size_t> &shape, const std::vector<ptrd
         ++m.nd;
     }
     return m;
+}
+
+// the twiddle tables of a half-length axis, from the long-double host generator
+static int half_tables(R2r::Axis &A, int type, Error &err)
+{
+    const long long N = A.N, L = A.L;
+    std::vector<double2> t((size_t)(2 * (L + 1) + (A.fused ? L : 0)));
+    auto cj = [](double2 w) { return make_double2(w.x, -w.y); };   // exp(+i..) from W = exp(-i..)
+    for (long long k = 0; k <= L; ++k) {
+        if (type == 4) {
+            t[(size_t)k] = cj(twiddle_at((uint64_t)k, (uint64_t)(2 * N)));
+            t[(size_t)(L + 1 + k)] = cj(twiddle_at((uint64_t)(4 * k + 1), (uint64_t)(8 * N)));
+        } else {
+            t[(size_t)k] = cj(twiddle_at((uint64_t)k, (uint64_t)N));
+            t[(size_t)(L + 1 + k)] = cj(twiddle_at((uint64_t)k, (uint64_t)(4 * N)));
+        }
+    }
+    if (A.fused)
+        for (long long j = 0; j < L; ++j) t[(size_t)(2 * (L + 1) + j)] = twiddle_at((uint64_t)j, (uint64_t)L);
+    A.tab = std::make_shared<DevBuf>();
+    if (A.tab->alloc(t.size() * sizeof(double2))) { err = {E_NOMEM, "DCT/DST table allocation failed"}; return E_NOMEM; }
+    if (hipMemcpy(A.tab->p, t.data(), t.size() * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        err = {E_HIP, "hipMemcpy of a DCT/DST table failed"};
+        return E_HIP;
+    }
+    return OK;
 }
 
 int r2r_plan_create(std::unique_ptr<Plan> &out, int kind, int type, int ortho, int dtype, size_t ndim,
@@ -479,6 +514,7 @@ int r2r_plan_create(std::unique_ptr<Plan> &out, int kind, int type, int ortho, i
         A.col = A.lines > 1 && std::llabs(a == 0 ? A.sa_in_first : A.sa_in_next) != 1;
         // lines along the unit-stride axis, N/2 13-smooth and within the LDS budget: one fused kernel
         A.fused = A.half && !A.col && A.L <= kR2rFusedMax && smooth13_r2r(A.L);
+        if (A.half && (rc = half_tables(A, type, err)) != OK) return rc;
         if (A.fused) {
             R->ax.push_back(std::move(A));
             continue;
@@ -528,6 +564,10 @@ int r2r_execute(Plan &P, const void *in, void *out, double fct, hipStream_t s, E
         Q.half = A.half;
         Q.col = A.col;
         Q.fct = a == 0 ? fct : 1.0;
+        const double2 *tb = A.tab ? (const double2 *)A.tab->p : nullptr;
+        Q.e1 = tb;
+        Q.e2 = tb ? tb + (A.L + 1) : nullptr;
+        Q.ftw = tb && A.fused ? tb + 2 * (A.L + 1) : nullptr;
         if (A.fused) {
             const int count = fused_count(A.L);
             const long long groups = (A.lines + count - 1) / count;

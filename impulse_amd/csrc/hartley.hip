@@ -57,6 +57,7 @@ struct RfpParams {
     RfpLines lin, lout;
     long long sa_in, sa_out;
     long long N, H, lines;
+    const double2 *ftw;   // fused lines: W_M^j (j < M = N/2), then W_N^k (k <= M/2)
 };
 
 // positions 2, 4, ... (the halfcomplex imaginary slots) of every line negated:
@@ -204,15 +205,13 @@ k_rfp_fused(const RfpParams P, int count, int flip, double fct)
             }
         }
         __syncthreads();
-        lds_fft(zl, M, count, -1.0, 0);   // ends with a barrier
+        lds_fft(zl, M, count, -1.0, 0, TwTable{P.ftw, M});   // ends with a barrier
         if (valid) {
             double *y = P.rdst + rfp_line_off(P.lout, line);
             const long long sa = P.sa_out;
             for (int k = i; 2 * k <= M; k += NT) {
                 const double2 zk = zl[lpad(t * M + k)], zm = zl[lpad(t * M + (k == 0 ? 0 : M - k))];
-                double sn, cs;
-                sincospi(-(double)k / (double)M, &sn, &cs);   // W_N^k
-                const double2 w = make_double2(cs, sn);
+                const double2 w = P.ftw[M + k];   // W_N^k
                 const double2 e1 = cscale(cadd(zk, cconj(zm)), 0.5), d1 = csub(zk, cconj(zm));
                 const double2 o1 = make_double2(0.5 * d1.y, -0.5 * d1.x);
                 const double2 e2 = cscale(cadd(zm, cconj(zk)), 0.5), d2 = csub(zm, cconj(zk));
@@ -287,6 +286,7 @@ struct Rfp {
         std::unique_ptr<Plan> plan;        // r2c (to the workspace) or c2r (from it)
         bool negcopy = false;              // forward r2r_fftpack without real2hermitian
         bool fused = false;                // one LDS kernel per line batch (no sub-plan)
+        std::shared_ptr<DevBuf> tab;       // fused: W_M^j (j < M), then W_N^k (k <= M/2)
     };
     std::vector<Axis> ax;
     std::unique_ptr<Plan> gen;             // genuine: n-D r2c into the workspace
@@ -416,6 +416,20 @@ int rfp_plan_create(std::unique_ptr<Plan> &out, int kind, int r2h, int forward, 
                       (sa_src == 1 || A.lines == 1);
             if (A.fused) {
                 A.negcopy = false;
+                const long long M = A.N / 2;
+                std::vector<double2> t((size_t)(M + M / 2 + 1));
+                for (long long j = 0; j < M; ++j) t[(size_t)j] = twiddle_at((uint64_t)j, (uint64_t)M);
+                for (long long k = 0; k <= M / 2; ++k) t[(size_t)(M + k)] = twiddle_at((uint64_t)k, (uint64_t)A.N);
+                A.tab = std::make_shared<DevBuf>();
+                if (A.tab->alloc(t.size() * sizeof(double2))) {
+                    err = {E_NOMEM, "r2r_fftpack table allocation failed"};
+                    return E_NOMEM;
+                }
+                if (hipMemcpy(A.tab->p, t.data(), t.size() * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) {
+                    (void)hipGetLastError();
+                    err = {E_HIP, "hipMemcpy of an r2r_fftpack table failed"};
+                    return E_HIP;
+                }
                 R->ax.push_back(std::move(A));
                 continue;
             }
@@ -521,6 +535,7 @@ int rfp_execute(Plan &P, const void *in, void *out, double fct, hipStream_t s, E
             Q.lin = a == 0 ? A.lin_first : A.lin_next;
             Q.sa_in = a == 0 ? A.sa_in_first : A.sa_in_next;
             Q.rdst = (double *)out;
+            Q.ftw = (const double2 *)A.tab->p;
             const int mode = R.kind == 7 ? 0 : 1;
             const int flip = R.kind == 7 && R.forward && !R.r2h;
             hipError_t e = rfp_fused_threads(M) == 256 ? launch_rfp_fused<256>(mode, Q, grid, lds, count, flip, f, s)

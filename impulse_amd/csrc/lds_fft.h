@@ -1,6 +1,9 @@
 // A mixed-radix Stockham FFT of runtime length on data in LDS, run by the
-// threads of a workgroup: radices 2, 3, 4, 5, 7, 8, 11, 13, 16, FP64, twiddles
-// from sincospi.  Shared by the ragged-batch kernel and the fused DCT kernel.
+// threads of a workgroup: radices 2, 3, 4, 5, 7, 8, 11, 13, 16, FP64.  The
+// twiddles come from sincospi (TwSincos: the ragged-batch kernel, whose
+// signals all have different lengths) or from a table W_L^j, j < L, of the
+// transform length (TwTable: the fused DCT and r2r_fftpack kernels, whose
+// lines share one length).
 #pragma once
 
 #include "butterflies.h"
@@ -20,23 +23,40 @@ __device__ __forceinline__ double2 twid32(int k, int L, double sign)
     return make_double2(c, sign * s);
 }
 
+struct TwSincos {
+    // exp(sign 2 pi i k / n)
+    __device__ __forceinline__ double2 operator()(int k, int n, double sign) const { return twid32(k, n, sign); }
+};
+
+struct TwTable {
+    const double2 *tab;   // W_L^j = exp(-2 pi i j / L), j < L (forward sign)
+    int L;
+    // exp(sign 2 pi i k / n) = W_L^(k L / n), conjugated for the backward sign; n divides L
+    __device__ __forceinline__ double2 operator()(int k, int n, double sign) const
+    {
+        const double2 w = tab[k * (L / n)];
+        return make_double2(w.x, -sign * w.y);
+    }
+};
+
 // LDS holds transform data at padded positions: one word skipped every 16, so
 // that the power-of-two strides of the stages (16 b, 16 p) spread over the banks.
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
 constexpr int lds_points_padded(int n) { return n + n / 16 + 1; }
 
 // Stockham stage on one transform in LDS: butterflies b = i, i + NT, ...
-// The twiddle roots are generated before the values are loaded, so the
-// sincospi evaluation does not hold the R NB values live.
-template <int R, int NB>
-__device__ __forceinline__ void rstage_body(double2 *lds, int base, int N, int p, int i, int NT, double s)
+// The twiddle roots are generated (or their table loads issued) before the
+// values are loaded, so that work does not hold the R NB values live.
+template <int R, int NB, typename TW>
+__device__ __forceinline__ void rstage_body(double2 *lds, int base, int N, int p, int i, int NT, double s,
+                                            const TW &tw)
 {
     const int nb = N / R;
     double2 w1[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
         const int k = (i + c * NT) % p;
-        w1[c] = k ? twid32(k, p * R, s) : make_double2(1.0, 0.0);
+        w1[c] = k ? tw(k, p * R, s) : make_double2(1.0, 0.0);
     }
     double2 v[NB][R];
 #pragma unroll
@@ -71,10 +91,10 @@ __device__ __forceinline__ void rstage_body(double2 *lds, int base, int N, int p
 
 // the stage as a call of its own (one copy per radix in the ragged kernel, which keeps its
 // register budget to the largest stage)
-template <int R, int NB>
-__device__ __noinline__ void rstage(double2 *lds, int base, int N, int p, int i, int NT, double s)
+template <int R, int NB, typename TW>
+__device__ __noinline__ void rstage(double2 *lds, int base, int N, int p, int i, int NT, double s, TW tw)
 {
-    rstage_body<R, NB>(lds, base, N, p, i, NT, s);
+    rstage_body<R, NB>(lds, base, N, p, i, NT, s, tw);
 }
 
 // Radix sequence of a 13-smooth L as 4-bit codes (stage 0 in the low bits),
@@ -105,7 +125,8 @@ __device__ __forceinline__ unsigned long long radix_codes(int L, int order, int 
 
 // `count` transforms of length L at lds + t L, blockDim.x / count threads each
 // (count divides blockDim.x).  Ends with a barrier.
-__device__ __noinline__ void lds_fft(double2 *lds, int L, int count, double s, int order)
+template <typename TW>
+__device__ __noinline__ void lds_fft(double2 *lds, int L, int count, double s, int order, TW tw)
 {
     int ns;
     unsigned long long codes = radix_codes(L, order, ns);
@@ -115,15 +136,15 @@ __device__ __noinline__ void lds_fft(double2 *lds, int L, int count, double s, i
     int p = 1;
     for (int st = 0; st < ns; ++st, codes >>= 4) {
         switch ((int)(codes & 15)) {
-            case 1: rstage<2, nb_rg(2)>(lds, base, L, p, i, NT, s); p *= 2; break;
-            case 2: rstage<3, nb_rg(3)>(lds, base, L, p, i, NT, s); p *= 3; break;
-            case 3: rstage<4, nb_rg(4)>(lds, base, L, p, i, NT, s); p *= 4; break;
-            case 4: rstage<5, nb_rg(5)>(lds, base, L, p, i, NT, s); p *= 5; break;
-            case 5: rstage<7, nb_rg(7)>(lds, base, L, p, i, NT, s); p *= 7; break;
-            case 6: rstage<8, nb_rg(8)>(lds, base, L, p, i, NT, s); p *= 8; break;
-            case 7: rstage<11, nb_rg(11)>(lds, base, L, p, i, NT, s); p *= 11; break;
-            case 8: rstage<13, nb_rg(13)>(lds, base, L, p, i, NT, s); p *= 13; break;
-            default: rstage<16, nb_rg(16)>(lds, base, L, p, i, NT, s); p *= 16; break;
+            case 1: rstage<2, nb_rg(2)>(lds, base, L, p, i, NT, s, tw); p *= 2; break;
+            case 2: rstage<3, nb_rg(3)>(lds, base, L, p, i, NT, s, tw); p *= 3; break;
+            case 3: rstage<4, nb_rg(4)>(lds, base, L, p, i, NT, s, tw); p *= 4; break;
+            case 4: rstage<5, nb_rg(5)>(lds, base, L, p, i, NT, s, tw); p *= 5; break;
+            case 5: rstage<7, nb_rg(7)>(lds, base, L, p, i, NT, s, tw); p *= 7; break;
+            case 6: rstage<8, nb_rg(8)>(lds, base, L, p, i, NT, s, tw); p *= 8; break;
+            case 7: rstage<11, nb_rg(11)>(lds, base, L, p, i, NT, s, tw); p *= 11; break;
+            case 8: rstage<13, nb_rg(13)>(lds, base, L, p, i, NT, s, tw); p *= 13; break;
+            default: rstage<16, nb_rg(16)>(lds, base, L, p, i, NT, s, tw); p *= 16; break;
         }
     }
     if (ns == 0) __syncthreads();

@@ -187,7 +187,7 @@ __device__ void col_pass(LOAD load, const RaggedTask &T, double s, double2 *lds,
             lds[lpad(t * L2 + n2)] = t < cnt ? load((long long)(c0 + t) + (long long)L1 * n2) : make_double2(0.0, 0.0);
         }
         __syncthreads();
-        lds_fft(lds, L2, ca, s, order);
+        lds_fft(lds, L2, ca, s, order, TwSincos{});
         {
             // t = tid % ca is fixed per thread (ca divides nthr); k2 advances by nthr / ca
             const int t = tid % ca, step = nthr / ca;
@@ -218,12 +218,12 @@ __device__ void row_pass(const RaggedTask &T, double s, double2 *lds, double2 *t
         double2 *row = tmp + (long long)r0 * L1;
         for (int e = tid; e < cb * L1; e += nthr) lds[lpad(e)] = e < cnt * L1 ? row[e] : make_double2(0.0, 0.0);
         __syncthreads();
-        lds_fft(lds, L1, cb, s, order);
+        lds_fft(lds, L1, cb, s, order, TwSincos{});
         if (spec) {
             const double2 *sp = spec + (long long)r0 * L1;
             for (int e = tid; e < cnt * L1; e += nthr) lds[lpad(e)] = cmul(lds[lpad(e)], sp[e]);
             __syncthreads();
-            lds_fft(lds, L1, cb, -s, order);
+            lds_fft(lds, L1, cb, -s, order, TwSincos{});
             for (int e = tid; e < cnt * L1; e += nthr) {
                 const int t = e / L1, a = e - t * L1;
                 row[e] = cmul(lds[lpad(e)], twid((long long)a * (r0 + t), L, -s));
@@ -250,7 +250,7 @@ __device__ void col_pass_out(EMIT emit, const RaggedTask &T, double s, double2 *
             lds[lpad(t * L2 + k2)] = t < cnt ? tmp[(long long)(c0 + t) + (long long)L1 * k2] : make_double2(0.0, 0.0);
         }
         __syncthreads();
-        lds_fft(lds, L2, ca, s, order);
+        lds_fft(lds, L2, ca, s, order, TwSincos{});
         for (int e = tid; e < ca * L2; e += nthr) {
             const int t = e % ca, b = e / ca;
             if (t < cnt) emit((long long)(c0 + t) + (long long)L1 * b, lds[lpad(t * L2 + b)]);
@@ -327,7 +327,7 @@ __device__ bool wg_fft(LOAD load, const RaggedTask &T, double s, double2 *lds, d
     if (T.L1 == 0) {
         for (long long m = tid; m < L; m += nthr) lds[lpad((int)m)] = load(m);
         __syncthreads();
-        lds_fft(lds, (int)L, 1, s, order);
+        lds_fft(lds, (int)L, 1, s, order, TwSincos{});
         return true;
     }
     col_pass(load, T, s, lds, tmp, order);
@@ -410,7 +410,7 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
                 wg_fft(aload, T, -1.0, lds, S0, nullptr, P.order);
                 for (long long m = tid; m < n2; m += nthr) lds[lpad((int)m)] = cmul(lds[lpad((int)m)], B[m]);
                 __syncthreads();
-                lds_fft(lds, (int)n2, 1, 1.0, P.order);
+                lds_fft(lds, (int)n2, 1, 1.0, P.order, TwSincos{});
             } else {
                 col_pass(aload, T, -1.0, lds, S0, P.order);
                 row_pass(T, -1.0, lds, S0, P.order, B);

@@ -16,6 +16,7 @@ namespace ifft {
 
 // launchers (fft_lds.hip)
 hipError_t launch_fft_lds(const FftParams &P, unsigned grid, unsigned block, size_t lds_bytes, hipStream_t s);
+int fft_lds_wpe(const FftParams &P);   // waves per SIMD of the generic kernel instantiation a launch takes
 hipError_t launch_real_post(const RealPostParams &P, hipStream_t s);
 hipError_t launch_pack_halfcomplex(const double2 *bins, double *hc, long long n, hipStream_t s);
 hipError_t launch_unpack_halfcomplex(const double *hc, double2 *bins, long long n, hipStream_t s);

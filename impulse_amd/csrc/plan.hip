@@ -1226,7 +1226,7 @@ int plan_execute(Plan &P, const void *in, void *out, double fct, hipStream_t str
                            std::to_string(pow2_io(fp)) + ">";
             } else {
                 e = launch_fft_lds(fp, ps.grid, ps.block, ps.lds, stream);
-                ps.kname = "k_fft_lds";
+                ps.kname = "k_fft_lds<" + std::to_string(fft_lds_wpe(fp)) + ">";
             }
         } else {
             RealPostParams rp = ps.rp;

@@ -21,11 +21,8 @@ namespace ifft {
 // Each radix's stage is a call of its own, so the kernel's register budget is
 // that of its largest stage rather than of all nine inlined into one switch
 // (210 registers, two waves per SIMD).  The kernel is built for three waves
-// per SIMD (168 registers) and four (128 registers).  Over 16 x 2112^2 complex
-// rows of plain c2c take 1.64 -> 1.07 ms at three and 1.29 ms at four; the
-// columns take 1.83 -> 1.50 ms at four, and the 2-D r2c, whose rows carry the
-// real-half load and store, 2.07 -> 1.70 ms with both passes at four.  The
-// launcher takes three waves for plain c2c passes along rows, four otherwise.
+// per SIMD (168 registers) and four (128 registers); fft_lds_wpe picks one per
+// launch.
 template <int R, int NB, int WPE>
 __device__ __noinline__ void stage(double2 *__restrict__ buf, int N, int p, int i, int NT,
                                       const double2 *__restrict__ tw, double s)
@@ -311,11 +308,15 @@ __global__ void k_unpack_halfcomplex(const double *__restrict__ hc, double2 *__r
 // host-side launchers (called from the planner / C ABI)
 // ---------------------------------------------------------------------------
 
+// Measured choice (MI355X): passes along rows with one or two transforms per
+// workgroup run faster with the larger register budget (2112-point c2c rows
+// 1.29 -> 1.07 ms over 16 x 2112^2; r2c of 4096 x 4096 rows 0.185 -> 0.162 ms;
+// 256 x 4095 0.030 -> 0.028 ms), wider row tiles and passes across transforms
+// with the higher occupancy (the rows of the 2112^2 r2c, three per workgroup:
+// 1.96 -> 1.71 ms with both passes; the 2112^2 columns 1.79 -> 1.51 ms).
 int fft_lds_wpe(const FftParams &P)
 {
-    const bool plain_rows = !P.col_load && !P.col_store && P.load_mode == LD_C2C && P.store_mode == ST_C2C &&
-                            !P.conv && !P.chirp_in && !P.chirp_out;
-    return plain_rows ? 3 : 4;
+    return (!P.col_load && !P.col_store && P.TB <= 2) ? 3 : 4;
 }
 
 hipError_t launch_fft_lds(const FftParams &P, unsigned grid, unsigned block, size_t lds_bytes, hipStream_t s)

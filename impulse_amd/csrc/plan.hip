@@ -618,7 +618,7 @@ struct Builder {
         // stays within the budget so that pass A's output could still sit in the
         // MALL (Infinity Cache) when pass B reads it.  Measured on MI355X the
         // smaller launches lose more than the cache saves, so it is off unless set.
-        static const long long mall = env_ll("IMPULSE_FFT_MALL_MB", 1LL << 30) * (1LL << 20);
+        const long long mall = env_ll("IMPULSE_FFT_MALL_MB", 1LL << 30) * (1LL << 20);
         const long long per_sig = n2 * (long long)sizeof(double2);
         const long long chunk = std::max<long long>(1, std::min<long long>(nlog, mall / per_sig));
         long long cn = chunk;   // signals in the current chunk (captured by the pass builders)

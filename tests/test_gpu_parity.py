@@ -369,6 +369,50 @@ def test_bluestein_three_pass(n):
             assert rel_l2(y, oracle.c2r(h, m, [1], fwd, 1.0 / m)) < TOL, ("c2r", m, fwd)
 
 
+@pytest.mark.parametrize("kind,n,batch", [("c2c", 16381, 5), ("c2c", 65536, 3), ("r2c", 65536, 3),
+                                          ("r2c", 32749, 5), ("c2r", 32762, 3)])
+def test_four_step_batch_chunks(kind, n, batch):
+    """Four-step batches split into chunks of signals (IMPULSE_FFT_MALL_MB=1: 1 MiB of workspace per
+    chunk, so one or two signals each): every chunk's passes address their signals through the
+    first-signal offset.  Three-pass Bluestein (16381, and 32762 = 2 * 16381 in c2r), the fused
+    row-pair r2c (65536), odd pairs (32749) and plain c2c, against the oracle and the unchunked plan."""
+    import os
+    torch = _torch()
+    print(f"case chunks {kind} n={n} batch={batch}", flush=True)
+    sp = torch.cuda.current_stream().cuda_stream
+    if kind == "c2c":
+        x = synth.complex_normal((batch, n), seed=n)
+        plan_kind, out_shape, out_dtype, want = _lib.C2C, (batch, n), torch.complex128, oracle.c2c(x, [1], True, 0.5)
+    elif kind == "r2c":
+        x = synth.uniform((batch, n), -1, 1, seed=n)
+        plan_kind, out_shape, out_dtype, want = (_lib.R2C, (batch, n // 2 + 1), torch.complex128,
+                                                 oracle.r2c(x, [1], True, 0.5))
+    else:
+        x = synth.complex_normal((batch, n // 2 + 1), seed=n)
+        plan_kind, out_shape, out_dtype, want = _lib.C2R, (batch, n), torch.float64, oracle.c2r(x, n, [1], True, 0.5)
+    xd = _dev(x)
+
+    def run():
+        yd = torch.empty(out_shape, dtype=out_dtype, device="cuda")
+        st_in = [v * xd.element_size() for v in xd.stride()]
+        st_out = [v * yd.element_size() for v in yd.stride()]
+        p = F.Plan(plan_kind, [batch, n], st_in, st_out, [1], True)
+        p.execute(xd.data_ptr(), yd.data_ptr(), 0.5, sp)
+        torch.cuda.synchronize()
+        return yd.cpu().numpy(), p.num_launches
+
+    whole, n_whole = run()
+    os.environ["IMPULSE_FFT_MALL_MB"] = "1"
+    try:
+        chunked, n_chunked = run()
+    finally:
+        del os.environ["IMPULSE_FFT_MALL_MB"]
+    assert n_chunked > n_whole, (n_chunked, n_whole)   # the batch did split
+    for row in range(batch):
+        assert rel_l2(chunked[row], want[row]) < TOL, (kind, n, row)
+    assert rel_l2(chunked, whole) < 1e-13
+
+
 # ---------------------------------------------------------------------------
 # error behaviour (pocketfft_hdronly.h:446-477 cases, forced from the host)
 # ---------------------------------------------------------------------------

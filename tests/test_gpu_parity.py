@@ -123,7 +123,7 @@ def test_lengths_vs_oracle(n):
         assert rel_l2(F.c2r(h, n, [1], fwd, 1.0 / n), oracle.c2r(h, n, [1], fwd, 1.0 / n)) < TOL, ("c2r", n, fwd)
 
 
-@pytest.mark.parametrize("n", [6, 64, 97, 4095, 4096, 10007, 65536])
+@pytest.mark.parametrize("n", [6, 64, 97, 4095, 4096, 10007, 16381, 65536])
 def test_strided_axis_vs_oracle(n):
     # transform along axis 0 of [n, 5] (batch contiguous, axis strided) and of a
     # negative-stride view

@@ -98,8 +98,10 @@ __device__ __noinline__ void rstage(double2 *lds, int base, int N, int p, int i,
 }
 
 // Radix sequence of a 13-smooth L as 4-bit codes (stage 0 in the low bits),
-// in registers: powers of two as 16s and one 2/4/8, odd radices ascending,
-// before (order 1, descending) or after (order 0) the powers of two.
+// in registers.  Order bit 0: odd radices ascending after the powers of two
+// (0) or descending before them (1).  Order bit 1: powers of two as 16s and one
+// 2/4/8 (0), or as 8s and one 2/4 (1), which keeps every thread of a
+// kPointsPerThread = 8 layout busy in those stages at the cost of more stages.
 __device__ __forceinline__ unsigned long long radix_codes(int L, int order, int &ns)
 {
     // code -> radix: 1:2 2:3 3:4 4:5 5:7 6:8 7:11 8:13 9:16
@@ -108,19 +110,24 @@ __device__ __forceinline__ unsigned long long radix_codes(int L, int order, int 
     int n = L;
     int twos = __builtin_ctz((unsigned)n);
     n >>= twos;
-    for (; twos >= 4; twos -= 4) pw |= 9ull << (4 * np++);
-    if (twos) pw |= (unsigned long long)(twos == 3 ? 6 : twos == 2 ? 3 : 1) << (4 * np++);
+    if (order & 2) {
+        for (; twos >= 3; twos -= 3) pw |= 6ull << (4 * np++);
+        if (twos) pw |= (unsigned long long)(twos == 2 ? 3 : 1) << (4 * np++);
+    } else {
+        for (; twos >= 4; twos -= 4) pw |= 9ull << (4 * np++);
+        if (twos) pw |= (unsigned long long)(twos == 3 ? 6 : twos == 2 ? 3 : 1) << (4 * np++);
+    }
     const int ps[5] = {3, 5, 7, 11, 13}, cs[5] = {2, 4, 5, 7, 8};
 #pragma unroll
     for (int q = 0; q < 5; ++q)
         while (n % ps[q] == 0) {
             n /= ps[q];
-            if (order == 1) od = (od << 4) | (unsigned long long)cs[q];   // descending: latest first
+            if (order & 1) od = (od << 4) | (unsigned long long)cs[q];   // descending: latest first
             else od |= (unsigned long long)cs[q] << (4 * no);
             ++no;
         }
     ns = np + no;
-    return order == 1 ? (od | (pw << (4 * no))) : (pw | (od << (4 * np)));
+    return (order & 1) ? (od | (pw << (4 * no))) : (pw | (od << (4 * np)));
 }
 
 // `count` transforms of length L at lds + t L, blockDim.x / count threads each

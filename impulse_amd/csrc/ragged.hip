@@ -68,7 +68,7 @@ struct RaggedParams {
     long long slot;
     double2 *spec;             // table of chirp-kernel spectra (unnormalized)
     int mode;                  // 0: transform the signals; 1: fill the spectrum table
-    int order;                 // radix order of LDS stages: 0 powers of two first, 1 odd radices first
+    int order;                 // radix order of LDS stages (radix_codes): bit 0 odd radices first, bit 1 radix-8 powers of two
     double fct;                // output scale
     double osign;              // +1: forward bins; -1: conjugated (backward sign)
 };
@@ -84,26 +84,6 @@ __host__ __device__ inline bool rg_smooth13(long long n)
     for (int k = 0; k < 6; ++k)
         while (n % ps[k] == 0) n /= ps[k];
     return n == 1;
-}
-
-// radix list of a 13-smooth n (powers of two as 16s and one 2/4/8)
-__host__ __device__ inline int rg_radices(long long n, int order, int *r)
-{
-    int k = 0, twos = 0;
-    while (n % 2 == 0) { n /= 2; ++twos; }
-    int odd[kMaxStages], no = 0;
-    const int ps[5] = {3, 5, 7, 11, 13};
-    for (int q = 0; q < 5; ++q)
-        while (n % ps[q] == 0) { odd[no++] = ps[q]; n /= ps[q]; }
-    if (order == 1)
-        for (int q = no - 1; q >= 0; --q) r[k++] = odd[q];
-    while (twos >= 4) { r[k++] = 16; twos -= 4; }
-    if (twos == 3) r[k++] = 8;
-    else if (twos == 2) r[k++] = 4;
-    else if (twos == 1) r[k++] = 2;
-    if (order == 0)
-        for (int q = 0; q < no; ++q) r[k++] = odd[q];
-    return k;
 }
 
 // transforms of length L per LDS batch: largest power of two c <= 64 with c L <= lds_max
@@ -804,10 +784,10 @@ int ragged_configure(Ragged &R, int threads, int lds_max, int order, int waves, 
 {
     const bool pow2 = lds_max > 0 && (lds_max & (lds_max - 1)) == 0;
     if ((threads != 256 && threads != 512 && threads != 1024) || !pow2 || lds_max < kBlueRow || lds_max > 8192 ||
-        lds_max > kPointsPerThread * threads || (order != 0 && order != 1) ||
+        lds_max > kPointsPerThread * threads || order < 0 || order > 3 ||
         (waves != 0 && waves != 4 && !(waves == 3 && threads == 256))) {
         err = {E_INVALID, "ragged configuration out of range (threads 256/512/1024; lds_points a power of two in "
-                          "[2048, 8192] and at most 8 per thread; radix order 0/1; waves per SIMD 0, 4, or 3 "
+                          "[2048, 8192] and at most 8 per thread; radix order 0-3; waves per SIMD 0, 4, or 3 "
                           "with 256 threads)"};
         return E_INVALID;
     }
@@ -908,7 +888,8 @@ int ragged_tune(Ragged &R, const float *x, double2 *y, hipStream_t s, int reps, 
     const Cfg cands[] = {{1024, 8192, 0, 0}, {1024, 8192, 1, 0}, {1024, 4096, 0, 0}, {512, 4096, 0, 0},
                          {512, 4096, 1, 0}, {512, 4096, 1, 4}, {512, 2048, 1, 0}, {512, 2048, 1, 4},
                          {256, 2048, 0, 0}, {256, 2048, 1, 0}, {256, 2048, 0, 4}, {256, 2048, 1, 4},
-                         {256, 2048, 0, 3}, {256, 2048, 1, 3}};
+                         {256, 2048, 0, 3}, {256, 2048, 1, 3}, {256, 2048, 3, 4}, {256, 2048, 2, 4},
+                         {256, 2048, 3, 3}, {512, 4096, 3, 4}};
     hipEvent_t a = nullptr, b = nullptr;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
         (void)hipGetLastError();

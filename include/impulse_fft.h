@@ -292,7 +292,8 @@ int impulse_fft_ragged_tune(impulse_fft_ragged batch, const float *in, void *out
                             double *best_ms);
 /* Pick a configuration by hand: threads 256/512/1024, lds_points
  * 2048/4096/8192 (longest transform kept in LDS; at most 8 per thread),
- * radix_order 0 (powers of two first) or 1 (odd radices first),
+ * radix_order 0-3 (bit 0: odd radices before the powers of two; bit 1: powers
+ * of two as radix-8 rather than radix-16 stages),
  * waves_per_simd 0 (registers as the workgroup size allows), 4 (at most 128
  * registers per lane: more resident workgroups, some spills) or, with 256
  * threads, 3 (at most 168). */

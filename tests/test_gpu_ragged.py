@@ -23,7 +23,8 @@ pytestmark = pytest.mark.gpu
 
 CONFIGS = [(1024, 8192, 0, 0), (1024, 8192, 1, 0), (1024, 4096, 0, 0), (512, 4096, 0, 0), (512, 4096, 1, 0),
            (512, 4096, 1, 4), (512, 2048, 1, 0), (512, 2048, 1, 4), (256, 2048, 0, 0), (256, 2048, 1, 0),
-           (256, 2048, 0, 4), (256, 2048, 1, 4), (256, 2048, 0, 3), (256, 2048, 1, 3)]
+           (256, 2048, 0, 4), (256, 2048, 1, 4), (256, 2048, 0, 3), (256, 2048, 1, 3), (256, 2048, 3, 4),
+           (256, 2048, 2, 4), (256, 2048, 3, 3), (512, 4096, 3, 4)]
 
 # every branch: LDS-resident (smooth / Bluestein, odd / even), split smooth (even with the paired
 # output, odd), split Bluestein with short columns (3001, 3022) and long ones, n = 1, 2, 3

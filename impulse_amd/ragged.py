@@ -81,6 +81,10 @@ class RaggedBatch:
         return ms.value
 
     def configure(self, threads, lds_points, radix_order=0, waves_per_simd=0):
+        """Pick a configuration by hand (impulse_fft_ragged_configure): threads 256/512/1024,
+        lds_points 2048/4096/8192 (at most 8 per thread), radix_order 0-3 (bit 0: odd radices
+        before the powers of two; bit 1: powers of two as radix-8 stages), waves_per_simd 0, 4,
+        or 3 with 256 threads."""
         check(lib().impulse_fft_ragged_configure(self.handle, int(threads), int(lds_points), int(radix_order),
                                                  int(waves_per_simd)))
 

@@ -13,7 +13,11 @@ with HIP events on the execution stream inside the timed region, and its
 algorithmic bytes per launch are the bytes each launch must read plus write
 (input read once, output written once).  cpu_baseline times the reference
 PocketFFT C++ (built from the reference sources into oracle/_ref) on a bounded
-sample of the same workload on this host's cores.  vendor_fft times torch.fft
+sample of the same workload on this host's cores (the job's CPU share, and one
+thread).  `c1` times the reference's CPU-runnable size, r2c [256, 4096] and
+[256, 4095] and back, on device buffers, end to end from host arrays through
+DataDesc / FFTDesc / apply, and through the C path with a plan per call, beside
+the reference PocketFFT on the same arrays.  vendor_fft times torch.fft
 (hipFFT / rocFFT on ROCm) on the same device inputs as a black-box comparator;
 it is reported beside the result and never enters `value`.
 """
@@ -40,6 +44,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the [256, 4096] / [256, 4095] r2c leg (device, host end to end, C path)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the forms and widened paths (ragged, MFMA, convolution, DCT, upfirdn, fftpack)")
@@ -116,16 +122,21 @@ def torch_dtype(torch, s):
     return {"c128": torch.complex128, "f64": torch.float64}[s]
 
 
-def cpu_baseline(plist, budget_s):
-    """Reference PocketFFT C++ (oracle/_ref) on this host, same workload, bounded time."""
+def host_cores():
+    """CPUs this process may run on, the machine's count, and the job's CPU share (the box gives one
+    GPU's job the share in OMP_NUM_THREADS; elsewhere the affinity mask is the limit)."""
+    total = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = total
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    use = min(affinity, share) if share > 0 else affinity
+    return max(1, use), total, affinity
+
+
+def _ref_rate(plist, data, threads, budget_s, max_reps=50):
     from oracle import ref
-    if not ref.available():
-        return None
-    threads = max(1, min(16, os.cpu_count() or 1))
-    data = []
-    for p in plist:
-        x = p.make_input()
-        data.append(x)
     total_flops, total_s, reps = 0.0, 0.0, 0
     t_start = time.perf_counter()
     while True:
@@ -138,12 +149,150 @@ def cpu_baseline(plist, budget_s):
             total_s += time.perf_counter() - t0
             total_flops += p.flops
         reps += 1
-        if time.perf_counter() - t_start > budget_s or reps >= 50:
+        if time.perf_counter() - t_start > budget_s or reps >= max_reps:
             break
-    return {"value": total_flops / total_s / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "reference",
-            "sample": f"the full 3-part workload repeated {reps}x on the host "
-                      f"(reference PocketFFT C++ c2c/r2c, nthreads={threads}, "
-                      f"built -O3 -march=x86-64-v3 from the reference sources)"}
+    return total_flops / total_s / 1e9, reps
+
+
+def cpu_baseline(plist, budget_s):
+    """Reference PocketFFT C++ (oracle/_ref) on this host's cores, same workload, bounded time: at the
+    job's full CPU share (`value`) and at nthreads = 1."""
+    from oracle import ref
+    if not ref.available():
+        return None
+    threads, total, affinity = host_cores()
+    data = [p.make_input() for p in plist]
+    v_all, reps_all = _ref_rate(plist, data, threads, budget_s)
+    v_one, reps_one = _ref_rate(plist, data, 1, budget_s / 2, max_reps=3)
+    return {"value": v_all, "unit": "GFLOP/s", "cores": threads, "kind": "reference",
+            "host_cpus": total, "affinity_cpus": affinity,
+            "single_thread": {"value": v_one, "unit": "GFLOP/s", "cores": 1, "reps": reps_one},
+            "sample": f"the full 3-part workload repeated {reps_all}x on the host (reference PocketFFT C++ "
+                      f"c2c/r2c, nthreads={threads} = the job's CPU share of a {total}-CPU host, "
+                      f"built -O3 -march=x86-64-v3 from the reference sources); single_thread: "
+                      f"{reps_one}x at nthreads=1"}
+
+
+def c1_leg(torch, reps, with_cpu, host_reps=10):
+    """Plain FP64 drop-in at the reference's CPU-runnable size: r2c float64 [256, 4096] and
+    [256, 4095] (4095 = 3^2*5*7*13) along axis 1 (FFTDesc axes=[1]), uniform[-1, 1] seed 0, then the
+    inverse c2r with fct 1/N.  Three ways:
+      * device: device-resident buffers, HIP events;
+      * host: end to end through DataDesc / FFTDesc / apply on host arrays (the drop-in as impulse
+        calls it: PCIe copies through the plan's pinned staging included);
+      * c_path: impulse's C path per row, a plan built and destroyed per call
+        (c_pocketfft/pocketfft.nim:216-288: make_rfft_plan, rfft_forward, destroy_rfft_plan).
+    With with_cpu, the reference PocketFFT (oracle/_ref) is timed on the same arrays at nthreads = 1 and
+    at the job's CPU share, and its results are the accuracy reference."""
+    from impulse_amd import _lib, synth
+    from impulse_amd import fft as F
+
+    L = _lib.lib()
+    rows = 256
+    sp = torch.cuda.current_stream().cuda_stream
+    threads, total_cpus, _ = host_cores()
+    out = {"workload": "r2c f64 [256, N] axes [1] + c2r back (fct 1/N), N = 4096 and 4095", "cases": []}
+    for n in (4096, 4095):
+        nb = n // 2 + 1
+        x = synth.uniform((rows, n), -1.0, 1.0, seed=0)
+        fl = rows * 2.5 * n * math.log2(n)
+        comp = rows * (n * 8 + nb * 16)
+        xd = torch.from_numpy(x).cuda()
+        Xd = torch.empty((rows, nb), dtype=torch.complex128, device="cuda")
+        zd = torch.empty_like(xd)
+        pf = F.Plan(_lib.R2C, [rows, n], [n * 8, 8], [nb * 16, 16], [1], True)
+        pb = F.Plan(_lib.C2R, [rows, n], [nb * 16, 16], [n * 8, 8], [1], False)
+        fwd = _timed(torch, lambda: pf.execute(xd.data_ptr(), Xd.data_ptr(), 1.0, sp), reps)
+        bwd = _timed(torch, lambda: pb.execute(Xd.data_ptr(), zd.data_ptr(), 1.0 / n, sp), reps)
+        X = Xd.cpu().numpy()
+        rt = float(np.linalg.norm(zd.cpu().numpy() - x) / np.linalg.norm(x))
+        case = {"n": n, "gflop": round(fl / 1e9, 4),
+                "device": {"fwd_ms": round(fwd, 4), "bwd_ms": round(bwd, 4),
+                           "fwd_gflops": round(fl / (fwd * 1e-3) / 1e9, 1),
+                           "fwd_compulsory_GBs": round(comp / (fwd * 1e-3) / 1e9, 1),
+                           "fwd_hbm_roofline_frac": round(comp / (fwd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                           "launches": pf.num_launches},
+                "round_trip_rel_l2": rt}
+        # end to end on host arrays through the drop-in API
+        Xh = np.empty((rows, nb), np.complex128)
+        zh = np.empty_like(x)
+        fdesc, bdesc = F.FFTDesc([1], True), F.FFTDesc([1], False, 1.0 / n)
+        dX, dx, dz = F.DataDesc(Xh, Xh.shape), F.DataDesc(x, x.shape), F.DataDesc(zh, zh.shape)
+        F.apply(fdesc, dX, dx)
+        F.apply(bdesc, dz, dX)
+        tf, tb = [], []
+        for _ in range(host_reps):
+            t0 = time.perf_counter()
+            F.apply(fdesc, dX, dx)
+            t1 = time.perf_counter()
+            F.apply(bdesc, dz, dX)
+            tb.append(time.perf_counter() - t1)
+            tf.append(t1 - t0)
+        hf, hb = 1e3 * float(np.median(tf)), 1e3 * float(np.median(tb))
+        case["host"] = {"fwd_ms": round(hf, 4), "bwd_ms": round(hb, 4), "fwd_gflops": round(fl / (hf * 1e-3) / 1e9, 2),
+                        "note": "PCIe-inclusive: host arrays staged through pinned buffers; not `value`"}
+        # impulse's C path, one plan per call per row
+        buf = np.empty(n)
+        t0 = time.perf_counter()
+        for r in range(rows):
+            buf[:] = x[r]
+            h = L.impulse_make_rfft_plan(n)
+            if L.impulse_rfft_forward(h, buf.ctypes.data, 1.0) != 0:
+                raise RuntimeError("Forward FFT calculation failed.")
+            L.impulse_destroy_rfft_plan(h)
+        cp = (time.perf_counter() - t0) * 1e3
+        case["c_path"] = {"ms": round(cp, 3), "per_call_us": round(cp * 1e3 / rows, 1),
+                          "gflops": round(fl / (cp * 1e-3) / 1e9, 3)}
+        if with_cpu:
+            case["cpu_reference"] = cpu_reference_c1(x, X, n, rows, threads, total_cpus, host_reps)
+        out["cases"].append(case)
+        del pf, pb, xd, Xd, zd
+    return out
+
+
+def cpu_reference_c1(x, X_gpu, n, rows, threads, total_cpus, reps):
+    """CPU baseline leg of the C1 workload: the reference PocketFFT C++ r2c / c2r with the byte strides
+    DataDesc.init makes (cpp_pocketfft/pocketfft.nim:178-199) at nthreads = 1 and at the job's CPU share,
+    and its C path as impulse calls it (plan per call) and with the plan reused.  Also the GPU result's
+    rel-L2 against the reference's."""
+    from oracle import ref
+    if not ref.available():
+        return None
+    fl = rows * 2.5 * n * math.log2(n)
+    res = {"host_cpus": total_cpus}
+    want = ref.r2c(x, [1], True)
+    res["gpu_vs_reference_rel_l2"] = float(np.linalg.norm(X_gpu - want) / np.linalg.norm(want))
+    for th in (1, threads):
+        tf, tb = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            Y = ref.r2c(x, [1], True, 1.0, th)
+            t1 = time.perf_counter()
+            ref.c2r(Y, n, [1], False, 1.0 / n, th)
+            tb.append(time.perf_counter() - t1)
+            tf.append(t1 - t0)
+        f = 1e3 * float(np.median(tf))
+        res[f"cpp_nthreads_{th}"] = {"fwd_ms": round(f, 4), "bwd_ms": round(1e3 * float(np.median(tb)), 4),
+                                     "fwd_gflops": round(fl / (f * 1e-3) / 1e9, 2)}
+    L = ref.lib()
+    buf = np.empty(n)
+    t0 = time.perf_counter()
+    for r in range(rows):
+        buf[:] = x[r]
+        h = L.make_rfft_plan(n)
+        L.rfft_forward(h, buf.ctypes.data, 1.0)
+        L.destroy_rfft_plan(h)
+    per_call = (time.perf_counter() - t0) * 1e3
+    h = L.make_rfft_plan(n)
+    t0 = time.perf_counter()
+    for r in range(rows):
+        buf[:] = x[r]
+        L.rfft_forward(h, buf.ctypes.data, 1.0)
+    reused = (time.perf_counter() - t0) * 1e3
+    L.destroy_rfft_plan(h)
+    res["c_path_plan_per_call"] = {"ms": round(per_call, 3), "gflops": round(fl / (per_call * 1e-3) / 1e9, 2)}
+    res["c_path_plan_reused"] = {"ms": round(reused, 3), "gflops": round(fl / (reused * 1e-3) / 1e9, 2)}
+    return res
 
 
 def ragged_form(torch, reps, loop_sample=600):
@@ -549,6 +698,7 @@ def main():
                           "hbm_roofline_frac": round(p.comp_bytes / (pms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                           "launches": npass})
 
+    c1 = None if args.no_c1 else c1_leg(torch, max(args.steps, 20), not args.no_cpu)
     forms = [] if args.no_forms else [ragged_form(torch, args.form_reps), mfma_form(torch, args.form_reps),
                                        conv_form(torch, args.form_reps), dct_form(torch, args.form_reps),
                                        upfirdn_form(torch, args.form_reps), fftpack_form(torch, args.form_reps)]
@@ -560,7 +710,8 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (SplitMix64 seed 0)",
         "config": {"workload": "FP64 drop-in at scale: c2c c128 4096x4096 axes [0,1] + r2c f64 2^22 x16 + "
                                "r2c f64 1000003 x4", "parts": [p.name for p in plist]},
-        "roofline": roofline, "cpu_baseline": cpu, "parts": part_rows, "launches": launches, "forms": forms,
+        "roofline": roofline, "cpu_baseline": cpu, "parts": part_rows, "launches": launches, "c1": c1,
+        "forms": forms,
         "vendor_fft": vendor,
     }
     print(json.dumps(out))

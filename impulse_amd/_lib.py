@@ -18,6 +18,7 @@ EXPORTS = (
     "impulse_fft_plan_profile", "impulse_fft_plan_profile_read", "impulse_fft_plan_destroy",
     "impulse_fft_c2c", "impulse_fft_r2c", "impulse_fft_c2r",
     "impulse_fft_cache_set_capacity", "impulse_fft_cache_size", "impulse_fft_cache_clear",
+    "impulse_fft_cache_stats",
     "impulse_make_cfft_plan", "impulse_destroy_cfft_plan", "impulse_cfft_forward",
     "impulse_cfft_backward", "impulse_cfft_length",
     "impulse_make_rfft_plan", "impulse_destroy_rfft_plan", "impulse_rfft_forward",
@@ -64,6 +65,7 @@ def _sig(L):
         getattr(L, n).argtypes = [sz, _szp, _pdp, _pdp, sz, _szp, i, i, _vp, _vp, d, sz]
     L.impulse_fft_cache_set_capacity.argtypes = [sz]
     L.impulse_fft_cache_size.restype = sz
+    L.impulse_fft_cache_stats.argtypes = [_szp, _szp]
     for kind in ("cfft", "rfft"):
         getattr(L, f"impulse_make_{kind}_plan").argtypes = [sz]
         getattr(L, f"impulse_make_{kind}_plan").restype = _vp

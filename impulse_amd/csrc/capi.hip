@@ -35,16 +35,30 @@ struct impulse_fft_upfirdn_s {
     ~impulse_fft_upfirdn_s() { ifft::upfirdn_destroy(u); }
 };
 
+// C path: the device plans behind make_cfft_plan / make_rfft_plan come from
+// the plan caches below, so that impulse's plan-per-call pattern
+// (c_pocketfft/pocketfft.nim:216-303 builds and destroys a plan in every
+// fft() call) pays a cache lookup, not a device allocation and a twiddle
+// upload, after the first call of a length.
 struct impulse_cfft_plan_s {
     size_t length = 0;
-    std::unique_ptr<ifft::Plan> fwd, bwd;
+    std::shared_ptr<ifft::Plan> fwd, bwd;
 };
+
+namespace {
+// one real length of the C path: r2c / c2r plans and the device buffers of the
+// halfcomplex repacking, shared by every rfft plan handle of that length
+struct RfftCore {
+    std::shared_ptr<ifft::Plan> r2c, c2r;
+    ifft::DevBuf x, bins, hc;   // device staging: reals, complex bins, halfcomplex
+    ifft::HostStage st;
+    std::mutex mtx;
+};
+}  // namespace
 
 struct impulse_rfft_plan_s {
     size_t length = 0;
-    std::unique_ptr<ifft::Plan> r2c, c2r;
-    ifft::DevBuf x, bins, hc;   // device staging: reals, complex bins, halfcomplex
-    std::mutex mtx;
+    std::shared_ptr<RfftCore> core;
 };
 
 namespace {
@@ -63,12 +77,16 @@ int fail(int code, const char *msg)
     return code;
 }
 
-// ---- LRU plan cache for the one-shot calls -------------------------------
-struct Cache {
+// ---- LRU plan caches ---------------------------------------------------------
+// Like get_plan (pocketfft_hdronly.h:2655-2706): a mutex-guarded LRU list,
+// looked up under the lock, built outside it, and inserted with a second
+// lookup in case another thread built the same plan meanwhile.
+template <class V> struct Lru {
     std::mutex mtx;
     size_t capacity = 16;
-    std::list<std::pair<std::string, std::shared_ptr<ifft::Plan>>> lru;
-    std::unordered_map<std::string, decltype(lru)::iterator> map;
+    size_t hits = 0, misses = 0;
+    std::list<std::pair<std::string, std::shared_ptr<V>>> lru;
+    std::unordered_map<std::string, typename decltype(lru)::iterator> map;
 
     void trim()
     {
@@ -77,11 +95,41 @@ struct Cache {
             lru.pop_back();
         }
     }
+    // most recently used entry of `key`, or null (counts a hit or a miss)
+    std::shared_ptr<V> find(const std::string &key)
+    {
+        std::lock_guard<std::mutex> lk(mtx);
+        auto it = map.find(key);
+        if (it == map.end()) {
+            ++misses;
+            return nullptr;
+        }
+        ++hits;
+        lru.splice(lru.begin(), lru, it->second);
+        return it->second->second;
+    }
+    // insert `v` unless another thread inserted `key` first; returns the entry kept
+    std::shared_ptr<V> insert(const std::string &key, std::shared_ptr<V> v)
+    {
+        std::lock_guard<std::mutex> lk(mtx);
+        auto it = map.find(key);
+        if (it != map.end()) return it->second->second;
+        lru.emplace_front(key, v);
+        map[key] = lru.begin();
+        trim();
+        return v;
+    }
 };
 
-Cache &cache()
+Lru<ifft::Plan> &cache()
 {
-    static Cache *c = new Cache();   // never destroyed: plans may outlive static teardown
+    static auto *c = new Lru<ifft::Plan>();   // never destroyed: plans may outlive static teardown
+    return *c;
+}
+
+Lru<RfftCore> &rcache()
+{
+    static auto *c = new Lru<RfftCore>();
     return *c;
 }
 
@@ -105,16 +153,8 @@ int cached_plan(std::shared_ptr<ifft::Plan> &out, int kind, int dtype, size_t nd
     int dev = 0;
     (void)hipGetDevice(&dev);
     const std::string key = cache_key(kind, dtype, ndim, shape, sin, sout, naxes, axes, forward, dev);
-    Cache &c = cache();
-    {
-        std::lock_guard<std::mutex> lk(c.mtx);
-        auto it = c.map.find(key);
-        if (it != c.map.end()) {
-            c.lru.splice(c.lru.begin(), c.lru, it->second);
-            out = it->second->second;
-            return ifft::OK;
-        }
-    }
+    auto &c = cache();
+    if ((out = c.find(key))) return ifft::OK;
     std::unique_ptr<ifft::Plan> p;
     ifft::Error err{0, ""};
     // DCT / DST (kinds 5, 6) carry the type and ortho flag in `forward` (2 type + ortho), the
@@ -125,19 +165,7 @@ int cached_plan(std::shared_ptr<ifft::Plan> &out, int kind, int dtype, size_t nd
                                                  naxes, axes, err)
                          : ifft::plan_create(p, kind, dtype, ndim, shape, sin, sout, naxes, axes, forward, err);
     if (rc != ifft::OK) return fail(err);
-    std::shared_ptr<ifft::Plan> sp(std::move(p));
-    {
-        std::lock_guard<std::mutex> lk(c.mtx);
-        auto it = c.map.find(key);
-        if (it != c.map.end()) {   // another thread won the race
-            out = it->second->second;
-            return ifft::OK;
-        }
-        c.lru.emplace_front(key, sp);
-        c.map[key] = c.lru.begin();
-        c.trim();
-    }
-    out = sp;
+    out = c.insert(key, std::shared_ptr<ifft::Plan>(std::move(p)));
     return ifft::OK;
 }
 
@@ -147,6 +175,16 @@ int oneshot(int kind, size_t ndim, const size_t *shape, const ptrdiff_t *sin, co
     if (shape)
         for (size_t d = 0; d < ndim; ++d)
             if (shape[d] == 0) return ifft::OK;   // pocketfft returns early on empty arrays
+    // in-place rules before any plan is built (pocketfft_hdronly.h:453-454): equal
+    // strides, and no in-place r2c / c2r
+    if (in && in == out) {
+        if (kind == IMPULSE_FFT_R2C || kind == IMPULSE_FFT_C2R || kind == IMPULSE_FFT_R2C_ADJOINT ||
+            kind == IMPULSE_FFT_C2R_ADJOINT)
+            return fail(ifft::E_INVALID, "in-place r2c/c2r is not supported");
+        if (sin && sout)
+            for (size_t d = 0; d < ndim; ++d)
+                if (sin[d] != sout[d]) return fail(ifft::E_INVALID, "stride mismatch");
+    }
     std::shared_ptr<ifft::Plan> p;
     int rc = cached_plan(p, kind, dtype, ndim, shape, sin, sout, naxes, axes, forward);
     if (rc != ifft::OK) return rc;
@@ -155,15 +193,14 @@ int oneshot(int kind, size_t ndim, const size_t *shape, const ptrdiff_t *sin, co
     return rc == ifft::OK ? rc : fail(err);
 }
 
-int make_1d(std::unique_ptr<ifft::Plan> &p, int kind, size_t n, int forward)
+// 1-D FP64 plan of the C path, unit stride, from the plan cache
+int cached_1d(std::shared_ptr<ifft::Plan> &p, int kind, size_t n, int forward)
 {
     const size_t shape[1] = {n};
     const ptrdiff_t sin[1] = {kind == IMPULSE_FFT_R2C ? 8 : 16};
     const ptrdiff_t sout[1] = {kind == IMPULSE_FFT_C2R ? 8 : 16};
     const size_t axes[1] = {0};
-    ifft::Error err{0, ""};
-    int rc = ifft::plan_create(p, kind, 0, 1, shape, sin, sout, 1, axes, forward, err);
-    return rc == ifft::OK ? rc : fail(err);
+    return cached_plan(p, kind, IMPULSE_FFT_F64, 1, shape, sin, sout, 1, axes, forward);
 }
 
 }  // namespace
@@ -380,26 +417,48 @@ int impulse_fft_c2r(size_t ndim, const size_t *shape_out, const ptrdiff_t *strid
 
 int impulse_fft_cache_set_capacity(size_t entries)
 {
-    Cache &c = cache();
-    std::lock_guard<std::mutex> lk(c.mtx);
-    c.capacity = entries;
-    c.trim();
+    for (int which = 0; which < 2; ++which) {
+        auto apply = [&](auto &c) {
+            std::lock_guard<std::mutex> lk(c.mtx);
+            c.capacity = entries;
+            c.trim();
+        };
+        if (which == 0) apply(cache());
+        else apply(rcache());
+    }
     return ifft::OK;
 }
 
 size_t impulse_fft_cache_size(void)
 {
-    Cache &c = cache();
+    auto &c = cache();
     std::lock_guard<std::mutex> lk(c.mtx);
     return c.lru.size();
 }
 
 int impulse_fft_cache_clear(void)
 {
-    Cache &c = cache();
+    {
+        auto &c = cache();
+        std::lock_guard<std::mutex> lk(c.mtx);
+        c.lru.clear();
+        c.map.clear();
+        c.hits = c.misses = 0;
+    }
+    auto &r = rcache();
+    std::lock_guard<std::mutex> lk(r.mtx);
+    r.lru.clear();
+    r.map.clear();
+    r.hits = r.misses = 0;
+    return ifft::OK;
+}
+
+int impulse_fft_cache_stats(size_t *hits, size_t *misses)
+{
+    auto &c = cache();
     std::lock_guard<std::mutex> lk(c.mtx);
-    c.lru.clear();
-    c.map.clear();
+    if (hits) *hits = c.hits;
+    if (misses) *misses = c.misses;
     return ifft::OK;
 }
 
@@ -619,7 +678,8 @@ impulse_cfft_plan impulse_make_cfft_plan(size_t length)
     auto *h = new (std::nothrow) impulse_cfft_plan_s;
     if (!h) return nullptr;
     h->length = length;
-    if (make_1d(h->fwd, IMPULSE_FFT_C2C, length, 1) != ifft::OK || make_1d(h->bwd, IMPULSE_FFT_C2C, length, 0) != ifft::OK) {
+    if (cached_1d(h->fwd, IMPULSE_FFT_C2C, length, 1) != ifft::OK ||
+        cached_1d(h->bwd, IMPULSE_FFT_C2C, length, 0) != ifft::OK) {
         delete h;
         return nullptr;
     }
@@ -648,13 +708,21 @@ impulse_rfft_plan impulse_make_rfft_plan(size_t length)
     auto *h = new (std::nothrow) impulse_rfft_plan_s;
     if (!h) return nullptr;
     h->length = length;
-    const size_t nb = length / 2 + 1;
-    if (make_1d(h->r2c, IMPULSE_FFT_R2C, length, 1) != ifft::OK ||
-        make_1d(h->c2r, IMPULSE_FFT_C2R, length, 0) != ifft::OK || h->x.alloc(length * 8) != ifft::OK ||
-        h->bins.alloc(nb * 16) != ifft::OK || h->hc.alloc(length * 8) != ifft::OK) {
-        if (g_last_error.empty()) g_last_error = "rfft plan allocation failed";
-        delete h;
-        return nullptr;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const std::string key = std::to_string(length) + '|' + std::to_string(dev);
+    auto &rc = rcache();
+    if (!(h->core = rc.find(key))) {
+        auto core = std::make_shared<RfftCore>();
+        const size_t nb = length / 2 + 1;
+        if (cached_1d(core->r2c, IMPULSE_FFT_R2C, length, 1) != ifft::OK ||
+            cached_1d(core->c2r, IMPULSE_FFT_C2R, length, 0) != ifft::OK || core->x.alloc(length * 8) != ifft::OK ||
+            core->bins.alloc(nb * 16) != ifft::OK || core->hc.alloc(length * 8) != ifft::OK) {
+            if (g_last_error.empty()) g_last_error = "rfft plan allocation failed";
+            delete h;
+            return nullptr;
+        }
+        h->core = rc.insert(key, core);
     }
     return h;
 }
@@ -664,29 +732,34 @@ void impulse_destroy_rfft_plan(impulse_rfft_plan plan) { delete plan; }
 static int rfft_run(impulse_rfft_plan plan, double c[], double fct, bool forward)
 {
     if (!plan || !c) { fail(ifft::E_INVALID, "NULL plan or data"); return -1; }
-    std::lock_guard<std::mutex> lk(plan->mtx);
     const size_t n = plan->length;
     if (n == 1) return 0;   // PocketFFT's C path returns unscaled here (pocketfft.c:1703,1739)
+    RfftCore &C = *plan->core;
+    std::lock_guard<std::mutex> lk(C.mtx);
     ifft::Error err{0, ""};
+    int rc = C.st.init(err);
     hipError_t e = hipSuccess;
-    int rc = ifft::OK;
-    if (forward) {
-        e = hipMemcpy(plan->x.p, c, n * 8, hipMemcpyHostToDevice);
-        if (e == hipSuccess) rc = ifft::plan_execute(*plan->r2c, plan->x.p, plan->bins.p, fct, nullptr, err);
-        if (e == hipSuccess && rc == ifft::OK)
-            e = ifft::launch_pack_halfcomplex((const double2 *)plan->bins.p, (double *)plan->hc.p, (long long)n, nullptr);
-        if (e == hipSuccess && rc == ifft::OK) e = hipMemcpy(c, plan->hc.p, n * 8, hipMemcpyDeviceToHost);
-    } else {
-        e = hipMemcpy(plan->hc.p, c, n * 8, hipMemcpyHostToDevice);
-        if (e == hipSuccess)
-            e = ifft::launch_unpack_halfcomplex((const double *)plan->hc.p, (double2 *)plan->bins.p, (long long)n, nullptr);
-        if (e == hipSuccess) rc = ifft::plan_execute(*plan->c2r, plan->bins.p, plan->x.p, fct, nullptr, err);
-        if (e == hipSuccess && rc == ifft::OK) e = hipMemcpy(c, plan->x.p, n * 8, hipMemcpyDeviceToHost);
+    if (rc == ifft::OK && forward) {
+        rc = C.st.to_device(C.x.p, c, n * 8, err);
+        if (rc == ifft::OK) rc = ifft::plan_execute(*C.r2c, C.x.p, C.bins.p, fct, C.st.stream, err);
+        if (rc == ifft::OK)
+            e = ifft::launch_pack_halfcomplex((const double2 *)C.bins.p, (double *)C.hc.p, (long long)n, C.st.stream);
+        if (rc == ifft::OK && e == hipSuccess) rc = C.st.to_host(c, C.hc.p, n * 8, err);
+    } else if (rc == ifft::OK) {
+        rc = C.st.to_device(C.hc.p, c, n * 8, err);
+        if (rc == ifft::OK)
+            e = ifft::launch_unpack_halfcomplex((const double *)C.hc.p, (double2 *)C.bins.p, (long long)n, C.st.stream);
+        if (rc == ifft::OK && e == hipSuccess) rc = ifft::plan_execute(*C.c2r, C.bins.p, C.x.p, fct, C.st.stream, err);
+        if (rc == ifft::OK && e == hipSuccess) rc = C.st.to_host(c, C.x.p, n * 8, err);
     }
-    if (rc != ifft::OK) { fail(err); return -1; }
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        fail(ifft::E_HIP, hipGetErrorString(e));
+    if (rc != ifft::OK || e != hipSuccess) {
+        ifft::Error e2;
+        if (C.st.stream) (void)C.st.sync(e2);
+        if (rc != ifft::OK) fail(err);
+        else {
+            (void)hipGetLastError();
+            fail(ifft::E_HIP, hipGetErrorString(e));
+        }
         return -1;
     }
     return 0;

@@ -47,6 +47,28 @@ struct DevBuf {
     int alloc(size_t n);
 };
 
+// Host <-> device copies of the host-pointer entry points: pinned staging
+// buffers used in turn, so that the CPU copy of one chunk overlaps the DMA of
+// the previous one, on a stream of its own (default priority).  Not
+// thread-safe: the owner serialises calls.
+struct HostStage {
+    hipStream_t stream = nullptr;
+    void *pin[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool pending[2] = {false, false};
+    size_t chunk = 0;
+    HostStage() = default;
+    HostStage(const HostStage &) = delete;
+    HostStage &operator=(const HostStage &) = delete;
+    ~HostStage();
+    int init(Error &err);
+    // host -> device, ordered on `stream`; the host buffer may be reused on return
+    int to_device(void *dev, const void *host, size_t n, Error &err);
+    // device -> host after the work already queued on `stream`; complete on return
+    int to_host(void *host, const void *dev, size_t n, Error &err);
+    int sync(Error &err);
+};
+
 // Which buffer a pass reads or writes; resolved at execute time.
 enum BufRef : int { REF_IN = 0, REF_OUT = 1, REF_S0 = 2, REF_S1 = 3, REF_TMP = 4, REF_NONE = 5 };
 
@@ -85,6 +107,7 @@ struct Plan {
     // host staging (execute_host)
     std::mutex host_mtx;
     DevBuf stage_in, stage_out;
+    HostStage host;
     ptrdiff_t in_lo = 0, in_hi = 0, out_lo = 0, out_hi = 0;   // byte spans relative to element 0
     bool out_dense = false;
     // per-launch event timing (impulse_fft_plan_profile)

@@ -54,6 +54,38 @@ __device__ __forceinline__ double2 real_tw(const FftParams &P, long long k)
 }
 
 // ---------------------------------------------------------------------------
+// element access of user arrays: double / double2, or float / float2 widened
+// to FP64 on load and rounded to nearest on store (the flag is uniform per
+// launch)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double2 gld_c(const void *p, int f32, long long i)
+{
+    if (f32) {
+        const float2 v = ((const float2 *)p)[i];
+        return make_double2((double)v.x, (double)v.y);
+    }
+    return ((const double2 *)p)[i];
+}
+
+__device__ __forceinline__ double gld_r(const void *p, int f32, long long i)
+{
+    return f32 ? (double)((const float *)p)[i] : ((const double *)p)[i];
+}
+
+__device__ __forceinline__ void gst_c(void *p, int f32, long long i, double2 v)
+{
+    if (f32) ((float2 *)p)[i] = make_float2((float)v.x, (float)v.y);
+    else ((double2 *)p)[i] = v;
+}
+
+__device__ __forceinline__ void gst_r(void *p, int f32, long long i, double v)
+{
+    if (f32) ((float *)p)[i] = (float)v;
+    else ((double *)p)[i] = v;
+}
+
+// ---------------------------------------------------------------------------
 // per-transform info and the load of one point
 // ---------------------------------------------------------------------------
 
@@ -67,26 +99,21 @@ struct TfInfo {
 __device__ __forceinline__ double2 load_point(const FftParams &P, const TfInfo &T, long long pos)
 {
     const Side &S = P.in;
+    const int f32 = P.src32;
     switch (P.load_mode) {
-        case LD_C2C: {
-            const double2 *x = (const double2 *)P.src;
-            return x[T.ia + pos * S.sax];
-        }
-        case LD_REAL_HALF: {
-            const double *x = (const double *)P.src;
-            return make_double2(x[T.ia + 2 * pos * S.sax], x[T.ia + (2 * pos + 1) * S.sax]);
-        }
+        case LD_C2C:
+            return gld_c(P.src, f32, T.ia + pos * S.sax);
+        case LD_REAL_HALF:
+            return make_double2(gld_r(P.src, f32, T.ia + 2 * pos * S.sax), gld_r(P.src, f32, T.ia + (2 * pos + 1) * S.sax));
         case LD_R2C_PAIR: {
-            const double *x = (const double *)P.src;
-            double a = x[T.ia + pos * S.sax];
-            double b = (T.flags & 2) ? x[T.ib + pos * S.sax] : 0.0;
+            double a = gld_r(P.src, f32, T.ia + pos * S.sax);
+            double b = (T.flags & 2) ? gld_r(P.src, f32, T.ib + pos * S.sax) : 0.0;
             return make_double2(a, b);
         }
         case LD_C2R_HALF: {
-            const double2 *x = (const double2 *)P.src;
             const long long M = P.M;
-            double2 xp = x[T.ia + pos * S.sax];
-            double2 xq = x[T.ia + (M - pos) * S.sax];
+            double2 xp = gld_c(P.src, f32, T.ia + pos * S.sax);
+            double2 xq = gld_c(P.src, f32, T.ia + (M - pos) * S.sax);
             if (P.fwd_flag) { xp = cconj(xp); xq = cconj(xq); }
             if (pos == 0) { xp.y = 0.0; xq.y = 0.0; }
             else if (P.adj_w) { xp = cscale(xp, 0.5); xq = cscale(xq, 0.5); }
@@ -95,12 +122,11 @@ __device__ __forceinline__ double2 load_point(const FftParams &P, const TfInfo &
             return make_double2(e.x - o.y, e.y + o.x);
         }
         default: {  // LD_C2R_PAIR
-            const double2 *x = (const double2 *)P.src;
             const long long L = P.M;
             const bool mirror = 2 * pos > L;
             const long long k = mirror ? L - pos : pos;
-            double2 a = x[T.ia + k * S.sax];
-            double2 b = (T.flags & 2) ? x[T.ib + k * S.sax] : make_double2(0.0, 0.0);
+            double2 a = gld_c(P.src, f32, T.ia + k * S.sax);
+            double2 b = (T.flags & 2) ? gld_c(P.src, f32, T.ib + k * S.sax) : make_double2(0.0, 0.0);
             if (P.fwd_flag != (int)mirror) { a = cconj(a); b = cconj(b); }
             if (pos == 0) { a.y = 0.0; b.y = 0.0; }
             else if (P.adj_w) { a = cscale(a, 0.5); b = cscale(b, 0.5); }
@@ -214,29 +240,27 @@ __device__ __forceinline__ void store_value(const FftParams &P, const TfInfo &T,
     switch (P.store_mode) {
         case ST_TWIDDLE: {
             double2 w = tw_sign(twolevel(P.tw_lo, P.tw_hi, P.tw_logS, (long long)P.tw_mul * T.j * n), (double)P.sign);
-            ((double2 *)P.dst)[T.oa + pos * S.sax] = cmul(z, w);
+            gst_c(P.dst, P.dst32, T.oa + pos * S.sax, cmul(z, w));
             return;
         }
         case ST_MUL_TABLE:
-            ((double2 *)P.dst)[T.oa + pos * S.sax] = cmul(z, P.mtab[pos]);
+            gst_c(P.dst, P.dst32, T.oa + pos * S.sax, cmul(z, P.mtab[pos]));
             return;
         case ST_C2R_HALF: {
             if (pos >= P.olim) return;
-            double *y = (double *)P.dst;
-            y[T.oa + 2 * pos * S.sax] = z.x * P.fct;
-            y[T.oa + (2 * pos + 1) * S.sax] = z.y * P.fct;
+            gst_r(P.dst, P.dst32, T.oa + 2 * pos * S.sax, z.x * P.fct);
+            gst_r(P.dst, P.dst32, T.oa + (2 * pos + 1) * S.sax, z.y * P.fct);
             return;
         }
         case ST_C2R_PAIR: {
             if (pos >= P.olim) return;
-            double *y = (double *)P.dst;
-            y[T.oa + pos * S.sax] = z.x * P.fct;
-            if (T.flags & 4) y[T.ob + pos * S.sax] = z.y * P.fct;
+            gst_r(P.dst, P.dst32, T.oa + pos * S.sax, z.x * P.fct);
+            if (T.flags & 4) gst_r(P.dst, P.dst32, T.ob + pos * S.sax, z.y * P.fct);
             return;
         }
         default:  // ST_C2C
             if (pos >= P.olim) return;
-            ((double2 *)P.dst)[T.oa + pos * S.sax] = cscale(z, P.fct);
+            gst_c(P.dst, P.dst32, T.oa + pos * S.sax, cscale(z, P.fct));
             return;
     }
 }

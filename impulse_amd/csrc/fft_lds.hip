@@ -82,7 +82,7 @@ __device__ void run_stages(double2 *buf, const FftParams &P, int i, double s)
 
 __device__ __forceinline__ void put_c(const FftParams &P, long long off, double2 y)
 {
-    ((double2 *)P.dst)[off] = y;
+    gst_c(P.dst, P.dst32, off, y);
 }
 
 // store local point n of transform t; `buf` is the transform's LDS row
@@ -131,18 +131,16 @@ __device__ __forceinline__ void store_point(const FftParams &P, const TfInfo &T,
         }
         case ST_C2R_HALF: {
             if (pos >= P.olim) return;
-            double *y = (double *)P.dst;
             double2 z = buf[n];
-            y[T.oa + 2 * pos * S.sax] = z.x * P.fct;
-            y[T.oa + (2 * pos + 1) * S.sax] = z.y * P.fct;
+            gst_r(P.dst, P.dst32, T.oa + 2 * pos * S.sax, z.x * P.fct);
+            gst_r(P.dst, P.dst32, T.oa + (2 * pos + 1) * S.sax, z.y * P.fct);
             return;
         }
         default: {  // ST_C2R_PAIR
             if (pos >= P.olim) return;
-            double *y = (double *)P.dst;
             double2 z = buf[n];
-            y[T.oa + pos * S.sax] = z.x * P.fct;
-            if (T.flags & 4) y[T.ob + pos * S.sax] = z.y * P.fct;
+            gst_r(P.dst, P.dst32, T.oa + pos * S.sax, z.x * P.fct);
+            if (T.flags & 4) gst_r(P.dst, P.dst32, T.ob + pos * S.sax, z.y * P.fct);
             return;
         }
     }
@@ -240,7 +238,6 @@ __global__ void __launch_bounds__(256) k_real_post(const RealPostParams P)
         long long oa, ob;
         int hb;
         side_base(P.out, (unsigned)u, oa, ob, hb);
-        double2 *y = (double2 *)P.dst;
         const double2 zk = Z[k], zm = Z[km];
         if (P.half) {
             // X[k] = (zk + zm*)/2 - i/2 W^k (zk - zm*),  W = W_{2M}; X[M-k] from the
@@ -255,8 +252,8 @@ __global__ void __launch_bounds__(256) k_real_post(const RealPostParams P)
             const double2 o2 = make_double2(0.5 * d2.y, -0.5 * d2.x);
             double2 Xm = cscale(cadd(e2, cmul(o2, make_double2(-w.x, w.y))), fm);
             if (!P.fwd_flag) { Xk = cconj(Xk); Xm = cconj(Xm); }
-            y[oa + k * P.out.sax] = Xk;
-            if (L - k != k) y[oa + (L - k) * P.out.sax] = Xm;
+            gst_c(P.dst, P.dst32, oa + k * P.out.sax, Xk);
+            if (L - k != k) gst_c(P.dst, P.dst32, oa + (L - k) * P.out.sax, Xm);
         } else {
             // odd pair: Xa = (zk + zm*)/2, Xb = -i/2 (zk - zm*) for k = 0 .. L/2
             const double2 za = cconj(zm);
@@ -265,8 +262,8 @@ __global__ void __launch_bounds__(256) k_real_post(const RealPostParams P)
             const double2 d = csub(zk, za);
             double2 xb = make_double2(0.5 * f * d.y, -0.5 * f * d.x);
             if (!P.fwd_flag) { xa = cconj(xa); xb = cconj(xb); }
-            y[oa + k * P.out.sax] = xa;
-            if (hb) y[ob + k * P.out.sax] = xb;
+            gst_c(P.dst, P.dst32, oa + k * P.out.sax, xa);
+            if (hb) gst_c(P.dst, P.dst32, ob + k * P.out.sax, xb);
         }
     }
 }

@@ -8,7 +8,8 @@
 // index j = g % ni and an outer index u = g / ni.  Element n of transform g
 // sits at signal position   pos = j*ipos + n*pst   of a signal whose element 0
 // is at   base(u) + j*isig ;  its address is   base + pos*sax   in units of the
-// array's scalar (double for real arrays, double2 for complex arrays).
+// array's scalar (double or float for real arrays, double2 or float2 for
+// complex arrays).
 // base(u) comes from one of three "side" layouts:
 //   SIDE_DIMS      : u is a physical batch index decomposed over nbd dims
 //   SIDE_DIMS_PAIR : u covers physical rows 2u and 2u+1 (row 2u+1 may not exist)
@@ -87,6 +88,8 @@ struct FftParams {
     long long ilim;            // input positions >= ilim load as zero (Bluestein zero padding)
     long long olim;            // output positions >= olim are not stored
     int M;                     // half length (real-half modes) or odd length (pair modes)
+    int src32, dst32;          // the source / destination array holds float (complex: float2)
+                               // rather than double; arithmetic stays FP64 (workspaces are FP64)
     int nload;                 // local points loaded per transform (the rest of the row is zero)
     int nstore;                // points stored per transform
     double fct;
@@ -122,6 +125,7 @@ struct RealPostParams {
     int adj_w;                 // interior bins x2 (adjoint of c2r)
     int M;                     // M (half) or L (pair)
     int fwd_flag;
+    int dst32;                 // the destination holds complex float rather than complex double
     double fct;
     long long ntf;             // logical signals
     long long zlen;            // Z stride per logical signal

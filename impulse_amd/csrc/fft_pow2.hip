@@ -109,10 +109,9 @@ __device__ __forceinline__ void real_split_store(const FftParams &P, const TfInf
         Xk = cconj(Xk);
         Xm = cconj(Xm);
     }
-    double2 *y = (double2 *)P.dst + U.oa;
     const long long sax = P.out.sax;
-    y[k * sax] = Xk;
-    if (km != k) y[km * sax] = Xm;
+    gst_c(P.dst, P.dst32, U.oa + k * sax, Xk);
+    if (km != k) gst_c(P.dst, P.dst32, U.oa + km * sax, Xm);
 }
 
 template <int R>
@@ -551,7 +550,7 @@ bool pow2_fast(const FftParams &P)
     const long long imax = (P.ni - 1) * P.in.ipos + (long long)(P.N - 1) * P.in.pst;
     const long long omax = (P.ni - 1) * P.out.ipos + (long long)(P.N - 1) * P.out.pst;
     const long long win = 0x7fffffffLL;   // buffer windows are addressed with 31-bit byte offsets
-    return P.load_mode == LD_C2C && !P.conj_in && !P.chirp_in && imax < P.ilim &&
+    return P.load_mode == LD_C2C && !P.src32 && !P.dst32 && !P.conj_in && !P.chirp_in && imax < P.ilim &&
            (P.store_mode == ST_C2C || (P.store_mode == ST_TWIDDLE && P.tw_mul == 1)) && !P.chirp_out &&
            !P.conj_out && omax < P.olim && P.src_span > 0 && P.src_span <= win && P.dst_span > 0 &&
            P.dst_span <= win;
@@ -562,7 +561,7 @@ bool pow2_fast(const FftParams &P)
 bool pow2_fast_load(const FftParams &P)
 {
     const long long imax = (P.ni - 1) * P.in.ipos + (long long)(P.N - 1) * P.in.pst;
-    return P.load_mode == LD_C2C && !P.conj_in && !P.chirp_in && imax < P.ilim && P.src_span > 0 &&
+    return P.load_mode == LD_C2C && !P.src32 && !P.conj_in && !P.chirp_in && imax < P.ilim && P.src_span > 0 &&
            P.src_span <= 0x7fffffffLL;
 }
 

@@ -236,6 +236,15 @@ struct Builder {
         return b;
     }
 
+    // float32 plans keep the user's arrays in float / complex float; the
+    // workspaces and tables stay FP64
+    bool user32(BufRef r) const { return plan.dtype == 1 && (r == REF_IN || r == REF_OUT); }
+    void io_flags(FftParams &P, BufRef src, BufRef dst) const
+    {
+        P.src32 = user32(src) ? 1 : 0;
+        P.dst32 = user32(dst) ? 1 : 0;
+    }
+
     const double2 *table_direct(long long n)   // W_n^j, j < n
     {
         std::vector<double2> v;
@@ -386,6 +395,7 @@ struct Builder {
     // consecutive points on the load / store side
     bool add_fft(FftParams P, BufRef src, BufRef dst, bool uses_fct, bool colL, bool colS)
     {
+        io_flags(P, src, dst);
         if (try_pow2(P, src, dst, uses_fct, colL, colS)) return true;
         const long long per = (long long)P.LD * (long long)sizeof(double2) * P.TB;
         const long long cap = max_lds_bytes() - (long long)sizeof(long long) * 8 * kMaxTB - 1024;
@@ -415,7 +425,13 @@ struct Builder {
     }
 
     // bytes one launch reads (input) or writes (output) if each byte moves once
+    // (float32 arrays move half the bytes of the FP64 figures below)
     static size_t side_bytes(const FftParams &P, bool input)
+    {
+        const size_t b = side_bytes64(P, input);
+        return (input ? P.src32 : P.dst32) ? b / 2 : b;
+    }
+    static size_t side_bytes64(const FftParams &P, bool input)
     {
         long long n;   // points per logical transform on that side
         if (input) {
@@ -839,6 +855,7 @@ struct Builder {
 
     bool add_conv_rows(FftParams P, BufRef src, BufRef dst)
     {
+        io_flags(P, src, dst);
         const int map = 0;
         const long long tb = pow2_tb_host(P.N, map);
         const long long per = (long long)pow2_lds_bytes(P.N, map);
@@ -884,6 +901,7 @@ struct Builder {
 
     bool add_rowpair(FftParams P, BufRef src, BufRef dst, bool uses_fct, long long signals, long long Ls)
     {
+        io_flags(P, src, dst);
         const int map = 2;
         const long long tb = pow2_tb_host(P.N, map);
         const long long per = (long long)pow2_lds_bytes(P.N, map);
@@ -911,7 +929,7 @@ struct Builder {
         ps.uses_fct = uses_fct;
         ps.name = "k_fft_pow2";
         ps.bytes_in = (size_t)(P.ntf * P.N * 16);
-        ps.bytes_out = (size_t)(signals * (Ls + 1) * 16);
+        ps.bytes_out = (size_t)(signals * (Ls + 1) * (P.dst32 ? 8 : 16));
         plan.passes.push_back(ps);
         return true;
     }
@@ -927,6 +945,7 @@ struct Builder {
         R.M = (int)Ls;
         R.fwd_flag = J.forward;
         R.adj_w = J.adj;
+        R.dst32 = user32(J.dst) ? 1 : 0;
         R.fct = 1.0;
         R.ntf = cn;
         R.zlen = zlen;
@@ -943,7 +962,7 @@ struct Builder {
         ps.name = "k_real_post";
         const long long per = pair ? Ls / 2 + 1 : Ls + 1;
         ps.bytes_in = (size_t)(R.ntf * Ls * 16);
-        ps.bytes_out = (size_t)(R.ntf * per * (pair ? 32 : 16));
+        ps.bytes_out = (size_t)(R.ntf * per * (pair ? 32 : 16)) / (R.dst32 ? 2 : 1);
         plan.passes.push_back(ps);
         return true;
     }
@@ -1045,7 +1064,7 @@ int plan_create(std::unique_ptr<Plan> &out, int kind, int dtype, size_t ndim, co
     int adj = 0;
     if (kind == 3) { kind = J_C2R; forward = !forward; adj = 1; }
     else if (kind == 4) { kind = J_R2C; forward = !forward; adj = 1; }
-    if (dtype != 0) { err = {E_UNSUPPORTED, "only IMPULSE_FFT_F64 is supported"}; return E_UNSUPPORTED; }
+    if (dtype != 0 && dtype != 1) { err = {E_INVALID, "unknown dtype"}; return E_INVALID; }
     if (ndim < 1 || !shape) { err = {E_INVALID, "ndim must be >= 1"}; return E_INVALID; }
     if (!stride_in || !stride_out) { err = {E_INVALID, "stride dimension mismatch"}; return E_INVALID; }
     if (naxes < 1 || !axes) { err = {E_INVALID, "no axes given"}; return E_INVALID; }
@@ -1070,7 +1089,8 @@ int plan_create(std::unique_ptr<Plan> &out, int kind, int dtype, size_t ndim, co
     for (size_t d = 0; d < ndim; ++d)
         if (shape[d] == 0) P.empty = true;
 
-    const ptrdiff_t in_elem = (kind == J_R2C) ? 8 : 16, out_elem = (kind == J_C2R) ? 8 : 16;
+    const ptrdiff_t es = dtype == 1 ? 4 : 8;   // bytes of one real scalar of the user's arrays
+    const ptrdiff_t in_elem = (kind == J_R2C) ? es : 2 * es, out_elem = (kind == J_C2R) ? es : 2 * es;
     const size_t last = axes[naxes - 1];
     std::vector<size_t> shape_c(P.shape);   // complex-side shape for r2c / c2r
     if (kind != J_C2C) shape_c[last] = P.shape[last] / 2 + 1;
@@ -1177,7 +1197,8 @@ int plan_execute(Plan &P, const void *in, void *out, double fct, hipStream_t str
         return E_INVALID;
     }
     if (in == out && P.sin != P.sout) { err = {E_INVALID, "stride mismatch"}; return E_INVALID; }
-    const uintptr_t ain = (P.kind == J_R2C) ? 8 : 16, aout = (P.kind == J_C2R) ? 8 : 16;
+    const uintptr_t es = P.dtype == 1 ? 4 : 8;
+    const uintptr_t ain = (P.kind == J_R2C) ? es : 2 * es, aout = (P.kind == J_C2R) ? es : 2 * es;
     if (((uintptr_t)in % ain) || ((uintptr_t)out % aout)) {
         err = {E_UNSUPPORTED, "data pointers must be aligned to the element size"};
         return E_UNSUPPORTED;
@@ -1214,7 +1235,7 @@ int plan_execute(Plan &P, const void *in, void *out, double fct, hipStream_t str
         if (prof) (void)hipEventRecord(ev[2 * pi], stream);
         if (ps.kind == Pass::FFT_LDS || ps.kind == Pass::FFT_POW2) {
             const void *srcp = resolve(ps.src);
-            FftParams fp = (ps.has_alt && ((uintptr_t)srcp % 16) == 0) ? ps.fp_alt : ps.fp;
+            FftParams fp = (ps.has_alt && ((uintptr_t)srcp % (ps.fp_alt.src32 ? 8 : 16)) == 0) ? ps.fp_alt : ps.fp;
             fp.src = srcp;
             fp.dst = resolve(ps.dst);
             window(ps.src, fp.src_lo, fp.src_span);
@@ -1285,13 +1306,122 @@ int plan_profile_read(Plan &P, double *pass_ms, int npass, int *executes, Error 
     return OK;
 }
 
+// ---------------------------------------------------------------------------
+// host staging
+// ---------------------------------------------------------------------------
+
+HostStage::~HostStage()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (int b = 0; b < 2; ++b) {
+        if (ev[b]) (void)hipEventDestroy(ev[b]);
+        if (pin[b]) (void)hipHostFree(pin[b]);
+    }
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+static int stage_fail(Error &err, hipError_t e, const char *what)
+{
+    (void)hipGetLastError();
+    err = {e == hipErrorOutOfMemory ? E_NOMEM : E_HIP, std::string(what) + ": " + hipGetErrorString(e)};
+    return err.code;
+}
+
+int HostStage::init(Error &err)
+{
+    if (stream) return OK;
+    // chunk of the pinned ring (IMPULSE_FFT_STAGE_KB, default 4 MiB)
+    const size_t ch = (size_t)env_ll("IMPULSE_FFT_STAGE_KB", 4096) * 1024;
+    hipError_t e;
+    for (int b = 0; b < 2; ++b) {
+        if ((e = hipHostMalloc(&pin[b], ch, hipHostMallocDefault)) != hipSuccess) {
+            pin[b] = nullptr;
+            return stage_fail(err, e, "pinned staging allocation failed");
+        }
+        if ((e = hipEventCreateWithFlags(&ev[b], hipEventDisableTiming)) != hipSuccess) {
+            ev[b] = nullptr;
+            return stage_fail(err, e, "hipEventCreate failed");
+        }
+    }
+    if ((e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) != hipSuccess) {
+        stream = nullptr;
+        return stage_fail(err, e, "hipStreamCreate failed");
+    }
+    chunk = ch;
+    return OK;
+}
+
+int HostStage::to_device(void *dev, const void *host, size_t n, Error &err)
+{
+    hipError_t e;
+    for (size_t off = 0, c = 0; off < n; off += chunk, ++c) {
+        const int b = (int)(c & 1);
+        const size_t len = std::min(chunk, n - off);
+        if (pending[b] && (e = hipEventSynchronize(ev[b])) != hipSuccess) return stage_fail(err, e, "staging copy failed");
+        pending[b] = false;
+        std::memcpy(pin[b], (const char *)host + off, len);
+        if ((e = hipMemcpyAsync((char *)dev + off, pin[b], len, hipMemcpyHostToDevice, stream)) != hipSuccess ||
+            (e = hipEventRecord(ev[b], stream)) != hipSuccess)
+            return stage_fail(err, e, "staging copy failed");
+        pending[b] = true;
+    }
+    return OK;
+}
+
+int HostStage::to_host(void *host, const void *dev, size_t n, Error &err)
+{
+    hipError_t e;
+    const size_t nch = (n + chunk - 1) / chunk;
+    auto issue = [&](size_t c) -> hipError_t {
+        const int b = (int)(c & 1);
+        hipError_t r;
+        if (pending[b] && (r = hipEventSynchronize(ev[b])) != hipSuccess) return r;
+        const size_t off = c * chunk, len = std::min(chunk, n - off);
+        if ((r = hipMemcpyAsync(pin[b], (const char *)dev + off, len, hipMemcpyDeviceToHost, stream)) != hipSuccess)
+            return r;
+        if ((r = hipEventRecord(ev[b], stream)) != hipSuccess) return r;
+        pending[b] = true;
+        return hipSuccess;
+    };
+    for (size_t c = 0; c < nch && c < 2; ++c)
+        if ((e = issue(c)) != hipSuccess) return stage_fail(err, e, "staging copy failed");
+    for (size_t c = 0; c < nch; ++c) {
+        const int b = (int)(c & 1);
+        if ((e = hipEventSynchronize(ev[b])) != hipSuccess) return stage_fail(err, e, "staging copy failed");
+        pending[b] = false;
+        const size_t off = c * chunk;
+        std::memcpy((char *)host + off, pin[b], std::min(chunk, n - off));
+        if (c + 2 < nch && (e = issue(c + 2)) != hipSuccess) return stage_fail(err, e, "staging copy failed");
+    }
+    return OK;
+}
+
+int HostStage::sync(Error &err)
+{
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return stage_fail(err, e, "stream synchronisation failed");
+    pending[0] = pending[1] = false;
+    return OK;
+}
+
+// The same argument rules as plan_execute (pocketfft_hdronly.h:446-477) are
+// applied before anything is staged: in-place calls need equal strides (and
+// c2c for the complex kinds), so that the input and output windows coincide and
+// one staging buffer with one base serves both.
 int plan_execute_host(Plan &P, const void *in, void *out, double fct, Error &err)
 {
     if (P.empty) return OK;
     if (!in || !out) { err = {E_INVALID, "null data pointer"}; return E_INVALID; }
-    std::lock_guard<std::mutex> lk(P.host_mtx);
-    const size_t nin = (size_t)(P.in_hi - P.in_lo), nout = (size_t)(P.out_hi - P.out_lo);
     const bool inplace = in == out;
+    if (inplace) {
+        if (!P.r2r && !P.rfp && P.kind != J_C2C) { err = {E_INVALID, "in-place r2c/c2r is not supported"}; return E_INVALID; }
+        if (P.sin != P.sout) { err = {E_INVALID, "stride mismatch"}; return E_INVALID; }
+        if (P.in_lo != P.out_lo || P.in_hi != P.out_hi) { err = {E_INVALID, "stride mismatch"}; return E_INVALID; }
+    }
+    std::lock_guard<std::mutex> lk(P.host_mtx);
+    int rc = P.host.init(err);
+    if (rc != OK) return rc;
+    const size_t nin = (size_t)(P.in_hi - P.in_lo), nout = (size_t)(P.out_hi - P.out_lo);
     if (P.stage_in.bytes < nin && P.stage_in.alloc(nin) != OK) { err = {E_NOMEM, "staging allocation failed"}; return E_NOMEM; }
     if (!inplace && P.stage_out.bytes < nout && P.stage_out.alloc(nout) != OK) {
         err = {E_NOMEM, "staging allocation failed"};
@@ -1300,21 +1430,15 @@ int plan_execute_host(Plan &P, const void *in, void *out, double fct, Error &err
     char *din = (char *)P.stage_in.p, *dout = inplace ? din : (char *)P.stage_out.p;
     const char *hin = (const char *)in + P.in_lo;
     char *hout = (char *)out + P.out_lo;
-    if (hipMemcpy(din, hin, nin, hipMemcpyHostToDevice) != hipSuccess) goto hip_fail;
-    if (!inplace && !P.out_dense && hipMemcpy(dout, hout, nout, hipMemcpyHostToDevice) != hipSuccess) goto hip_fail;
-    {
-        int rc = plan_execute(P, din - P.in_lo, dout - P.out_lo, fct, nullptr, err);
-        if (rc != OK) return rc;
+    if ((rc = P.host.to_device(din, hin, nin, err)) != OK) return rc;
+    // an output with gaps keeps the bytes the transform does not write
+    if (!inplace && !P.out_dense && (rc = P.host.to_device(dout, hout, nout, err)) != OK) return rc;
+    if ((rc = plan_execute(P, din - P.in_lo, dout - P.out_lo, fct, P.host.stream, err)) != OK) {
+        Error e2;
+        (void)P.host.sync(e2);
+        return rc;
     }
-    if (hipDeviceSynchronize() != hipSuccess) goto hip_fail;
-    if (hipMemcpy(hout, dout, nout, hipMemcpyDeviceToHost) != hipSuccess) goto hip_fail;
-    return OK;
-hip_fail:
-    {
-        hipError_t e = hipGetLastError();
-        err = {E_HIP, std::string("HIP error during host execution: ") + hipGetErrorString(e)};
-    }
-    return E_HIP;
+    return P.host.to_host(hout, dout, nout, err);
 }
 
 }  // namespace ifft

@@ -20,8 +20,12 @@ surfaces so that tests read like the reference's own:
 
 Host arrays go through the host-pointer entry points (plans from the native
 LRU cache).  torch tensors on the GPU go through ``impulse_fft_execute`` on
-torch's current stream with plans from a per-process cache.
+torch's current stream with plans from a per-process LRU keyed by the request,
+the device and the stream (a plan's workspace is used by one stream at a time).
+Arrays of float64 / complex128 run as IMPULSE_FFT_F64, float32 / complex64 as
+IMPULSE_FFT_F32 (the reference's ``T = float32`` instantiation).
 """
+import collections
 import ctypes
 import enum
 import math
@@ -217,6 +221,15 @@ def _is_torch(x):
     return type(x).__module__.startswith("torch")
 
 
+def _dtype_of(buf):
+    name = str(buf.dtype).replace("torch.", "")
+    if name in ("float64", "complex128"):
+        return _lib.F64
+    if name in ("float32", "complex64"):
+        return _lib.F32
+    raise TypeError(f"unsupported element type {name}: the FFT takes float64/complex128 or float32/complex64")
+
+
 class DataDesc:
     """Shape, BYTE strides and buffer of one array (cpp_pocketfft/pocketfft.nim:158-199).
 
@@ -229,6 +242,9 @@ class DataDesc:
         if buf is None:
             raise ValueError("buffer is nil")
         shape = [int(s) for s in shape]
+        if _is_torch(buf) and (buf.is_conj() or buf.is_neg()):
+            # a lazy view: its memory holds the unconjugated / unnegated values
+            raise ValueError("DataDesc takes a tensor's memory as it is: call resolve_conj() / resolve_neg() first")
         itemsize = buf.element_size() if _is_torch(buf) else buf.dtype.itemsize
         if stride is None:
             acc, st = itemsize, [0] * len(shape)
@@ -250,6 +266,11 @@ class DataDesc:
         if _is_torch(self.buf):
             return self.buf.is_complex()
         return np.iscomplexobj(self.buf)
+
+    @property
+    def dtype(self):
+        """IMPULSE_FFT_F64 for double-based buffers, IMPULSE_FFT_F32 for float-based ones."""
+        return _dtype_of(self.buf)
 
     def ptr(self):
         return self.buf.data_ptr() if _is_torch(self.buf) else self.buf.ctypes.data
@@ -291,7 +312,8 @@ class DCTDesc:
 
 
 class Plan:
-    """An n-D plan (impulse_fft_plan_create) for device execution."""
+    """An n-D plan (impulse_fft_plan_create) for device execution.  Calls of one
+    plan share its workspace, so they must not overlap on different streams."""
 
     def __init__(self, kind, shape, stride_in, stride_out, axes, forward, dtype=_lib.F64, r2r_type=2, ortho=False):
         h = ctypes.c_void_p()
@@ -327,25 +349,45 @@ class Plan:
         return lib().impulse_fft_plan_device_bytes(self.handle)
 
 
-_plan_cache = {}
+PLAN_CACHE_CAPACITY = 16
+_plan_cache = collections.OrderedDict()
 _plan_lock = threading.Lock()
 
 
-def get_plan(kind, shape, stride_in, stride_out, axes, forward, r2r_type=2, ortho=False):
-    key = (kind, tuple(shape), tuple(stride_in), tuple(stride_out), tuple(axes), bool(forward), int(r2r_type),
-           bool(ortho))
+def get_plan(kind, shape, stride_in, stride_out, axes, forward, r2r_type=2, ortho=False, dtype=_lib.F64,
+             device=None, stream=None):
+    """Plan for a device request, from an LRU of PLAN_CACHE_CAPACITY entries keyed
+    by the request, the device and the stream (one workspace per stream)."""
+    key = (kind, int(dtype), tuple(shape), tuple(stride_in), tuple(stride_out), tuple(axes), bool(forward),
+           int(r2r_type), bool(ortho), device, stream)
     with _plan_lock:
         p = _plan_cache.get(key)
-    if p is None:
-        p = Plan(kind, shape, stride_in, stride_out, axes, forward, r2r_type=r2r_type, ortho=ortho)
-        with _plan_lock:
-            _plan_cache[key] = p
+        if p is not None:
+            _plan_cache.move_to_end(key)
+            return p
+    p = Plan(kind, shape, stride_in, stride_out, axes, forward, dtype=dtype, r2r_type=r2r_type, ortho=ortho)
+    with _plan_lock:
+        q = _plan_cache.get(key)
+        if q is not None:      # another thread built it meanwhile
+            _plan_cache.move_to_end(key)
+            return q
+        _plan_cache[key] = p
+        while len(_plan_cache) > PLAN_CACHE_CAPACITY:
+            _plan_cache.popitem(last=False)
     return p
 
 
 def _current_stream():
     import torch
     return torch.cuda.current_stream().cuda_stream
+
+
+def _device_plan(kind, shape, descIn, descOut, axes, forward, dtype, r2r_type=2, ortho=False):
+    import torch
+    dev = descIn.buf.device.index if descIn.buf.device.index is not None else torch.cuda.current_device()
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    p = get_plan(kind, shape, descIn.stride, descOut.stride, axes, forward, r2r_type, ortho, dtype, dev, stream)
+    return p, stream
 
 
 def apply(fft, descOut, descIn):
@@ -357,8 +399,9 @@ def apply(fft, descOut, descIn):
             raise TypeError("DCT / DST take real arrays")
         kind = _lib.DST if fft.sine else _lib.DCT
         if _is_torch(descIn.buf) and descIn.buf.is_cuda:
-            p = get_plan(kind, descIn.shape, descIn.stride, descOut.stride, fft.axes, True, fft.dctType, fft.ortho)
-            p.execute(descIn.ptr(), descOut.ptr(), fft.scalingFactor, _current_stream())
+            p, stream = _device_plan(kind, descIn.shape, descIn, descOut, fft.axes, True, _lib.F64, fft.dctType,
+                                     fft.ortho)
+            p.execute(descIn.ptr(), descOut.ptr(), fft.scalingFactor, stream)
             return
         fn = lib().impulse_fft_dst if fft.sine else lib().impulse_fft_dct
         check(fn(len(descIn.shape), sizes(descIn.shape), strides(descIn.stride), strides(descOut.stride),
@@ -373,13 +416,16 @@ def apply(fft, descOut, descIn):
         kind, shape = _lib.C2R, descOut.shape
     else:
         raise TypeError("Not implemented")
+    dtype = descIn.dtype
+    if descOut.dtype != dtype:
+        raise TypeError("input and output must have the same precision (float64/complex128 or float32/complex64)")
     if _is_torch(descIn.buf) and descIn.buf.is_cuda:
-        p = get_plan(kind, shape, descIn.stride, descOut.stride, fft.axes, fft.forward)
-        p.execute(descIn.ptr(), descOut.ptr(), fft.scalingFactor, _current_stream())
+        p, stream = _device_plan(kind, shape, descIn, descOut, fft.axes, fft.forward, dtype)
+        p.execute(descIn.ptr(), descOut.ptr(), fft.scalingFactor, stream)
         return
     fn = {_lib.C2C: lib().impulse_fft_c2c, _lib.R2C: lib().impulse_fft_r2c, _lib.C2R: lib().impulse_fft_c2r}[kind]
     check(fn(len(shape), sizes(shape), strides(descIn.stride), strides(descOut.stride), len(fft.axes),
-             sizes(fft.axes), 1 if fft.forward else 0, _lib.F64, descIn.ptr(), descOut.ptr(),
+             sizes(fft.axes), 1 if fft.forward else 0, dtype, descIn.ptr(), descOut.ptr(),
              fft.scalingFactor, fft.nthreads))
 
 
@@ -387,44 +433,53 @@ def apply(fft, descOut, descIn):
 # pocketfft-style n-D helpers on numpy arrays (byte strides taken from numpy)
 # ---------------------------------------------------------------------------
 
+_COMPLEX_OF = {np.dtype(np.float64): np.complex128, np.dtype(np.float32): np.complex64}
+_REAL_OF = {np.dtype(np.complex128): np.float64, np.dtype(np.complex64): np.float32}
+_DT = {np.dtype(np.float64): _lib.F64, np.dtype(np.complex128): _lib.F64, np.dtype(np.float32): _lib.F32,
+       np.dtype(np.complex64): _lib.F32}
+
+
 def c2c(x, axes, forward=True, fct=1.0, out=None):
+    """pocketfft::c2c on a complex128 or complex64 numpy array (any byte strides)."""
     x = np.asarray(x)
-    if x.dtype != np.complex128:
-        raise TypeError("c2c needs complex128")
+    if x.dtype not in _REAL_OF:
+        raise TypeError("c2c needs complex128 or complex64")
     if out is None:
-        out = np.empty(x.shape, np.complex128)
+        out = np.empty(x.shape, x.dtype)
     check(lib().impulse_fft_c2c(x.ndim, sizes(x.shape), strides(x.strides), strides(out.strides), len(axes),
-                                sizes(axes), 1 if forward else 0, _lib.F64, x.ctypes.data, out.ctypes.data,
+                                sizes(axes), 1 if forward else 0, _DT[x.dtype], x.ctypes.data, out.ctypes.data,
                                 float(fct), 1))
     return out
 
 
 def r2c(x, axes, forward=True, fct=1.0, out=None):
+    """pocketfft::r2c on a float64 or float32 numpy array (any byte strides)."""
     x = np.asarray(x)
-    if x.dtype != np.float64:
-        raise TypeError("r2c needs float64")
+    if x.dtype not in _COMPLEX_OF:
+        raise TypeError("r2c needs float64 or float32")
     axes = list(axes)
     oshape = list(x.shape)
     oshape[axes[-1]] = x.shape[axes[-1]] // 2 + 1
     if out is None:
-        out = np.empty(oshape, np.complex128)
+        out = np.empty(oshape, _COMPLEX_OF[x.dtype])
     check(lib().impulse_fft_r2c(x.ndim, sizes(x.shape), strides(x.strides), strides(out.strides), len(axes),
-                                sizes(axes), 1 if forward else 0, _lib.F64, x.ctypes.data, out.ctypes.data,
+                                sizes(axes), 1 if forward else 0, _DT[x.dtype], x.ctypes.data, out.ctypes.data,
                                 float(fct), 1))
     return out
 
 
 def c2r(x, n, axes, forward=False, fct=1.0, out=None):
+    """pocketfft::c2r on a complex128 or complex64 numpy array; ``n`` is the real length along axes[-1]."""
     x = np.asarray(x)
-    if x.dtype != np.complex128:
-        raise TypeError("c2r needs complex128")
+    if x.dtype not in _REAL_OF:
+        raise TypeError("c2r needs complex128 or complex64")
     axes = list(axes)
     oshape = list(x.shape)
     oshape[axes[-1]] = int(n)
     if out is None:
-        out = np.empty(oshape, np.float64)
+        out = np.empty(oshape, _REAL_OF[x.dtype])
     check(lib().impulse_fft_c2r(len(oshape), sizes(oshape), strides(x.strides), strides(out.strides), len(axes),
-                                sizes(axes), 1 if forward else 0, _lib.F64, x.ctypes.data, out.ctypes.data,
+                                sizes(axes), 1 if forward else 0, _DT[x.dtype], x.ctypes.data, out.ctypes.data,
                                 float(fct), 1))
     return out
 

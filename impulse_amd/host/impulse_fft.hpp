@@ -294,25 +294,40 @@ template <class T> struct FFTDesc {
 
 template <class T> struct is_complex : std::false_type {};
 template <class T> struct is_complex<std::complex<T>> : std::true_type {};
+template <class T> struct scalar_of { using type = T; };
+template <class T> struct scalar_of<std::complex<T>> { using type = T; };
+
+// IMPULSE_FFT_F64 for double / complex<double>, IMPULSE_FFT_F32 for float /
+// complex<float> (the reference's `T: SomeFloat`, cpp_pocketfft/pocketfft.nim:35-69)
+template <class T> constexpr int dtype_of()
+{
+    using S = typename scalar_of<T>::type;
+    static_assert(std::is_same<S, double>::value || std::is_same<S, float>::value,
+                  "the FFT takes double / complex<double> or float / complex<float>");
+    return std::is_same<S, float>::value ? IMPULSE_FFT_F32 : IMPULSE_FFT_F64;
+}
 
 // c2c / r2c / c2r by element types.  c2r gets the real OUTPUT shape
-// (README_pocketfft.md:89-92).
+// (README_pocketfft.md:89-92).  Input and output share one precision; the
+// engine computes in FP64 for both.
 template <class F, class In, class Out> void apply(const FFTDesc<F> &fft, DataDesc<Out> &descOut, const DataDesc<In> &descIn)
 {
-    static_assert(std::is_same<F, double>::value, "the engine computes in FP64");
+    static_assert(std::is_same<typename scalar_of<In>::type, typename scalar_of<Out>::type>::value,
+                  "input and output must have the same precision");
+    constexpr int dt = dtype_of<In>();
     const double fct = (double)fft.scalingFactor;
     const int fwd = fft.forward ? 1 : 0;
     if constexpr (is_complex<In>::value && is_complex<Out>::value) {
         check_status(impulse_fft_c2c(descIn.shape.size(), descIn.shape.data(), descIn.stride.data(),
-                                     descOut.stride.data(), fft.axes.size(), fft.axes.data(), fwd, IMPULSE_FFT_F64,
+                                     descOut.stride.data(), fft.axes.size(), fft.axes.data(), fwd, dt,
                                      descIn.buf, descOut.buf, fct, fft.nthreads));
     } else if constexpr (is_complex<Out>::value) {
         check_status(impulse_fft_r2c(descIn.shape.size(), descIn.shape.data(), descIn.stride.data(),
-                                     descOut.stride.data(), fft.axes.size(), fft.axes.data(), fwd, IMPULSE_FFT_F64,
+                                     descOut.stride.data(), fft.axes.size(), fft.axes.data(), fwd, dt,
                                      descIn.buf, descOut.buf, fct, fft.nthreads));
     } else if constexpr (is_complex<In>::value) {
         check_status(impulse_fft_c2r(descOut.shape.size(), descOut.shape.data(), descIn.stride.data(),
-                                     descOut.stride.data(), fft.axes.size(), fft.axes.data(), fwd, IMPULSE_FFT_F64,
+                                     descOut.stride.data(), fft.axes.size(), fft.axes.data(), fwd, dt,
                                      descIn.buf, descOut.buf, fct, fft.nthreads));
     } else {
         static_assert(is_complex<In>::value || is_complex<Out>::value, "Not implemented");

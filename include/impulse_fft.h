@@ -19,7 +19,9 @@
  * shape[axis]/2+1 bins along the last entry of `axes`, r2c with forward=0
  * conjugates its output, c2r with forward=1 conjugates its input, and c2r
  * ignores the imaginary part of bin 0 (and of bin n/2 for even n).
- * Results are computed on the GPU in FP64 (IMPULSE_FFT_F64).
+ * Arrays hold double / complex<double> (IMPULSE_FFT_F64) or, for the c2c /
+ * r2c / c2r kinds, float / complex<float> (IMPULSE_FFT_F32, the reference's
+ * T = float instantiation); the arithmetic is FP64 on the GPU either way.
  */
 #ifndef IMPULSE_FFT_H
 #define IMPULSE_FFT_H
@@ -68,7 +70,9 @@ typedef enum impulse_fft_kind {
 
 typedef enum impulse_fft_dtype {
     IMPULSE_FFT_F64 = 0, /* double / interleaved complex<double> (Nim float64 / Complex64) */
-    IMPULSE_FFT_F32 = 1  /* float input/output; arithmetic stays FP64 (FFT convolution) */
+    IMPULSE_FFT_F32 = 1  /* float / interleaved complex<float> (Nim float32 / Complex32): the
+                            c2c / r2c / c2r plans and one-shots and the FFT convolution;
+                            inputs are widened to FP64, outputs rounded to nearest float */
 } impulse_fft_dtype;
 
 /* ------------------------------------------------------------------------- */
@@ -97,7 +101,10 @@ int impulse_fft_execute(impulse_fft_plan plan, const void *data_in, void *data_o
                         void *stream);
 
 /* Execute on HOST pointers: copies in, runs, copies out, synchronous.  This is
- * the drop-in for pocketfft's host-memory calls. */
+ * the drop-in for pocketfft's host-memory calls.  The copies go through a
+ * pinned staging ring of the plan, on a stream of the plan's own; concurrent
+ * host calls of one plan are serialised.  in == out needs equal strides and
+ * a C2C (or real-to-real) plan, as pocketfft's sanity_check requires. */
 int impulse_fft_execute_host(impulse_fft_plan plan, const void *data_in, void *data_out,
                              double fct);
 
@@ -203,10 +210,15 @@ int impulse_fft_fftpack_plan_create(impulse_fft_plan *plan, int kind, int real2h
                                     const ptrdiff_t *stride_in, const ptrdiff_t *stride_out,
                                     size_t naxes, const size_t *axes);
 
-/* Plan cache control (PocketFFT's is a fixed 16-entry LRU, pocketfft_hdronly.h:50-52). */
+/* Plan cache control (PocketFFT's is a fixed 16-entry LRU, pocketfft_hdronly.h:50-52).
+ * The one-shot calls and the C path's make_cfft_plan / make_rfft_plan take their
+ * device plans from it (the C path's real plans from a second LRU of the same
+ * capacity keyed by length).  cache_stats reports the plan lookups that found
+ * a plan (hits) and that had to build one (misses) since the last clear. */
 int impulse_fft_cache_set_capacity(size_t entries);
 size_t impulse_fft_cache_size(void);
 int impulse_fft_cache_clear(void);
+int impulse_fft_cache_stats(size_t *hits, size_t *misses);
 
 /* ------------------------------------------------------------------------- */
 /* FFT convolution with a backward pass (no PocketFFT counterpart)            */
@@ -347,7 +359,9 @@ typedef struct impulse_rfft_plan_s *impulse_rfft_plan;
  * cfft_length (c_pocketfft/pocketfft.h:18-23).  `c` holds `length` interleaved
  * complex doubles; the transform is done in place and scaled by `fct`.
  * make_* returns NULL on failure (length 0, no device, out of memory);
- * *_forward/_backward return 0 on success and -1 on failure, as PocketFFT. */
+ * *_forward/_backward return 0 on success and -1 on failure, as PocketFFT.
+ * The device plans come from the plan cache, so building and destroying a
+ * plan per call (as impulse's fft() does) costs a lookup after the first. */
 impulse_cfft_plan impulse_make_cfft_plan(size_t length);
 void impulse_destroy_cfft_plan(impulse_cfft_plan plan);
 int impulse_cfft_forward(impulse_cfft_plan plan, double c[], double fct);

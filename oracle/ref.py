@@ -32,7 +32,7 @@ def lib():
             f.argtypes = [ctypes.c_size_t, szp, pdp, pdp, ctypes.c_size_t, szp, ctypes.c_int,
                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_size_t]
             f.restype = ctypes.c_int
-        for name in ("ref_c2c_f32", "ref_r2c_f32"):
+        for name in ("ref_c2c_f32", "ref_r2c_f32", "ref_c2r_f32"):
             f = getattr(L, name)
             f.argtypes = [ctypes.c_size_t, szp, pdp, pdp, ctypes.c_size_t, szp, ctypes.c_int,
                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_size_t]
@@ -95,9 +95,11 @@ def c2r(x, n, axes, forward=False, fct=1.0, nthreads=1, out=None):
     axes = list(axes)
     oshape = list(x.shape)
     oshape[axes[-1]] = n
+    rdt = np.float64 if x.dtype == np.complex128 else np.float32
+    f = lib().ref_c2r_f64 if x.dtype == np.complex128 else lib().ref_c2r_f32
     if out is None:
-        out = np.empty(oshape, np.float64)
-    _call(lib().ref_c2r_f64, oshape, x, out, axes, forward, fct, nthreads)
+        out = np.empty(oshape, rdt)
+    _call(f, oshape, x, out, axes, forward, fct, nthreads)
     return out
 
 

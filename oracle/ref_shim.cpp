@@ -81,6 +81,18 @@ int ref_r2c_f32(size_t ndim, const size_t *shape_in, const ptrdiff_t *sin, const
     return 0;
 }
 
+int ref_c2r_f32(size_t ndim, const size_t *shape_out, const ptrdiff_t *sin, const ptrdiff_t *sout,
+                size_t naxes, const size_t *axes, int forward, const float *in, float *out,
+                float fct, size_t nthreads)
+{
+    try {
+        pocketfft::c2r<float>(mk_shape(ndim, shape_out), mk_stride(ndim, sin), mk_stride(ndim, sout),
+                              mk_shape(naxes, axes), forward != 0,
+                              reinterpret_cast<const std::complex<float> *>(in), out, fct, nthreads);
+    } catch (...) { return 1; }
+    return 0;
+}
+
 int ref_dct_f64(size_t ndim, const size_t *shape, const ptrdiff_t *sin, const ptrdiff_t *sout, size_t naxes,
                 const size_t *axes, int type, const double *in, double *out, double fct, int ortho, size_t nthreads)
 {

@@ -140,6 +140,30 @@ int main(int argc, char **argv)
         apply(inv, backDesc, dOutDesc);
         CHECK(errcalc(back.data(), dIn.data(), 5) < 1e-15);
     }
+    // the same README example through FFTDesc[float32] / DataDesc[float32] (T = float):
+    // float inputs, complex<float> outputs, values rounded from the FP64 result
+    {
+        const std::vector<float> dIn = {1.0f, 2.0f, 1.0f, -1.0f, 1.5f};
+        std::vector<std::complex<float>> dOut(dIn.size());
+        auto dInDesc = DataDesc<float>::init(const_cast<float *>(dIn.data()), std::vector<size_t>{dIn.size()});
+        auto dOutDesc = DataDesc<std::complex<float>>::init(dOut.data(), std::vector<size_t>{dOut.size()});
+        apply(FFTDesc<float>::init({0}, true), dOutDesc, dInDesc);
+        const std::complex<float> e1((float)2.081559480312316, (float)-1.651098762732523),
+            e2((float)-1.831559480312316, (float)1.608220406444071);
+        CHECK(dOut[0] == std::complex<float>(4.5f, 0.0f) && dOut[1] == e1 && dOut[2] == e2);
+        CHECK(dOut[3] == std::complex<float>(0, 0) && dOut[4] == std::complex<float>(0, 0));
+        std::vector<float> back(dIn.size());
+        auto backDesc = DataDesc<float>::init(back.data(), std::vector<size_t>{back.size()});
+        apply(FFTDesc<float>::init({0}, false, 1.0f / 5.0f), backDesc, dOutDesc);
+        for (size_t i = 0; i < 5; ++i) CHECK(std::fabs(back[i] - dIn[i]) <= 4e-7f * 2.0f);
+        // c2c of fft([0,1,2,3]) in complex<float>: exact
+        std::vector<std::complex<float>> a = {{0, 0}, {1, 0}, {2, 0}, {3, 0}}, b(4);
+        auto da = DataDesc<std::complex<float>>::init(a.data(), std::vector<size_t>{4});
+        auto db = DataDesc<std::complex<float>>::init(b.data(), std::vector<size_t>{4});
+        apply(FFTDesc<float>::init({0}, true), db, da);
+        CHECK(b[0] == std::complex<float>(6, 0) && b[1] == std::complex<float>(-2, 2));
+        CHECK(b[2] == std::complex<float>(-2, 0) && b[3] == std::complex<float>(-2, -2));
+    }
     // cpp_pocketfft/pocketfft.nim:322-337: default DCT (type 2) of [4, 3, 5, 10]
     {
         const std::vector<double> dIn = {4.0, 3.0, 5.0, 10.0};

@@ -4,7 +4,9 @@ Run in the container that holds the reference tree, after
 `make -C oracle ref` has built oracle/_ref/libpocketfft_ref.so from the
 reference sources in place:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py          # golden.npz (and golden_r2r.npz)
+    python tests/golden/make_golden.py --r2r    # golden_r2r.npz only
+    python tests/golden/make_golden.py --f32    # golden_f32.npz (T = float)
 
 Inputs come from impulse_amd.synth (SplitMix64, seed recorded per case) and
 are stored next to the outputs, so the fixture is self-contained data.
@@ -116,8 +118,47 @@ def r2r():
     print(out, os.path.getsize(out), "bytes,", len(g), "arrays")
 
 
+def f32():
+    """pocketfft::c2c / r2c / c2r<float> (the reference's T = float instantiation, reached from impulse
+    through FFTDesc[float32] / DataDesc[float32]) on float32 inputs: 1-D, n-D views and rows of the
+    [256, 4096] / [256, 4095] workloads."""
+    g = {}
+    for n in LENGTHS_FULL + LENGTHS_FWD:
+        x = synth.complex_normal((n,), seed=6000 + n).astype(np.complex64)
+        g[f"c2c_{n}_in"] = x
+        g[f"c2c_{n}_fwd"] = ref.c2c(x, [0], True)
+        g[f"c2c_{n}_bwd"] = ref.c2c(x, [0], False, 0.5)
+        r = synth.uniform((n,), -1.0, 1.0, seed=7000 + n).astype(np.float32)
+        g[f"r2c_{n}_in"] = r
+        g[f"r2c_{n}_out"] = ref.r2c(r, [0], True)
+        g[f"r2c_{n}_bwd"] = ref.r2c(r, [0], False, 2.0)
+        c = synth.complex_normal((n // 2 + 1,), seed=8000 + n).astype(np.complex64)
+        g[f"c2r_{n}_in"] = c
+        g[f"c2r_{n}_out"] = ref.c2r(c, n, [0], False, 1.0 / n)
+    a = synth.complex_normal((12, 40), seed=17).astype(np.complex64)
+    g["nd_c2c_in"] = a
+    g["nd_c2c_01"] = ref.c2c(a[:, ::2], [0, 1], True)
+    g["nd_c2c_10_bwd"] = ref.c2c(a[::-1, :], [1, 0], False, 0.25)
+    b = synth.uniform((6, 10, 9), -1.0, 1.0, seed=18).astype(np.float32)
+    g["nd_r2c_in"] = b
+    g["nd_r2c_012"] = ref.r2c(b, [0, 1, 2], True)
+    g["nd_r2c_20"] = ref.r2c(b, [2, 0], True)
+    h = synth.complex_normal((6, 10, 5), seed=19).astype(np.complex64)
+    g["nd_c2r_in"] = h
+    g["nd_c2r_012"] = ref.c2r(h, 9, [0, 1, 2], False, 1.0 / 540)
+    for n in (4096, 4095):
+        rows = synth.uniform((256, n), -1.0, 1.0, seed=0)[:2].astype(np.float32)
+        g[f"rows_{n}_in"] = rows
+        g[f"rows_{n}_out"] = ref.r2c(rows, [1], True)
+    out = os.path.join(HERE, "golden_f32.npz")
+    np.savez_compressed(out, **g)
+    print(out, os.path.getsize(out), "bytes,", len(g), "arrays")
+
+
 if __name__ == "__main__":
     if "--r2r" in sys.argv:
         r2r()
+    elif "--f32" in sys.argv:
+        f32()
     else:
         main()

@@ -175,3 +175,27 @@ def test_conv_full_size_finite_differences():
           f"h=1e-2 {worst[1e-2]:.3e}")
     assert worst[1e-6] <= 2e-3
     assert worst[1e-2] <= 1e-7
+
+
+def test_gradients_through_conjugate_views():
+    """torch passes lazily conjugated gradients (a conj bit, e.g. from a loss built on Y.conj()); the
+    backward passes must read their values, not the raw memory behind the view.  Linear functionals:
+    sum(x * grad) equals the loss exactly."""
+    torch.manual_seed(1)
+    for n in (64, 4095, 10007):
+        x = torch.randn(3, n, dtype=torch.float64, device="cuda", requires_grad=True)
+        Y = AG.rfftn(x, [1])
+        G = torch.randn_like(Y)
+        loss = (Y.conj() * G).real.sum()
+        (gx,) = torch.autograd.grad(loss, x)
+        assert abs(float(loss) - float((x.detach() * gx).sum())) <= 1e-10 * float(x.norm() * gx.norm()), n
+        Z = torch.randn(3, n // 2 + 1, dtype=torch.complex128, device="cuda", requires_grad=True)
+        z = AG.irfftn(Z.conj(), n, [1])          # a conjugate view as the input, too
+        w = torch.randn_like(z)
+        loss2 = (z * w).sum()
+        (gZ,) = torch.autograd.grad(loss2, Z)
+        assert abs(float(loss2) - float((Z.detach().conj() * gZ).real.sum())) <= \
+            1e-10 * float(Z.detach().abs().norm() * gZ.abs().norm()), n
+    with pytest.raises(ValueError):
+        from impulse_amd import fft as F
+        F.DataDesc(torch.zeros(4, dtype=torch.complex128, device="cuda").conj(), [4])

@@ -122,3 +122,42 @@ def test_ragged_packed_offsets():
     from impulse_amd import ragged
     ins, outs = ragged.packed_offsets([5, 8, 1])
     assert list(ins) == [0, 5, 13] and list(outs) == [0, 3, 8]
+
+
+def test_inplace_rules_before_planning():
+    # pocketfft's sanity_check (pocketfft_hdronly.h:453-454): in == out needs equal strides; r2c / c2r
+    # never run in place.  Both are rejected before a plan is built, so no device is touched.
+    L = _lib.lib()
+    buf = np.zeros(16, np.complex128)
+    p = buf.ctypes.data
+    rc = L.impulse_fft_c2c(1, _lib.sizes([4]), _lib.strides([-16]), _lib.strides([32]), 1, _lib.sizes([0]), 1,
+                           _lib.F64, p, p, 1.0, 1)
+    assert rc == 1 and b"stride mismatch" in L.impulse_fft_last_error()
+    rc = L.impulse_fft_r2c(1, _lib.sizes([8]), _lib.strides([8]), _lib.strides([8]), 1, _lib.sizes([0]), 1,
+                           _lib.F64, p, p, 1.0, 1)
+    assert rc == 1 and b"in-place" in L.impulse_fft_last_error()
+    rc = L.impulse_fft_c2r(1, _lib.sizes([8]), _lib.strides([16]), _lib.strides([16]), 1, _lib.sizes([0]), 0,
+                           _lib.F32, p, p, 1.0, 1)
+    assert rc == 1
+
+
+def test_mirror_dtype_rules():
+    # the C++ path's T (cpp_pocketfft/pocketfft.nim:35-69): float64 / complex128 -> F64,
+    # float32 / complex64 -> F32, anything else or mixed precisions is an error
+    assert F.DataDesc(np.zeros(4), [4]).dtype == _lib.F64
+    assert F.DataDesc(np.zeros(4, np.complex64), [4]).dtype == _lib.F32
+    with pytest.raises(TypeError):
+        F.DataDesc(np.zeros(4, np.int32), [4]).dtype
+    with pytest.raises(TypeError):
+        F.apply(F.FFTDesc([0], True), F.DataDesc(np.zeros(3, np.complex128), [3]),
+                F.DataDesc(np.zeros(4, np.float32), [4]))
+    with pytest.raises(TypeError):
+        F.c2c(np.zeros(4, np.float32), [0])
+
+
+def test_python_plan_cache_is_bounded_and_keyed_by_stream():
+    # host-side key logic only: a bounded LRU whose key carries the device and the stream
+    import inspect
+    src = inspect.getsource(F.get_plan)
+    assert "device" in src and "stream" in src
+    assert F.PLAN_CACHE_CAPACITY >= 1

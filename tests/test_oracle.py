@@ -8,12 +8,14 @@ DFT, and (d) closed forms and the reference's known-answer tests, chosen so
 that a wrong sign, a dropped term, a transposed operand or a wrong scale
 factor fails at least one of them.
 """
+import os
+
 import numpy as np
 import pytest
 
 import oracle
 from oracle import ref
-from tests.conftest import rel_l2
+from tests.conftest import ROOT, rel_l2
 
 TOL = 1e-14
 
@@ -335,3 +337,31 @@ def test_fftpack_hartley_against_reference_library():
             for fw in (True, False):
                 assert rel_l2(oracle.r2r_fftpack(v[::-1, :, ::2], axes, r2h, fw),
                               ref.r2r_fftpack(v[::-1, :, ::2], axes, r2h, fw)) < TOL
+
+
+def test_oracle_vs_reference_float_golden():
+    """The reference's T = float instantiation (pocketfft::c2c / r2c / c2r<float>, written by
+    tests/golden/make_golden.py --f32) against the oracle on the same float inputs widened to double:
+    the two differ only by the reference's float rounding, <= 5e-7 rel-L2 here (measured 2.3e-7)."""
+    g = np.load(os.path.join(ROOT, "tests", "golden", "golden_f32.npz"))
+    ns = sorted({int(k.split("_")[1]) for k in g.files if k.startswith("c2c_") and k.endswith("_in")})
+    assert len(ns) >= 40
+    for n in ns:
+        x = g[f"c2c_{n}_in"].astype(np.complex128)
+        assert rel_l2(g[f"c2c_{n}_fwd"], oracle.c2c(x, [0], True)) < 5e-7, n
+        assert rel_l2(g[f"c2c_{n}_bwd"], oracle.c2c(x, [0], False, 0.5)) < 5e-7, n
+        r = g[f"r2c_{n}_in"].astype(np.float64)
+        assert rel_l2(g[f"r2c_{n}_out"], oracle.r2c(r, [0], True)) < 5e-7, n
+        assert rel_l2(g[f"r2c_{n}_bwd"], oracle.r2c(r, [0], False, 2.0)) < 5e-7, n
+        c = g[f"c2r_{n}_in"].astype(np.complex128)
+        assert rel_l2(g[f"c2r_{n}_out"], oracle.c2r(c, n, [0], False, 1.0 / n)) < 5e-7, n
+    a = g["nd_c2c_in"].astype(np.complex128)
+    assert rel_l2(g["nd_c2c_01"], oracle.c2c(a[:, ::2], [0, 1], True)) < 5e-7
+    assert rel_l2(g["nd_c2c_10_bwd"], oracle.c2c(a[::-1, :], [1, 0], False, 0.25)) < 5e-7
+    b = g["nd_r2c_in"].astype(np.float64)
+    assert rel_l2(g["nd_r2c_012"], oracle.r2c(b, [0, 1, 2], True)) < 5e-7
+    assert rel_l2(g["nd_r2c_20"], oracle.r2c(b, [2, 0], True)) < 5e-7
+    h = g["nd_c2r_in"].astype(np.complex128)
+    assert rel_l2(g["nd_c2r_012"], oracle.c2r(h, 9, [0, 1, 2], False, 1.0 / 540)) < 5e-7
+    for n in (4096, 4095):
+        assert rel_l2(g[f"rows_{n}_out"], oracle.r2c(g[f"rows_{n}_in"].astype(np.float64), [1])) < 5e-7

@@ -1,38 +1,34 @@
 // Ragged batches: thousands of real transforms of unequal lengths in one
-// hipGraph launch of persistent kernels.
+// persistent launch.
 //
-// For a real signal of length n the engine forms a complex signal z of length
-// M (M = n/2, z[m] = x[2m] + i x[2m+1] for even n; for odd n, M = n and two
-// signals of the same length share one transform as z = x + i x2, separated as
-// X = (Z[k] + conj Z[n-k]) / 2 and X2 = (Z[k] - conj Z[n-k]) / 2i),
-// transforms it, and writes the n/2+1 bins.  The host plans every signal for
-// the configuration's LDS budget B (points) as tasks of at most B points:
-//   small (M 13-smooth and M <= B, or a Bluestein length n2 <= B): one task,
-//     the whole transform in LDS (Stockham stages), the bins written from LDS;
-//   big, M 13-smooth: four-step M = L1 L2 through a workspace of the signal's
-//     own (not of the workgroup): column tiles (FFTs of length L2 over batches
-//     of columns, times W_M^(n1 k2)) write it, and row tiles read it back --
-//     bands of rows k2 with their mirror rows L2 - k2, which hold Z[k] and
-//     Z[M-k] for every bin k of the band, so the real post-process and the
-//     transposed store to natural order happen in the same tile;
-//   big Bluestein (n2 = L1 L2 7-smooth, n2 >= M + K - 1, K = M outputs for even
-//     n, (n+1)/2 for odd): column tiles of z times the chirp, row tiles (forward
-//     FFT, times the chirp kernel's spectrum, inverse FFT, times W_n2^(+a k2)),
-//     inverse column tiles (the bins of an odd single signal, or c in natural
-//     order for the split), and for even n and pairs split tiles (X from Z[k]
-//     and Z[Q-k]).
-// The tasks of one phase are independent, so each phase is one persistent
-// kernel whose workgroups claim tasks with one atomic each (longest first); the
-// kernel boundaries order the phases.  One execute is one hipGraph replay of a
-// counter reset and the phase kernels.  The chirp kernel's spectrum depends only
-// on n: it is computed once per distinct length when the batch is created, in
-// natural order.  Twiddles and the chirp exp(-i pi j^2 / M) are generated in
-// FP64 with sincospi (the chirp index j^2 mod 2M in 64-bit integers).  Input is
-// float32; arithmetic and the output bins (complex128) are FP64.
+// Every workgroup takes whole signals from a queue (one vector atomic per
+// claim; the host orders the queue by decreasing cost, so the tail of the
+// launch is short signals).  For a real signal of length n the workgroup forms
+// a complex signal z of length M (M = n/2, z[m] = x[2m] + i x[2m+1] for even n;
+// for odd n, M = n and two signals of the same length share one transform as
+// z = x + i x2, separated as X = (Z[k] + conj Z[n-k]) / 2 and
+// X2 = (Z[k] - conj Z[n-k]) / 2i), transforms it, and writes the n/2+1 bins.  The host plans each signal (task table):
+//   M 13-smooth, fits the LDS budget : Stockham stages in LDS;
+//   M 13-smooth, longer              : four-step M = L1 L2 through the
+//                                      workgroup's global scratch (batches of
+//                                      columns, then batches of rows, in LDS);
+//   otherwise                        : Bluestein with a 7-smooth n2 >= M+K-1,
+//                                      where K = M outputs are needed for even n
+//                                      and K = (n+1)/2 for odd n; the chirp
+//                                      kernel's spectrum is computed once per
+//                                      distinct length when the batch is
+//                                      created and kept in a device table, so an
+//                                      execute runs two transforms of length n2
+//                                      per Bluestein signal.
+// Twiddles and the chirp exp(-i pi j^2 / M) are generated in FP64 with
+// sincospi (the chirp index j^2 mod 2M in 64-bit integers).  Phases that hand
+// data to other threads of the workgroup through global scratch are separated
+// by a workgroup barrier.  Input is float32; arithmetic and the
+// output bins (complex128) are FP64.
 //
-// The autotuner chooses the workgroup size, the LDS budget B (which decides
-// which signals are small and how big ones split into tiles) and the radix
-// order of the LDS stages.
+// The autotuner chooses the workgroup size, the LDS budget per transform
+// (which decides which lengths stay in LDS and how four-step lengths split)
+// and the radix order of the LDS stages.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -46,34 +42,18 @@
 
 namespace ifft {
 
-// task kinds; the phase of each is given in brackets
-enum RgKind : int {
-    RG_SMALL = 0,   // [1] whole transform in LDS
-    RG_COLS = 1,    // [1] column tile of a smooth split: load z, FFT, twiddle -> workspace
-    RG_BCOLS = 2,   // [1] column tile of a Bluestein split: z times the chirp, FFT, twiddle -> workspace
-    RG_ROWS = 3,    // [2] band of row pairs (k2, L2 - k2) of a smooth split -> bins
-    RG_SELF = 4,    // [2] the self-paired rows 0 and L2/2 of a smooth split -> bins (and X[M])
-    RG_BROWS = 5,   // [2] row tile of a Bluestein split: FFT, times the spectrum, inverse FFT, twiddle, in place
-    RG_ICOLS = 6,   // [3] inverse column tile of a Bluestein split -> bins (odd single) or c in natural order
-    RG_SPLIT = 7    // [4] Bluestein even / pair: bins from c[k] and c[Q - k]
-};
-constexpr int kPhases = 4;
-
 struct RaggedTask {
     long long in_off;    // first float of the signal
     long long out_off;   // first complex bin of the output
-    long long in_off2;   // pair: the second odd signal of the same length (z = x + i x2)
-    long long out_off2;
-    long long ws_off;    // split transforms: first point of the signal's workspace
-    long long spec_off;  // Bluestein: first point of the chirp kernel's spectrum (natural order)
-    int kind;
     int n;               // real length
     int M;               // complex length (n/2 or n)
     int L;               // transform length: M, or n2 for Bluestein
-    int L1, L2;          // split L = L1 L2: rows of L1 contiguous points, columns of L2 (0: in LDS)
-    int first, count;    // tile: first column / row / row pair / bin, and how many
-    int width;           // LDS batch: columns (COLS, BCOLS, ICOLS), row slots (ROWS), rows (BROWS)
+    int L1, L2;          // four-step split of L (0: L stays in LDS)
+    int ca, cb;          // columns per LDS batch in pass A (length L2) / rows in pass B (length L1)
     int blue;            // Bluestein
+    long long spec_off;  // Bluestein: first point of the chirp kernel's spectrum in the table
+    long long in_off2;   // pair: the second odd signal of the same length (z = x + i x2)
+    long long out_off2;
     int pair;
     int pad_;
 };
@@ -84,10 +64,9 @@ struct RaggedParams {
     const RaggedTask *tasks;
     int ntasks;
     int *counter;
-    double2 *ws;               // workspaces of the split transforms
-    double2 *scratch;          // spectrum mode: per workgroup, `slot` complex points
+    double2 *scratch;          // per workgroup: 2 buffers of `slot` complex points
     long long slot;
-    double2 *spec;             // table of chirp-kernel spectra (unnormalized, natural order)
+    double2 *spec;             // table of chirp-kernel spectra (unnormalized)
     int mode;                  // 0: transform the signals; 1: fill the spectrum table
     int order;                 // radix order of LDS stages (radix_codes): bit 0 odd radices first, bit 1 radix-8 powers of two
     double fct;                // output scale

@@ -14,6 +14,7 @@ LIB_PATH = os.environ.get("IMPULSE_FFT_LIB") or os.path.join(_HERE, "libimpulse_
 # Every function declared in include/impulse_fft.h.
 EXPORTS = (
     "impulse_fft_plan_create", "impulse_fft_execute", "impulse_fft_execute_host",
+    "impulse_fft_execute_noalloc",
     "impulse_fft_plan_device_bytes", "impulse_fft_plan_num_launches", "impulse_fft_plan_pass_info",
     "impulse_fft_plan_profile", "impulse_fft_plan_profile_read", "impulse_fft_plan_destroy",
     "impulse_fft_c2c", "impulse_fft_r2c", "impulse_fft_c2r",
@@ -53,6 +54,7 @@ def _sig(L):
     L.impulse_fft_plan_create.argtypes = [ctypes.POINTER(_vp), i, i, sz, _szp, _pdp, _pdp, sz, _szp, i]
     L.impulse_fft_execute.argtypes = [_vp, _vp, _vp, d, _vp]
     L.impulse_fft_execute_host.argtypes = [_vp, _vp, _vp, d]
+    L.impulse_fft_execute_noalloc.argtypes = [_vp, _vp, _vp, d, _vp]
     L.impulse_fft_plan_device_bytes.argtypes = [_vp]
     L.impulse_fft_plan_device_bytes.restype = sz
     L.impulse_fft_plan_num_launches.argtypes = [_vp]

@@ -232,6 +232,14 @@ int impulse_fft_execute(impulse_fft_plan plan, const void *data_in, void *data_o
     return rc == ifft::OK ? rc : fail(err);
 }
 
+int impulse_fft_execute_noalloc(impulse_fft_plan plan, const void *data_in, void *data_out, double fct, void *stream)
+{
+    if (!plan || !plan->plan) return fail(ifft::E_INVALID, "plan is NULL");
+    ifft::Error err{0, ""};
+    int rc = ifft::plan_execute_noalloc(*plan->plan, data_in, data_out, fct, (hipStream_t)stream, err);
+    return rc == ifft::OK ? rc : fail(err);
+}
+
 int impulse_fft_execute_host(impulse_fft_plan plan, const void *data_in, void *data_out, double fct)
 {
     if (!plan || !plan->plan) return fail(ifft::E_INVALID, "plan is NULL");

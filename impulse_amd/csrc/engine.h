@@ -124,6 +124,8 @@ int plan_create(std::unique_ptr<Plan> &out, int kind, int dtype, size_t ndim, co
                 int forward, Error &err);
 int plan_execute(Plan &plan, const void *in, void *out, double fct, hipStream_t stream, Error &err);
 int plan_execute_host(Plan &plan, const void *in, void *out, double fct, Error &err);
+// the same as plan_execute with the workspace in the library's static per-device scratch (no allocation)
+int plan_execute_noalloc(Plan &plan, const void *in, void *out, double fct, hipStream_t stream, Error &err);
 int plan_profile(Plan &plan, int max_executes, Error &err);
 void plan_spans(Plan &P, ptrdiff_t in_elem, ptrdiff_t out_elem);   // host byte spans of the shape/strides
 

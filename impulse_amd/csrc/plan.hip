@@ -1172,7 +1172,87 @@ int plan_create(std::unique_ptr<Plan> &out, int kind, int dtype, size_t ndim, co
 // execution
 // ---------------------------------------------------------------------------
 
+// Workspace pointers of one execute: the plan's own buffers, or a borrowed region
+// (impulse_fft_execute_noalloc).
+struct Workspace {
+    void *s0, *s1, *tmp;
+    size_t b0, b1, bt;
+};
+
+static int plan_execute_ws(Plan &P, const void *in, void *out, double fct, hipStream_t stream, const Workspace &W,
+                           Error &err);
+
 int plan_execute(Plan &P, const void *in, void *out, double fct, hipStream_t stream, Error &err)
+{
+    const Workspace W{P.s0.p, P.s1.p, P.tmp.p, P.s0.bytes, P.s1.bytes, P.tmp.bytes};
+    return plan_execute_ws(P, in, out, fct, stream, W, err);
+}
+
+// ---------------------------------------------------------------------------
+// execution without allocation: the workspace comes from one static device
+// array per GPU instead of the plan's buffers
+// ---------------------------------------------------------------------------
+
+// Every call on a GPU shares that GPU's copy of this array (it keeps the last
+// call's contents; the passes write every workspace word before reading it), so
+// calls that overlap on one GPU race.  The host uses the per-device address
+// from hipGetSymbolAddress, never the array's name.
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mtx;
+static void *g_scratch_ptr[kMaxDevices];
+
+static int scratch_address(void **out, Error &err)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        (void)hipGetLastError();
+        err = {E_NODEVICE, "no usable current device for the no-allocation scratch"};
+        return E_NODEVICE;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mtx);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            err = {E_HIP, std::string("hipGetSymbolAddress of the no-allocation scratch failed: ") +
+                              hipGetErrorString(e)};
+            return E_HIP;
+        }
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return OK;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+int plan_execute_noalloc(Plan &P, const void *in, void *out, double fct, hipStream_t stream, Error &err)
+{
+    if (P.r2r || P.rfp) {
+        err = {E_UNSUPPORTED, "no-allocation execution takes c2c / r2c / c2r plans"};
+        return E_UNSUPPORTED;
+    }
+    const size_t need = align256(P.s0_bytes) + align256(P.s1_bytes) + align256(P.tmp_bytes);
+    if (need > kNoAllocScratchBytes) {
+        err = {E_UNSUPPORTED, "the plan's workspace (" + std::to_string(need) + " bytes) exceeds the " +
+                                  std::to_string(kNoAllocScratchBytes) + "-byte no-allocation scratch"};
+        return E_UNSUPPORTED;
+    }
+    char *base = nullptr;
+    int rc = scratch_address((void **)&base, err);
+    if (rc != OK) return rc;
+    const size_t o1 = align256(P.s0_bytes), o2 = o1 + align256(P.s1_bytes);
+    const Workspace W{P.s0_bytes ? base : nullptr, P.s1_bytes ? base + o1 : nullptr, P.tmp_bytes ? base + o2 : nullptr,
+                      P.s0_bytes, P.s1_bytes, P.tmp_bytes};
+    return plan_execute_ws(P, in, out, fct, stream, W, err);
+}
+
+static int plan_execute_ws(Plan &P, const void *in, void *out, double fct, hipStream_t stream, const Workspace &W,
+                           Error &err)
 {
     if (P.empty) return OK;
     if (!in || !out) { err = {E_INVALID, "null data pointer"}; return E_INVALID; }
@@ -1210,9 +1290,9 @@ int plan_execute(Plan &P, const void *in, void *out, double fct, hipStream_t str
         switch (r) {
             case REF_IN: return const_cast<void *>(in);
             case REF_OUT: return out;
-            case REF_S0: return P.s0.p;
-            case REF_S1: return P.s1.p;
-            case REF_TMP: return P.tmp.p;
+            case REF_S0: return W.s0;
+            case REF_S1: return W.s1;
+            case REF_TMP: return W.tmp;
             default: return nullptr;
         }
     };
@@ -1223,9 +1303,9 @@ int plan_execute(Plan &P, const void *in, void *out, double fct, hipStream_t str
         switch (r) {
             case REF_IN: lo = P.in_lo; span = P.in_hi - P.in_lo; break;
             case REF_OUT: lo = P.out_lo; span = P.out_hi - P.out_lo; break;
-            case REF_S0: lo = 0; span = (long long)P.s0.bytes; break;
-            case REF_S1: lo = 0; span = (long long)P.s1.bytes; break;
-            case REF_TMP: lo = 0; span = (long long)P.tmp.bytes; break;
+            case REF_S0: lo = 0; span = (long long)W.b0; break;
+            case REF_S1: lo = 0; span = (long long)W.b1; break;
+            case REF_TMP: lo = 0; span = (long long)W.bt; break;
             default: lo = 0; span = 0; break;
         }
     };

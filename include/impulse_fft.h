@@ -100,6 +100,20 @@ int impulse_fft_plan_create(impulse_fft_plan *plan, int kind, int dtype, size_t 
 int impulse_fft_execute(impulse_fft_plan plan, const void *data_in, void *data_out, double fct,
                         void *stream);
 
+/* The same as impulse_fft_execute (device pointers, asynchronous on `stream`,
+ * same kernels and results) with the workspace taken from a static device
+ * array of the library (one per GPU, 64 MiB, held until the process ends)
+ * instead of the plan's buffers: the call allocates nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory at call time (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Plans whose workspace exceeds the
+ * scratch, and DCT / DST / FFTPACK / Hartley plans, return
+ * IMPULSE_FFT_ERR_UNSUPPORTED.  Every call on one GPU shares that GPU's array:
+ * calls that may overlap on one GPU (other streams, other threads) race and
+ * must be ordered by the caller. */
+int impulse_fft_execute_noalloc(impulse_fft_plan plan, const void *data_in, void *data_out, double fct,
+                                void *stream);
+
 /* Execute on HOST pointers: copies in, runs, copies out, synchronous.  This is
  * the drop-in for pocketfft's host-memory calls.  The copies go through a
  * pinned staging ring of the plan, on a stream of the plan's own; concurrent

@@ -1,35 +1,40 @@
 // Ragged batches: thousands of real transforms of unequal lengths in one
-// persistent launch.
+// hipGraph launch of persistent kernels.
 //
-// Every workgroup takes whole signals from a queue (one vector atomic per
-// claim; the host orders the queue by decreasing cost, so the tail of the
-// launch is short signals).  For a real signal of length n the workgroup forms
-// a complex signal z of length M (M = n/2, z[m] = x[2m] + i x[2m+1] for even n;
-// for odd n, M = n and two signals of the same length share one transform as
-// z = x + i x2, separated as X = (Z[k] + conj Z[n-k]) / 2 and
-// X2 = (Z[k] - conj Z[n-k]) / 2i), transforms it, and writes the n/2+1 bins.  The host plans each signal (task table):
-//   M 13-smooth, fits the LDS budget : Stockham stages in LDS;
-//   M 13-smooth, longer              : four-step M = L1 L2 through the
-//                                      workgroup's global scratch (batches of
-//                                      columns, then batches of rows, in LDS);
-//   otherwise                        : Bluestein with a 7-smooth n2 >= M+K-1,
-//                                      where K = M outputs are needed for even n
-//                                      and K = (n+1)/2 for odd n; the chirp
-//                                      kernel's spectrum is computed once per
-//                                      distinct length when the batch is
-//                                      created and kept in a device table, so an
-//                                      execute runs two transforms of length n2
-//                                      per Bluestein signal.
-// Twiddles and the chirp exp(-i pi j^2 / M) are generated in FP64 with
-// sincospi (the chirp index j^2 mod 2M in 64-bit integers).  Phases that hand
-// data to other threads of the workgroup through global scratch are separated
-// by a workgroup barrier.  Input is float32; arithmetic and the
-// output bins (complex128) are FP64.
+// For a real signal of length n the engine forms a complex signal z of length
+// M (M = n/2, z[m] = x[2m] + i x[2m+1] for even n; for odd n, M = n and two
+// signals of the same length share one transform as z = x + i x2, separated as
+// X = (Z[k] + conj Z[n-k]) / 2 and X2 = (Z[k] - conj Z[n-k]) / 2i),
+// transforms it, and writes the n/2+1 bins.  The host plans every signal for
+// the configuration's LDS budget B (points) as tasks of at most B points:
+//   small (M 13-smooth and M <= B, or a Bluestein length n2 <= B): one task,
+//     the whole transform in LDS (Stockham stages), the bins written from LDS;
+//   big, M 13-smooth: four-step M = L1 L2 through a workspace of the signal's
+//     own (not of the workgroup): column tiles (FFTs of length L2 over batches
+//     of columns, times W_M^(n1 k2)) write it, and row tiles read it back --
+//     bands of rows k2 with their mirror rows L2 - k2, which hold Z[k] and
+//     Z[M-k] for every bin k of the band, so the real post-process and the
+//     transposed store to natural order happen in the same tile;
+//   big Bluestein (n2 = L1 L2 7-smooth, n2 >= M + K - 1, K = M outputs for even
+//     n, (n+1)/2 for odd): column tiles of z times the chirp, row tiles (forward
+//     FFT, times the chirp kernel's spectrum, inverse FFT, times W_n2^(+a k2)),
+//     inverse column tiles (the bins of an odd single signal, or c in natural
+//     order for the split), and for even n and pairs split tiles (X from Z[k]
+//     and Z[Q-k]).
+// The tasks of one phase are independent, so each phase is one persistent
+// kernel whose workgroups claim tasks with one atomic each (longest first); the
+// kernel boundaries order the phases.  One execute is one hipGraph replay of a
+// counter reset and the phase kernels.  The chirp kernel's spectrum depends only
+// on n: it is computed once per distinct length when the batch is created, in
+// natural order.  Twiddles and the chirp exp(-i pi j^2 / M) are generated in
+// FP64 with sincospi (the chirp index j^2 mod 2M in 64-bit integers).  Input is
+// float32; arithmetic and the output bins (complex128) are FP64.
 //
-// The autotuner chooses the workgroup size, the LDS budget per transform
-// (which decides which lengths stay in LDS and how four-step lengths split)
-// and the radix order of the LDS stages.
+// The autotuner chooses the workgroup size, the LDS budget B (which decides
+// which signals are small and how big ones split into tiles) and the radix
+// order of the LDS stages.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -42,18 +47,34 @@
 
 namespace ifft {
 
+// task kinds; the phase of each is given in brackets
+enum RgKind : int {
+    RG_SMALL = 0,   // [1] whole transform in LDS
+    RG_COLS = 1,    // [1] column tile of a smooth split: load z, FFT, twiddle -> workspace
+    RG_BCOLS = 2,   // [1] column tile of a Bluestein split: z times the chirp, FFT, twiddle -> workspace
+    RG_ROWS = 3,    // [2] band of row pairs (k2, L2 - k2) of a smooth split -> bins
+    RG_SELF = 4,    // [2] the self-paired rows 0 and L2/2 of a smooth split -> bins (and X[M])
+    RG_BROWS = 5,   // [2] row tile of a Bluestein split: FFT, times the spectrum, inverse FFT, twiddle, in place
+    RG_ICOLS = 6,   // [3] inverse column tile of a Bluestein split -> bins (odd single) or c in natural order
+    RG_SPLIT = 7    // [4] Bluestein even / pair: bins from c[k] and c[Q - k]
+};
+constexpr int kPhases = 4;
+
 struct RaggedTask {
     long long in_off;    // first float of the signal
     long long out_off;   // first complex bin of the output
+    long long in_off2;   // pair: the second odd signal of the same length (z = x + i x2)
+    long long out_off2;
+    long long ws_off;    // split transforms: first point of the signal's workspace
+    long long spec_off;  // Bluestein: first point of the chirp kernel's spectrum (natural order)
+    int kind;
     int n;               // real length
     int M;               // complex length (n/2 or n)
     int L;               // transform length: M, or n2 for Bluestein
-    int L1, L2;          // four-step split of L (0: L stays in LDS)
-    int ca, cb;          // columns per LDS batch in pass A (length L2) / rows in pass B (length L1)
+    int L1, L2;          // split L = L1 L2: rows of L1 contiguous points, columns of L2 (0: in LDS)
+    int first, count;    // tile: first column / row / row pair / bin, and how many
+    int width;           // LDS batch: columns (COLS, BCOLS, ICOLS), row slots (ROWS), rows (BROWS)
     int blue;            // Bluestein
-    long long spec_off;  // Bluestein: first point of the chirp kernel's spectrum in the table
-    long long in_off2;   // pair: the second odd signal of the same length (z = x + i x2)
-    long long out_off2;
     int pair;
     int pad_;
 };
@@ -64,9 +85,10 @@ struct RaggedParams {
     const RaggedTask *tasks;
     int ntasks;
     int *counter;
-    double2 *scratch;          // per workgroup: 2 buffers of `slot` complex points
+    double2 *ws;               // workspaces of the split transforms
+    double2 *scratch;          // spectrum mode: per workgroup, `slot` complex points
     long long slot;
-    double2 *spec;             // table of chirp-kernel spectra (unnormalized)
+    double2 *spec;             // table of chirp-kernel spectra (unnormalized, natural order)
     int mode;                  // 0: transform the signals; 1: fill the spectrum table
     int order;                 // radix order of LDS stages (radix_codes): bit 0 odd radices first, bit 1 radix-8 powers of two
     double fct;                // output scale
@@ -139,199 +161,243 @@ __device__ __forceinline__ double2 chirp(long long j, long long M, double inv2m)
 // workgroup-scope release/acquire covers global memory too: the waves of a
 // workgroup share one CU and its write-through L1 (no threadgroup split).  A
 // device-scope fence here would write back and invalidate the whole L2.
+// (Data handed between workgroups goes through a kernel boundary instead.)
 __device__ __forceinline__ void wg_sync_global() { __syncthreads(); }
 
-// Complex transform of length T.L (sign s) by the workgroup; load(m) gives
-// input point m.  Returns true with the result in LDS (natural order), or
-// false with the result in `out` in row layout: out[k2 L1 + k1] = X[k2 + L2 k1]
-// (four-step, used for the chirp-kernel spectrum table).  `tmp` is global
-// scratch of L points.
+// One LDS batch of columns [c0, c0 + cnt) (cnt <= width, width divides the
+// workgroup) of a split transform L = L1 L2: column n1 holds points
+// n1 + L1 n2 (load(m) gives point m).  Each column is transformed with sign s,
+// multiplied by W_L^(s n1 k2) and stored at out[n1 + L1 k2].  The strided
+// accesses are `width` points wide.
 template <class LOAD>
-__device__ bool wg_fft(LOAD load, const RaggedTask &T, double s, double2 *lds, double2 *tmp, double2 *out,
-                       int order);
-
-// Column pass of a four-step transform L = L1 L2: columns n1 (length L2,
-// stride L1, point m = n1 + L1 n2 from load(m)) are transformed with sign s in
-// LDS batches of ca and written times W_L^(s n1 k2) to tmp[n1 + L1 k2].  The
-// batches make the strided accesses ca points wide.
-template <class LOAD>
-__device__ void col_pass(LOAD load, const RaggedTask &T, double s, double2 *lds, double2 *tmp, int order)
+__device__ void col_batch(LOAD load, long long L, int L1, int L2, int c0, int cnt, int width, double s, double2 *lds,
+                          double2 *out, int order)
 {
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const long long L = T.L;
-    const int L1 = T.L1, L2 = T.L2, ca = T.ca;
-    for (int c0 = 0; c0 < L1; c0 += ca) {
-        const int cnt = min(ca, L1 - c0);
-        for (int e = tid; e < ca * L2; e += nthr) {
-            const int t = e % ca, n2 = e / ca;
-            lds[lpad(t * L2 + n2)] = t < cnt ? load((long long)(c0 + t) + (long long)L1 * n2) : make_double2(0.0, 0.0);
-        }
-        __syncthreads();
-        lds_fft(lds, L2, ca, s, order, TwSincos{});
-        {
-            // t = tid % ca is fixed per thread (ca divides nthr); k2 advances by nthr / ca
-            const int t = tid % ca, step = nthr / ca;
-            const long long n1 = c0 + t;
-            double2 w = twid(n1 * (long long)(tid / ca), L, s);
-            const double2 ws = twid(n1 * (long long)step, L, s);
-            for (int k2 = tid / ca; k2 < L2; k2 += step) {
-                if (t < cnt) tmp[n1 + (long long)L1 * k2] = cmul(lds[lpad(t * L2 + k2)], w);
-                w = cmul(w, ws);
-            }
-        }
-        __syncthreads();
+    for (int e = tid; e < width * L2; e += nthr) {
+        const int t = e % width, n2 = e / width;
+        lds[lpad(t * L2 + n2)] = t < cnt ? load((long long)(c0 + t) + (long long)L1 * n2) : make_double2(0.0, 0.0);
     }
-    wg_sync_global();
+    __syncthreads();
+    lds_fft(lds, L2, width, s, order, TwSincos{});
+    {
+        // t = tid % width is fixed per thread (width divides nthr); k2 advances by nthr / width
+        const int t = tid % width, step = nthr / width;
+        const long long n1 = c0 + t;
+        double2 w = twid(n1 * (long long)(tid / width), L, s);
+        const double2 ws = twid(n1 * (long long)step, L, s);
+        for (int k2 = tid / width; k2 < L2; k2 += step) {
+            if (t < cnt) out[n1 + (long long)L1 * k2] = cmul(lds[lpad(t * L2 + k2)], w);
+            w = cmul(w, ws);
+        }
+    }
+    __syncthreads();
 }
 
-// Row pass: every row k2 of tmp (L1 contiguous points) is transformed in
-// place with sign s, in LDS batches of cb rows.  With `spec` (Bluestein):
-// forward transform, times spec[k2 L1 + k1], inverse transform, times
-// W_L^(+a k2) -- the middle of the chirp convolution in one pass.
-__device__ void row_pass(const RaggedTask &T, double s, double2 *lds, double2 *tmp, int order, const double2 *spec)
+// Rows [r0, r0 + cnt) (cnt <= width) of `buf` (rows of L1 contiguous points)
+// transformed in place with sign s.
+__device__ void row_batch(int L1, int r0, int cnt, int width, double s, double2 *lds, double2 *buf, int order)
 {
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const long long L = T.L;
-    const int L1 = T.L1, L2 = T.L2, cb = T.cb;
-    for (int r0 = 0; r0 < L2; r0 += cb) {
-        const int cnt = min(cb, L2 - r0);
-        double2 *row = tmp + (long long)r0 * L1;
-        for (int e = tid; e < cb * L1; e += nthr) lds[lpad(e)] = e < cnt * L1 ? row[e] : make_double2(0.0, 0.0);
-        __syncthreads();
-        lds_fft(lds, L1, cb, s, order, TwSincos{});
-        if (spec) {
-            const double2 *sp = spec + (long long)r0 * L1;
-            for (int e = tid; e < cnt * L1; e += nthr) lds[lpad(e)] = cmul(lds[lpad(e)], sp[e]);
-            __syncthreads();
-            lds_fft(lds, L1, cb, -s, order, TwSincos{});
-            for (int e = tid; e < cnt * L1; e += nthr) {
-                const int t = e / L1, a = e - t * L1;
-                row[e] = cmul(lds[lpad(e)], twid((long long)a * (r0 + t), L, -s));
-            }
-        } else {
-            for (int e = tid; e < cnt * L1; e += nthr) row[e] = lds[lpad(e)];
-        }
-        __syncthreads();
-    }
-    wg_sync_global();
+    double2 *row = buf + (long long)r0 * L1;
+    for (int e = tid; e < width * L1; e += nthr) lds[lpad(e)] = e < cnt * L1 ? row[e] : make_double2(0.0, 0.0);
+    __syncthreads();
+    lds_fft(lds, L1, width, s, order, TwSincos{});
+    for (int e = tid; e < cnt * L1; e += nthr) row[e] = lds[lpad(e)];
+    __syncthreads();
 }
 
-// Column pass producing final values: columns a of tmp (length L2, stride
-// L1) transformed with sign s; emit(k, v) receives c[k], k = a + L1 b.
+// Bluestein row tile: rows [r0, r0 + cnt) of the workspace (row k2, point k1)
+// in place: forward FFT, times the chirp kernel's spectrum (natural order: bin
+// k2 + L2 k1, read in runs of cnt bins), inverse FFT, times W_L^(+a k2).
+__device__ void brows_batch(long long L, int L1, int L2, int r0, int cnt, int width, double2 *lds, double2 *ws,
+                            const double2 *spec, int order)
+{
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    double2 *row = ws + (long long)r0 * L1;
+    for (int e = tid; e < width * L1; e += nthr) lds[lpad(e)] = e < cnt * L1 ? row[e] : make_double2(0.0, 0.0);
+    __syncthreads();
+    lds_fft(lds, L1, width, -1.0, order, TwSincos{});
+    for (int e = tid; e < cnt * L1; e += nthr) {
+        const int t = e % cnt, k1 = e / cnt;
+        const int q = lpad(t * L1 + k1);
+        lds[q] = cmul(lds[q], spec[(long long)(r0 + t) + (long long)L2 * k1]);
+    }
+    __syncthreads();
+    lds_fft(lds, L1, width, 1.0, order, TwSincos{});
+    for (int e = tid; e < cnt * L1; e += nthr) {
+        const int t = e / L1, a = e - t * L1;
+        row[e] = cmul(lds[lpad(e)], twid((long long)a * (r0 + t), L, 1.0));
+    }
+    __syncthreads();
+}
+
+// Inverse column tile: columns [c0, c0 + cnt) of the workspace (length L2,
+// stride L1) transformed with sign +1; emit(k, v) receives c[k], k = a + L1 b,
+// for the tile's own columns (so an in-place emit touches only its columns).
 template <class EMIT>
-__device__ void col_pass_out(EMIT emit, const RaggedTask &T, double s, double2 *lds, const double2 *tmp, int order)
+__device__ void icols_batch(EMIT emit, int L1, int L2, int c0, int cnt, int width, double2 *lds, const double2 *ws,
+                            int order)
 {
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int L1 = T.L1, L2 = T.L2, ca = T.ca;
-    for (int c0 = 0; c0 < L1; c0 += ca) {
-        const int cnt = min(ca, L1 - c0);
-        for (int e = tid; e < ca * L2; e += nthr) {
-            const int t = e % ca, k2 = e / ca;
-            lds[lpad(t * L2 + k2)] = t < cnt ? tmp[(long long)(c0 + t) + (long long)L1 * k2] : make_double2(0.0, 0.0);
-        }
-        __syncthreads();
-        lds_fft(lds, L2, ca, s, order, TwSincos{});
-        for (int e = tid; e < ca * L2; e += nthr) {
-            const int t = e % ca, b = e / ca;
-            if (t < cnt) emit((long long)(c0 + t) + (long long)L1 * b, lds[lpad(t * L2 + b)]);
-        }
-        __syncthreads();
+    for (int e = tid; e < width * L2; e += nthr) {
+        const int t = e % width, k2 = e / width;
+        lds[lpad(t * L2 + k2)] = t < cnt ? ws[(long long)(c0 + t) + (long long)L1 * k2] : make_double2(0.0, 0.0);
     }
-    wg_sync_global();
+    __syncthreads();
+    lds_fft(lds, L2, width, 1.0, order, TwSincos{});
+    for (int e = tid; e < width * L2; e += nthr) {
+        const int t = e % width, b = e / width;
+        if (t < cnt) emit((long long)(c0 + t) + (long long)L1 * b, lds[lpad(t * L2 + b)]);
+    }
+    __syncthreads();
 }
 
-// Output of a four-step transform left in row layout (tmp[k2 L1 + k1] =
-// Z[k2 + L2 k1]) in tiles of 16 x 32, transposed through LDS so that both
-// the row reads and the natural-order writes are contiguous.  Even n: the
-// half-length split X[k] = E + W_n^k O with Z[M-k] read from the mirrored
-// tile, and X[M] at the end; odd n: X[k] = Z[k], k <= n/2.
-constexpr int kTR = 16, kTC = 32;
-static_assert(2 * kTC * (kTR + 1) <= 2048, "two padded output tiles must fit the smallest LDS budget");
-__device__ void post_tiles(const RaggedTask &T, double2 *lds, const double2 *tmp, double2 *y, double2 *y2,
-                           bool half, double f, double fo)
+// Bin k of the signal from Z[k] (zk) and Z[Q - k] (zm), Q the complex length M:
+// even n: X[k] = E + W_n^k O with E = (Z[k] + conj Z[M-k]) / 2 and
+// O = (Z[k] - conj Z[M-k]) / 2i, for every k <= M (k = M from zk = zm = Z[0]
+// gives Re Z[0] - Im Z[0]); odd pair: X[k] = E and
+// X2[k] = O for k <= n/2; odd single: X[k] = Z[k] for k <= n/2.  f scales the
+// real parts, fo the imaginary parts (fo = -f conjugates).
+__device__ __forceinline__ void emit_bin(const RaggedTask &T, long long k, double2 zk, double2 zm, double2 *y,
+                                         double2 *y2, double f, double fo)
 {
-    const int tid = threadIdx.x, nthr = blockDim.x;
-    const int L1 = T.L1, L2 = T.L2, n = T.n;
-    const long long M = T.L;
-    const bool pair = y2 != nullptr;
-    const long long limit = half ? M - 1 : n / 2;
-    double2 *lds2 = lds + kTC * (kTR + 1);   // second tile (pairs)
-    for (int r = 0; r < L2; r += kTR)
-        for (int c = 0; c < L1; c += kTC) {
-            for (int e = tid; e < kTR * kTC; e += nthr) {
-                const int t = e / kTC, u = e % kTC, k2 = r + t, k1 = c + u;
-                if (k2 >= L2 || k1 >= L1) continue;
-                const long long k = k2 + (long long)L2 * k1;
-                if (k > limit) continue;
-                const double2 zk = tmp[(long long)k2 * L1 + k1];
-                double2 v = zk;
-                if (half || pair) {
-                    // Z[(M - k) mod M]: row L2 - k2 reversed, or row 0 reversed
-                    const long long pm = k2 == 0 ? (long long)((L1 - k1) % L1)
-                                                 : (long long)(L2 - k2) * L1 + (L1 - 1 - k1);
-                    const double2 zm = tmp[pm];
-                    const double2 e1 = cscale(cadd(zk, cconj(zm)), 0.5), d1 = csub(zk, cconj(zm));
-                    const double2 o1 = make_double2(0.5 * d1.y, -0.5 * d1.x);
-                    if (half) {
-                        v = cadd(e1, cmul(o1, twid32((int)k, n, -1.0)));
-                    } else {
-                        v = e1;
-                        lds2[u * (kTR + 1) + t] = make_double2(f * o1.x, fo * o1.y);
-                    }
-                }
-                lds[u * (kTR + 1) + t] = make_double2(f * v.x, fo * v.y);
-            }
-            __syncthreads();
-            for (int e = tid; e < kTR * kTC; e += nthr) {
-                const int t = e % kTR, u = e / kTR, k2 = r + t, k1 = c + u;
-                if (k2 >= L2 || k1 >= L1) continue;
-                const long long k = k2 + (long long)L2 * k1;
-                if (k <= limit) {
-                    y[k] = lds[u * (kTR + 1) + t];
-                    if (pair) y2[k] = lds2[u * (kTR + 1) + t];
-                }
-            }
-            __syncthreads();
+    const int n = T.n;
+    if (!(n & 1)) {
+        const double2 w = twid32((int)k, n, -1.0);
+        const double2 e1 = cscale(cadd(zk, cconj(zm)), 0.5), d1 = csub(zk, cconj(zm));
+        const double2 o1 = make_double2(0.5 * d1.y, -0.5 * d1.x);
+        const double2 v = cadd(e1, cmul(o1, w));
+        y[k] = make_double2(f * v.x, fo * v.y);
+    } else if (k <= n / 2) {
+        if (T.pair) {
+            const double2 e1 = cscale(cadd(zk, cconj(zm)), 0.5), d1 = csub(zk, cconj(zm));
+            y[k] = make_double2(f * e1.x, fo * e1.y);
+            y2[k] = make_double2(f * 0.5 * d1.y, -fo * 0.5 * d1.x);
+        } else {
+            y[k] = make_double2(f * zk.x, fo * zk.y);
         }
-    if (half && tid == 0) {
-        const double2 z0 = tmp[0];
-        y[M] = make_double2(f * (z0.x - z0.y), 0.0);
     }
 }
 
-template <class LOAD>
-__device__ bool wg_fft(LOAD load, const RaggedTask &T, double s, double2 *lds, double2 *tmp, double2 *out, int order)
+// Row band of a smooth split (rows of L1 points, Z[k2 + L2 k1] at row k2, point
+// k1): the rows k2 = r0 .. r0 + R - 1 (slots 0 .. R-1) and their mirrors
+// L2 - k2 (slots R .. 2R-1; 1 <= k2 < L2 - k2), transformed in LDS.  Z[M - k]
+// of a bin k = k2 + L2 k1 is point L1 - 1 - k1 of the mirror row, so every bin
+// of the band's 2R rows is finished here; the stores run over R consecutive
+// bins (k2 fastest).
+__device__ void rows_split(const RaggedTask &T, double2 *lds, const double2 *ws, double2 *y, double2 *y2, double f,
+                           double fo, int order)
 {
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const long long L = T.L;
-    if (T.L1 == 0) {
-        for (long long m = tid; m < L; m += nthr) lds[lpad((int)m)] = load(m);
-        __syncthreads();
-        lds_fft(lds, (int)L, 1, s, order, TwSincos{});
-        return true;
+    const int L1 = T.L1, L2 = T.L2, r0 = T.first, R = T.count, W = T.width;
+    for (int e = tid; e < W * L1; e += nthr) {
+        const int t = e / L1, k1 = e - t * L1;
+        double2 v = make_double2(0.0, 0.0);
+        if (t < 2 * R) {
+            const int row = t < R ? r0 + t : L2 - r0 - (t - R);
+            v = ws[(long long)row * L1 + k1];
+        }
+        lds[lpad(e)] = v;
     }
-    col_pass(load, T, s, lds, tmp, order);
-    row_pass(T, s, lds, tmp, order, nullptr);
-    for (long long m = tid; m < L; m += nthr) out[m] = tmp[m];
-    wg_sync_global();
-    return false;
+    __syncthreads();
+    lds_fft(lds, L1, W, -1.0, order, TwSincos{});
+    for (int e = tid; e < 2 * R * L1; e += nthr) {
+        const int t = e % (2 * R), k1 = e / (2 * R);
+        const int row = t < R ? r0 + t : L2 - r0 - (t - R);
+        const int pt = t < R ? t + R : t - R;
+        const double2 zk = lds[lpad(t * L1 + k1)], zm = lds[lpad(pt * L1 + L1 - 1 - k1)];
+        emit_bin(T, (long long)row + (long long)L2 * k1, zk, zm, y, y2, f, fo);
+    }
+    __syncthreads();
 }
+
+// The self-paired rows of a smooth split: row 0 (Z[M - L2 k1] is its point
+// (L1 - k1) mod L1) in slot 0, and for even L2 row L2/2 (mirror point
+// L1 - 1 - k1) in slot 1; for even n also X[M] = Re Z[0] - Im Z[0].
+__device__ void self_split(const RaggedTask &T, double2 *lds, const double2 *ws, double2 *y, double2 *y2, double f,
+                           double fo, int order)
+{
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int L1 = T.L1, L2 = T.L2;
+    const bool mid = (L2 & 1) == 0;
+    for (int e = tid; e < 2 * L1; e += nthr) {
+        const int t = e / L1, k1 = e - t * L1;
+        lds[lpad(e)] = (t == 0 || mid) ? ws[(long long)(t ? L2 / 2 : 0) * L1 + k1] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    lds_fft(lds, L1, 2, -1.0, order, TwSincos{});
+    for (int e = tid; e < 2 * L1; e += nthr) {
+        const int t = e & 1, k1 = e >> 1;
+        if (t == 1 && !mid) continue;
+        const double2 zk = lds[lpad(t * L1 + k1)];
+        const int mk = t == 0 ? (L1 - k1) % L1 : L1 - 1 - k1;
+        const double2 zm = lds[lpad(t * L1 + mk)];
+        emit_bin(T, (long long)(t ? L2 / 2 : 0) + (long long)L2 * k1, zk, zm, y, y2, f, fo);
+    }
+    if (!(T.n & 1) && tid == 0) {
+        const double2 z0 = lds[lpad(0)];
+        y[T.M] = make_double2(f * (z0.x - z0.y), 0.0);
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// the kernels
+// ---------------------------------------------------------------------------
 
 __global__ void k_ragged_reset(int *counter)
 {
-    if (threadIdx.x == 0) counter[0] = 0;
+    if (threadIdx.x < kPhases) counter[64 * threadIdx.x] = 0;
+}
+
+// the chirp kernel's spectrum of one Bluestein length into the table, natural
+// order (create time; the split path goes through the workgroup's scratch `S0`)
+__device__ void spectrum_task(const RaggedTask &T, const RaggedParams &P, double2 *lds, double2 *S0)
+{
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const long long n = T.n, M = T.M, n2 = T.L;
+    const bool half = (n % 2) == 0, pair = T.pair != 0;
+    const long long K = (half || pair) ? M : n / 2 + 1;
+    const double inv2m = 0.5 / (double)M;
+    double2 *B = P.spec + T.spec_off;
+    // chirp kernel b[m] = conj(w[m]) for 0 <= m < K, conj(w[n2 - m]) for n2 - m < M (zero between)
+    auto bload = [&](long long m) -> double2 {
+        if (m < K) return cconj(chirp(m, M, inv2m));
+        if (m > n2 - M) return cconj(chirp(n2 - m, M, inv2m));
+        return make_double2(0.0, 0.0);
+    };
+    if (T.L1 == 0) {
+        for (long long m = tid; m < n2; m += nthr) lds[lpad((int)m)] = bload(m);
+        __syncthreads();
+        lds_fft(lds, (int)n2, 1, -1.0, P.order, TwSincos{});
+        for (long long m = tid; m < n2; m += nthr) B[m] = lds[lpad((int)m)];
+        __syncthreads();
+        return;
+    }
+    // width = columns per batch, count = rows per batch
+    for (int c0 = 0; c0 < T.L1; c0 += T.width)
+        col_batch(bload, n2, T.L1, T.L2, c0, min(T.width, T.L1 - c0), T.width, -1.0, lds, S0, P.order);
+    for (int r0 = 0; r0 < T.L2; r0 += T.count)
+        row_batch(T.L1, r0, min(T.count, T.L2 - r0), T.count, -1.0, lds, S0, P.order);
+    // row layout (S0[k2 L1 + k1] = bin k2 + L2 k1) to natural order
+    for (long long m = tid; m < n2; m += nthr) {
+        const long long k2 = m / T.L1, k1 = m - k2 * T.L1;
+        B[k2 + (long long)T.L2 * k1] = S0[m];
+    }
+    __syncthreads();
 }
 
 // One instance per workgroup size and register budget: WPE = 4 caps the
 // registers at 128 per lane (four waves per SIMD, some spills), WPE = 0 lets
-// the compiler take what the workgroup size allows (1024 threads: 128).
+// the compiler take what the workgroup size allows (1024 threads: 128).  One
+// launch per phase: the workgroups claim the phase's tasks in order.
 template <int MAXT, int WPE>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ? WPE : 1))) k_ragged(const RaggedParams P)
 {
     extern __shared__ double2 lds[];
     __shared__ int s_task;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    double2 *S0 = P.scratch + (long long)blockIdx.x * P.slot;   // four-step scratch
     for (;;) {
         if (tid == 0) s_task = atomicAdd(P.counter, 1);
         __syncthreads();
@@ -339,119 +405,104 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
         __syncthreads();
         if (task >= P.ntasks) break;
         const RaggedTask T = P.tasks[task];
+        if (P.mode == 1) {
+            spectrum_task(T, P, lds, P.scratch + (long long)blockIdx.x * P.slot);
+            continue;
+        }
         const long long n = T.n, M = T.M;
         const bool half = (n % 2) == 0, pair = T.pair != 0;
         const float *x = P.x + T.in_off;
         const float *x2 = P.x + (pair ? T.in_off2 : T.in_off);
         double2 *y = P.y + T.out_off;
         double2 *y2 = pair ? P.y + T.out_off2 : nullptr;
+        double2 *ws = P.ws + T.ws_off;
         auto zin = [&](long long m) -> double2 {
             return half ? make_double2((double)x[2 * m], (double)x[2 * m + 1])
                         : make_double2((double)x[m], pair ? (double)x2[m] : 0.0);
         };
         const double inv2m = 0.5 / (double)M;
         const double f = P.fct, fo = P.fct * P.osign;
-        if (P.mode == 1) {
-            // chirp kernel b[m] = conj(w[m]) for 0 <= m < K, conj(w[n2 - m]) for n2 - m < M
-            // (indices mod n2, zero between); its unnormalized spectrum goes to the table
-            // (row layout when the transform is split)
-            const long long n2 = T.L, K = (half || pair) ? M : n / 2 + 1;
-            double2 *B = P.spec + T.spec_off;
-            auto bload = [&](long long m) -> double2 {
-                if (m < K) return cconj(chirp(m, M, inv2m));
-                if (m > n2 - M) return cconj(chirp(n2 - m, M, inv2m));
-                return make_double2(0.0, 0.0);
-            };
-            if (wg_fft(bload, T, -1.0, lds, S0, B, P.order))
-                for (long long m = tid; m < n2; m += nthr) B[m] = lds[lpad((int)m)];
-            __syncthreads();
-            continue;
-        }
-        bool in_lds = true;    // Z in LDS (natural order) or in S0 (natural order)
-        if (!T.blue) {
-            if (T.L1 == 0) {
-                wg_fft(zin, T, -1.0, lds, S0, nullptr, P.order);
-            } else {
-                // columns, rows in place, then the transposing output pass
-                col_pass(zin, T, -1.0, lds, S0, P.order);
-                row_pass(T, -1.0, lds, S0, P.order, nullptr);
-                post_tiles(T, lds, S0, y, y2, half, f, fo);
-                __syncthreads();
-                continue;
+        switch (T.kind) {
+            case RG_COLS:
+                col_batch(zin, T.L, T.L1, T.L2, T.first, T.count, T.width, -1.0, lds, ws, P.order);
+                break;
+            case RG_BCOLS: {
+                auto aload = [&](long long m) -> double2 {
+                    return m < M ? cmul(zin(m), chirp(m, M, inv2m)) : make_double2(0.0, 0.0);
+                };
+                col_batch(aload, T.L, T.L1, T.L2, T.first, T.count, T.width, -1.0, lds, ws, P.order);
+                break;
             }
-        } else {
-            // A = FFT(z w); C = A B; c = IFFT(C) (unnormalized; 1/n2 applied with the output chirp)
-            const long long n2 = T.L;
-            const double2 *B = P.spec + T.spec_off;
-            auto aload = [&](long long m) -> double2 {
-                return m < M ? cmul(zin(m), chirp(m, M, inv2m)) : make_double2(0.0, 0.0);
-            };
-            if (T.L1 == 0) {
-                wg_fft(aload, T, -1.0, lds, S0, nullptr, P.order);
-                for (long long m = tid; m < n2; m += nthr) lds[lpad((int)m)] = cmul(lds[lpad((int)m)], B[m]);
-                __syncthreads();
-                lds_fft(lds, (int)n2, 1, 1.0, P.order, TwSincos{});
-            } else {
-                col_pass(aload, T, -1.0, lds, S0, P.order);
-                row_pass(T, -1.0, lds, S0, P.order, B);
+            case RG_ROWS: rows_split(T, lds, ws, y, y2, f, fo, P.order); break;
+            case RG_SELF: self_split(T, lds, ws, y, y2, f, fo, P.order); break;
+            case RG_BROWS:
+                brows_batch(T.L, T.L1, T.L2, T.first, T.count, T.width, lds, ws, P.spec + T.spec_off, P.order);
+                break;
+            case RG_ICOLS:
                 if (half || pair) {
-                    // c in natural order back into S0 (each column rewrites the points it read)
-                    auto keep = [&](long long k, double2 v) { S0[k] = v; };
-                    col_pass_out(keep, T, 1.0, lds, S0, P.order);
-                    in_lds = false;
+                    // c[k] for k < M in natural order, in place: each column rewrites points it read
+                    auto keep = [&](long long k, double2 v) {
+                        if (k < M) ws[k] = v;
+                    };
+                    icols_batch(keep, T.L1, T.L2, T.first, T.count, T.width, lds, ws, P.order);
                 } else {
                     const long long K = n / 2 + 1;
-                    const double sc = 1.0 / (double)n2;
+                    const double sc = 1.0 / (double)T.L;
                     auto emit = [&](long long k, double2 v) {
                         if (k < K) {
                             const double2 z = cscale(cmul(v, chirp(k, M, inv2m)), sc);
                             y[k] = make_double2(f * z.x, fo * z.y);
                         }
                     };
-                    col_pass_out(emit, T, 1.0, lds, S0, P.order);
-                    continue;
+                    icols_batch(emit, T.L1, T.L2, T.first, T.count, T.width, lds, ws, P.order);
                 }
+                break;
+            case RG_SPLIT: {
+                // Z[k] = c[k] w[k] / n2; bins k in [first, first + count) with their mirrors
+                const double sc = 1.0 / (double)T.L;
+                auto Z = [&](long long k) -> double2 { return cscale(cmul(ws[k], chirp(k, M, inv2m)), sc); };
+                for (long long k = T.first + tid; k < (long long)T.first + T.count; k += nthr) {
+                    const long long km = (M - k) % M;
+                    const double2 zk = Z(k), zm = Z(km);
+                    emit_bin(T, k, zk, zm, y, y2, f, fo);
+                    if (half && M - k != k) emit_bin(T, M - k, zm, zk, y, y2, f, fo);
+                }
+                __syncthreads();
+                break;
+            }
+            default: {   // RG_SMALL: the whole transform in LDS
+                const long long L = T.L;
+                if (!T.blue) {
+                    for (long long m = tid; m < L; m += nthr) lds[lpad((int)m)] = zin(m);
+                    __syncthreads();
+                    lds_fft(lds, (int)L, 1, -1.0, P.order, TwSincos{});
+                } else {
+                    // A = FFT(z w); C = A B; c = IFFT(C) (unnormalized; 1/n2 applied with the output chirp)
+                    const double2 *B = P.spec + T.spec_off;
+                    for (long long m = tid; m < L; m += nthr)
+                        lds[lpad((int)m)] = m < M ? cmul(zin(m), chirp(m, M, inv2m)) : make_double2(0.0, 0.0);
+                    __syncthreads();
+                    lds_fft(lds, (int)L, 1, -1.0, P.order, TwSincos{});
+                    for (long long m = tid; m < L; m += nthr) lds[lpad((int)m)] = cmul(lds[lpad((int)m)], B[m]);
+                    __syncthreads();
+                    lds_fft(lds, (int)L, 1, 1.0, P.order, TwSincos{});
+                }
+                const double bscale = T.blue ? 1.0 / (double)L : 1.0;
+                auto Zl = [&](long long k) -> double2 {
+                    const double2 v = lds[lpad((int)k)];
+                    return T.blue ? cscale(cmul(v, chirp(k, M, inv2m)), bscale) : v;
+                };
+                const long long kend = half ? M / 2 : n / 2;
+                for (long long k = tid; k <= kend; k += nthr) {
+                    const long long km = (M - k) % M;
+                    const double2 zk = Zl(k), zm = Zl(km);
+                    emit_bin(T, k, zk, zm, y, y2, f, fo);
+                    if (half && M - k != k) emit_bin(T, M - k, zm, zk, y, y2, f, fo);
+                }
+                __syncthreads();
+                break;
             }
         }
-        // Z[k] = FFT_M(z)[k], k < K (Bluestein: c[k] w[k] / n2)
-        const double bscale = T.blue ? 1.0 / (double)T.L : 1.0;
-        auto Z = [&](long long k) -> double2 {
-            const double2 v = in_lds ? lds[lpad((int)k)] : S0[k];
-            return T.blue ? cscale(cmul(v, chirp(k, M, inv2m)), bscale) : v;
-        };
-        if (half) {
-            // X[k] = E + W_n^k O with E = (Z[k] + conj Z[M-k]) / 2, O = (Z[k] - conj Z[M-k]) / 2i;
-            // X[M-k] from the same pair with W_n^{M-k} = -conj(W_n^k)
-            for (long long k = tid; k <= M / 2; k += nthr) {
-                const long long km = (M - k) % M;
-                const double2 zk = Z(k), zm = Z(km);
-                const double2 w = twid32((int)k, (int)n, -1.0);
-                const double2 e1 = cscale(cadd(zk, cconj(zm)), 0.5), d1 = csub(zk, cconj(zm));
-                const double2 o1 = make_double2(0.5 * d1.y, -0.5 * d1.x);
-                const double2 x1 = cadd(e1, cmul(o1, w));
-                y[k] = make_double2(f * x1.x, fo * x1.y);
-                if (M - k != k) {
-                    const double2 e2 = cscale(cadd(zm, cconj(zk)), 0.5), d2 = csub(zm, cconj(zk));
-                    const double2 o2 = make_double2(0.5 * d2.y, -0.5 * d2.x);
-                    const double2 x2 = cadd(e2, cmul(o2, make_double2(-w.x, w.y)));
-                    y[M - k] = make_double2(f * x2.x, fo * x2.y);
-                }
-            }
-        } else if (pair) {
-            // X[k] = (Z[k] + conj Z[n-k]) / 2, X2[k] = (Z[k] - conj Z[n-k]) / 2i
-            for (long long k = tid; k <= n / 2; k += nthr) {
-                const double2 zk = Z(k), zm = Z((n - k) % n);
-                const double2 e1 = cscale(cadd(zk, cconj(zm)), 0.5), d1 = csub(zk, cconj(zm));
-                y[k] = make_double2(f * e1.x, fo * e1.y);
-                y2[k] = make_double2(f * 0.5 * d1.y, -fo * 0.5 * d1.x);
-            }
-        } else {
-            for (long long k = tid; k <= n / 2; k += nthr) {
-                const double2 v = Z(k);
-                y[k] = make_double2(f * v.x, fo * v.y);
-            }
-        }
-        __syncthreads();
     }
 }
 
@@ -462,16 +513,15 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
 struct Ragged {
     int device = 0;
     std::vector<size_t> n, in_off, out_off;   // as given
-    std::vector<std::pair<int, int>> order;   // queue of (signal, paired signal or -1), decreasing cost
+    std::vector<std::pair<int, int>> units;   // (signal, paired signal or -1)
     int forward = 1;
     // configuration
     int threads = 0, lds_max = 0, rorder = 0, waves = 0;
-    // derived for the configuration
+    // derived for the configuration: the tasks of the phases back to back
     std::vector<RaggedTask> tasks;
-    long long slot = 0;
-    int grid = 0;
-    DevBuf dtasks, counter, scratch;
-    // chirp-kernel spectra, one per distinct Bluestein length
+    int phase_off[kPhases] = {0, 0, 0, 0}, phase_cnt[kPhases] = {0, 0, 0, 0}, phase_grid[kPhases] = {0, 0, 0, 0};
+    DevBuf dtasks, counter, ws;
+    // chirp-kernel spectra, one per distinct Bluestein length, natural order
     std::vector<std::pair<long long, long long>> spec_index;   // (2 n + pair, offset), sorted
     DevBuf spec;
     double flops = 0.0;
@@ -523,8 +573,8 @@ static long long transform_len(long long n, bool pair)
     return rg_good7(M + K - 1);
 }
 
-// largest divisor of a 13-smooth L that is <= cap
-static long long largest_divisor_le(long long L, long long cap)
+// divisors of a 13-smooth L
+static std::vector<long long> divisors(long long L)
 {
     std::vector<long long> divs = {1};
     long long r = L;
@@ -535,101 +585,191 @@ static long long largest_divisor_le(long long L, long long cap)
         long long pw = 1;
         for (int k = 1; k <= e; ++k) {
             pw *= p;
-            for (size_t q = 0; q < n0; ++q)
-                if (divs[q] * pw <= cap) divs.push_back(divs[q] * pw);
+            for (size_t q = 0; q < n0; ++q) divs.push_back(divs[q] * pw);
         }
     }
-    return *std::max_element(divs.begin(), divs.end());
+    return divs;
 }
 
-// Rows of a split Bluestein transform are at most this long: the spectrum
-// table is stored in the row layout of this split, whatever the configuration.
-constexpr int kBlueRow = 2048;
+static int pow2_floor(long long v)
+{
+    int c = 1;
+    while (2LL * c <= v) c *= 2;
+    return c;
+}
 
-// Plan one signal for an LDS budget of lds_max points (>= kBlueRow); false if
-// no split fits.  Split transforms L = L1 L2 take the longest rows L1 that fit
-// (L2 = L / L1 as short as possible), so the column passes move batches of
-// ca = lds_max / L2 columns and their strided accesses are wide.
-static bool plan_task(long long n, bool pair, int lds_max, RaggedTask &t)
+// Split L = L1 L2 (rows of L1 points, columns of L2) for tiles of at most B
+// points.  A column tile holds `ca` columns (its strided reads and writes are
+// ca points wide); a row tile holds `cb` rows, or for a smooth split (row_pairs)
+// cb / 2 rows and their mirrors (its stores are cb / 2 bins wide).  The split
+// with the widest narrow side wins, then the one with the fewest tiles.
+static bool split_tiles(long long L, int B, bool row_pairs, int &L1, int &L2, int &ca, int &cb)
+{
+    int best = 0;
+    long long best_tiles = 0;
+    for (long long d : divisors(L)) {
+        const long long e = L / d;   // d = L1 (row length), e = L2 (column length)
+        if (d < 2 || e < 2 || e > B || (row_pairs ? 2 * d : d) > B) continue;
+        const int c = std::min(64, pow2_floor(B / e));
+        const int r = std::min(64, pow2_floor(B / d));   // row slots
+        const int width = std::min(c, r);
+        const long long tiles = (d + c - 1) / c + (e + r - 1) / r;
+        if (width > best || (width == best && tiles < best_tiles)) {
+            best = width;
+            best_tiles = tiles;
+            L1 = (int)d;
+            L2 = (int)e;
+            ca = c;
+            cb = r;
+        }
+    }
+    return best > 0;
+}
+
+// Tasks of one unit (a signal, or an odd pair) for an LDS budget of B points,
+// appended to the phase lists; `ws` counts the workspace points handed out.
+static bool plan_unit(long long n, bool pair, int B, long long spec_off, RaggedTask base,
+                      std::vector<RaggedTask> (&ph)[kPhases], long long &ws)
 {
     const long long M = (n % 2 == 0) ? n / 2 : n;
-    t.n = (int)n;
-    t.M = (int)M;
-    t.blue = rg_smooth13(M) ? 0 : 1;
-    t.spec_off = 0;
-    t.pair = pair ? 1 : 0;
-    t.pad_ = 0;
-    t.in_off2 = t.out_off2 = 0;
+    base.n = (int)n;
+    base.M = (int)M;
+    base.blue = rg_smooth13(M) ? 0 : 1;
+    base.spec_off = base.blue ? spec_off : 0;
+    base.pair = pair ? 1 : 0;
+    base.pad_ = 0;
     const long long L = transform_len(n, pair);
     if (L > (1LL << 30)) return false;
-    t.L = (int)L;
-    t.L1 = t.L2 = t.ca = t.cb = 0;
-    const long long row_cap = t.blue ? kBlueRow : lds_max;
-    if (L <= row_cap) return true;
-    const long long L1 = largest_divisor_le(L, row_cap), L2 = L / L1;
-    if (L2 > lds_max || L1 < 2) return false;
-    t.L1 = (int)L1;
-    t.L2 = (int)L2;
-    t.ca = rg_batch(L2, lds_max);
-    t.cb = rg_batch(L1, lds_max);
+    base.L = (int)L;
+    base.L1 = base.L2 = base.first = base.count = base.width = 0;
+    base.ws_off = 0;
+    if (L <= B) {
+        base.kind = RG_SMALL;
+        ph[0].push_back(base);
+        return true;
+    }
+    int L1 = 0, L2 = 0, ca = 0, cb = 0;
+    if (!split_tiles(L, B, !base.blue, L1, L2, ca, cb)) return false;
+    base.L1 = L1;
+    base.L2 = L2;
+    base.ws_off = ws;
+    ws += L;
+    auto add = [&](int phase, int kind, int first, int count, int width) {
+        RaggedTask t = base;
+        t.kind = kind;
+        t.first = first;
+        t.count = count;
+        t.width = width;
+        ph[phase].push_back(t);
+    };
+    for (int c0 = 0; c0 < L1; c0 += ca) add(0, base.blue ? RG_BCOLS : RG_COLS, c0, std::min(ca, L1 - c0), ca);
+    if (!base.blue) {
+        // bands of row pairs (k2, L2 - k2), 1 <= k2 <= (L2 - 1) / 2, at most cb / 2 pairs each
+        // (power-of-two bands, so that their slots divide the workgroup), then rows 0 and L2/2
+        const int P = (L2 - 1) / 2, Rmax = cb / 2;
+        for (int r0 = 1; r0 <= P;) {
+            const int R = std::min(Rmax, pow2_floor(P - r0 + 1));
+            add(1, RG_ROWS, r0, R, 2 * R);
+            r0 += R;
+        }
+        add(1, RG_SELF, 0, (L2 % 2 == 0) ? 2 : 1, 2);
+        return true;
+    }
+    for (int r0 = 0; r0 < L2; r0 += cb) add(1, RG_BROWS, r0, std::min(cb, L2 - r0), cb);
+    for (int c0 = 0; c0 < L1; c0 += ca) add(2, RG_ICOLS, c0, std::min(ca, L1 - c0), ca);
+    if (n % 2 == 0 || pair) {
+        const long long kend = (n % 2 == 0) ? M / 2 : n / 2;   // bins 0 .. kend (with mirrors for even n)
+        for (long long k0 = 0; k0 <= kend; k0 += B) add(3, RG_SPLIT, (int)k0, (int)std::min<long long>(B, kend + 1 - k0), 0);
+    }
     return true;
 }
 
-constexpr long long kScratchCap = 16LL << 30;
-
-static int build_tasks(Ragged &R, int threads, int lds_max, int waves, Error &err)
+// work of a task in points times the log of its transform length (queue order: largest first)
+static double task_cost(const RaggedTask &t)
 {
-    std::vector<RaggedTask> tasks(R.order.size());
-    long long slot = 1;
-    for (size_t q = 0; q < R.order.size(); ++q) {
-        const size_t i = (size_t)R.order[q].first;
-        const int j = R.order[q].second;
-        RaggedTask &t = tasks[q];
-        if (!plan_task((long long)R.n[i], j >= 0, lds_max, t)) {
-            err = {E_UNSUPPORTED, "ragged signal of length " + std::to_string(R.n[i]) +
-                                      " has no four-step split for this configuration"};
-            return E_UNSUPPORTED;
-        }
-        t.in_off = (long long)R.in_off[i];
-        t.out_off = (long long)R.out_off[i];
-        if (j >= 0) {
-            t.in_off2 = (long long)R.in_off[(size_t)j];
-            t.out_off2 = (long long)R.out_off[(size_t)j];
-        }
-        if (t.blue)
-            t.spec_off = std::lower_bound(R.spec_index.begin(), R.spec_index.end(),
-                                          std::make_pair(2 * (long long)R.n[i] + (j >= 0), -1LL))->second;
-        if (t.L1 || t.blue) slot = std::max<long long>(slot, t.L);
+    switch (t.kind) {
+        case RG_SMALL: return (t.blue ? 2.0 : 1.0) * (double)t.L * std::log2((double)t.L + 1.0);
+        case RG_COLS: case RG_BCOLS: case RG_ICOLS: return (double)t.width * t.L2 * std::log2((double)t.L2 + 1.0);
+        case RG_ROWS: case RG_SELF: return (double)t.width * t.L1 * std::log2((double)t.L1 + 1.0);
+        case RG_BROWS: return 2.0 * t.width * t.L1 * std::log2((double)t.L1 + 1.0);
+        default: return (double)t.count;
     }
+}
+
+static long long spec_offset(const Ragged &R, long long n, bool pair)
+{
+    return std::lower_bound(R.spec_index.begin(), R.spec_index.end(), std::make_pair(2 * n + (pair ? 1 : 0), -1LL))
+        ->second;
+}
+
+static int resident_grid(const Ragged &R, int threads, int waves, size_t lds)
+{
     int ncu = 0, per = 0;
-    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, R.device) != hipSuccess || ncu <= 0)
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, R.device) != hipSuccess || ncu <= 0) {
+        (void)hipGetLastError();
         ncu = 256;
-    const size_t lds = (size_t)lds_points_padded(lds_max) * sizeof(double2);
+    }
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, ragged_fn(threads, waves), threads, lds) != hipSuccess ||
         per <= 0) {
         (void)hipGetLastError();
         per = 1;
     }
-    // persistent grid: every resident workgroup, with the per-workgroup scratch
-    // held to kScratchCap in total (fewer resident workgroups past that)
-    const long long per_wg = slot * (long long)sizeof(double2);
-    long long grid_ll = std::min<long long>((long long)ncu * per, (long long)tasks.size());
-    grid_ll = std::max<long long>(1, std::min<long long>(grid_ll, std::max<long long>(ncu, kScratchCap / per_wg)));
-    const int grid = (int)grid_ll;
-    const size_t sbytes = (size_t)grid * (size_t)per_wg;
-    if (sbytes > R.scratch.bytes && R.scratch.alloc(sbytes)) {
-        err = {E_NOMEM, "ragged scratch allocation failed"};
+    return ncu * per;
+}
+
+static int build_tasks(Ragged &R, int threads, int lds_max, int waves, Error &err)
+{
+    std::vector<RaggedTask> ph[kPhases];
+    long long ws = 0;
+    for (const auto &u : R.units) {
+        const size_t i = (size_t)u.first;
+        const bool pair = u.second >= 0;
+        const long long n = (long long)R.n[i];
+        const long long M = (n % 2 == 0) ? n / 2 : n;
+        RaggedTask base;
+        std::memset(&base, 0, sizeof(base));
+        base.in_off = (long long)R.in_off[i];
+        base.out_off = (long long)R.out_off[i];
+        if (pair) {
+            base.in_off2 = (long long)R.in_off[(size_t)u.second];
+            base.out_off2 = (long long)R.out_off[(size_t)u.second];
+        }
+        const long long so = rg_smooth13(M) ? 0 : spec_offset(R, n, pair);
+        if (!plan_unit(n, pair, lds_max, so, base, ph, ws)) {
+            err = {E_UNSUPPORTED, "ragged signal of length " + std::to_string(n) +
+                                      " has no tile split for this configuration"};
+            return E_UNSUPPORTED;
+        }
+    }
+    std::vector<RaggedTask> all;
+    int off[kPhases], cnt[kPhases];
+    for (int q = 0; q < kPhases; ++q) {
+        std::stable_sort(ph[q].begin(), ph[q].end(),
+                         [](const RaggedTask &a, const RaggedTask &b) { return task_cost(a) > task_cost(b); });
+        off[q] = (int)all.size();
+        cnt[q] = (int)ph[q].size();
+        all.insert(all.end(), ph[q].begin(), ph[q].end());
+    }
+    if (all.size() > (size_t)INT_MAX) { err = {E_UNSUPPORTED, "too many ragged tasks"}; return E_UNSUPPORTED; }
+    const size_t wsb = (size_t)std::max<long long>(ws, 1) * sizeof(double2);
+    if (wsb > R.ws.bytes && R.ws.alloc(wsb)) { err = {E_NOMEM, "ragged workspace allocation failed"}; return E_NOMEM; }
+    if (all.size() * sizeof(RaggedTask) > R.dtasks.bytes && R.dtasks.alloc(all.size() * sizeof(RaggedTask))) {
+        err = {E_NOMEM, "ragged task table allocation failed"};
         return E_NOMEM;
     }
-    if (hipMemcpy(R.dtasks.p, tasks.data(), tasks.size() * sizeof(RaggedTask), hipMemcpyHostToDevice) !=
-        hipSuccess) {
+    if (hipMemcpy(R.dtasks.p, all.data(), all.size() * sizeof(RaggedTask), hipMemcpyHostToDevice) != hipSuccess) {
         (void)hipGetLastError();
         err = {E_HIP, "ragged task upload failed"};
         return E_HIP;
     }
-    R.tasks.swap(tasks);
-    R.slot = slot;
-    R.grid = grid;
+    const size_t lds = (size_t)lds_points_padded(lds_max) * sizeof(double2);
+    const int res = resident_grid(R, threads, waves, lds);
+    for (int q = 0; q < kPhases; ++q) {
+        R.phase_off[q] = off[q];
+        R.phase_cnt[q] = cnt[q];
+        R.phase_grid[q] = std::max(1, std::min(res, cnt[q]));
+    }
+    R.tasks.swap(all);
     R.threads = threads;
     R.lds_max = lds_max;
     R.waves = waves;
@@ -637,12 +777,12 @@ static int build_tasks(Ragged &R, int threads, int lds_max, int waves, Error &er
     return OK;
 }
 
-// Spectrum table: one entry per distinct Bluestein length, filled by the
-// persistent kernel in spectrum mode.
+// Spectrum table: one entry per distinct Bluestein length, natural order, filled
+// by the persistent kernel in spectrum mode (1024 threads, 8192 points of LDS).
 static int build_spectra(Ragged &R, Error &err)
 {
     std::vector<long long> ns;   // keys 2 n + pair
-    for (const auto &u : R.order) {
+    for (const auto &u : R.units) {
         const long long n = (long long)R.n[(size_t)u.first];
         const long long M = (n % 2 == 0) ? n / 2 : n;
         if (!rg_smooth13(M)) ns.push_back(2 * n + (u.second >= 0 ? 1 : 0));
@@ -651,29 +791,47 @@ static int build_spectra(Ragged &R, Error &err)
     ns.erase(std::unique(ns.begin(), ns.end()), ns.end());
     R.spec_index.clear();
     if (ns.empty()) return OK;
+    constexpr int kB = 8192;
     std::vector<RaggedTask> st(ns.size());
     long long off = 0, slot = 1;
     for (size_t q = 0; q < ns.size(); ++q) {
-        if (!plan_task(ns[q] / 2, (ns[q] & 1) != 0, 8192, st[q])) {
-            err = {E_UNSUPPORTED, "no four-step split for the Bluestein length of a signal of length " +
-                                      std::to_string(ns[q] / 2)};
-            return E_UNSUPPORTED;
+        RaggedTask &t = st[q];
+        std::memset(&t, 0, sizeof(t));
+        const long long n = ns[q] / 2;
+        const bool pair = (ns[q] & 1) != 0;
+        t.n = (int)n;
+        t.M = (int)((n % 2 == 0) ? n / 2 : n);
+        t.pair = pair ? 1 : 0;
+        t.blue = 1;
+        t.kind = RG_SMALL;
+        const long long L = transform_len(n, pair);
+        t.L = (int)L;
+        if (L > kB) {
+            int L1 = 0, L2 = 0, ca = 0, cb = 0;
+            if (!split_tiles(L, kB, false, L1, L2, ca, cb)) {
+                err = {E_UNSUPPORTED, "no split for the Bluestein length of a signal of length " + std::to_string(n)};
+                return E_UNSUPPORTED;
+            }
+            t.L1 = L1;
+            t.L2 = L2;
+            t.width = ca;   // columns per batch
+            t.count = cb;   // rows per batch
+            slot = std::max(slot, L);
         }
-        st[q].in_off = st[q].out_off = 0;
-        st[q].spec_off = off;
+        t.spec_off = off;
         R.spec_index.push_back({ns[q], off});
-        off += st[q].L;
-        if (st[q].L1) slot = std::max<long long>(slot, st[q].L);
+        off += L;
     }
-    // longest first
     std::stable_sort(st.begin(), st.end(), [](const RaggedTask &a, const RaggedTask &b) { return a.L > b.L; });
     if (R.spec.alloc((size_t)off * sizeof(double2))) { err = {E_NOMEM, "spectrum table allocation failed"}; return E_NOMEM; }
-    DevBuf dt, ws;
+    DevBuf dt, scr;
     int ncu = 0;
-    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, R.device) != hipSuccess || ncu <= 0)
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, R.device) != hipSuccess || ncu <= 0) {
+        (void)hipGetLastError();
         ncu = 256;
+    }
     const int grid = (int)std::min<long long>(ncu, (long long)st.size());
-    if (dt.alloc(st.size() * sizeof(RaggedTask)) || ws.alloc((size_t)grid * (size_t)slot * sizeof(double2))) {
+    if (dt.alloc(st.size() * sizeof(RaggedTask)) || scr.alloc((size_t)grid * (size_t)slot * sizeof(double2))) {
         err = {E_NOMEM, "spectrum workspace allocation failed"};
         return E_NOMEM;
     }
@@ -681,7 +839,7 @@ static int build_spectra(Ragged &R, Error &err)
     P.tasks = (const RaggedTask *)dt.p;
     P.ntasks = (int)st.size();
     P.counter = (int *)R.counter.p;
-    P.scratch = (double2 *)ws.p;
+    P.scratch = (double2 *)scr.p;
     P.slot = slot;
     P.spec = (double2 *)R.spec.p;
     P.mode = 1;
@@ -693,7 +851,7 @@ static int build_spectra(Ragged &R, Error &err)
         hipLaunchKernelGGL(k_ragged_reset, dim3(1), dim3(64), 0, 0, (int *)R.counter.p);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = launch_ragged(P, 1024, 0, grid, (size_t)lds_points_padded(8192) * sizeof(double2), 0);
+    if (e == hipSuccess) e = launch_ragged(P, 1024, 0, grid, (size_t)lds_points_padded(kB) * sizeof(double2), 0);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -725,46 +883,26 @@ int ragged_create(Ragged **out, size_t count, const size_t *lengths, const size_
         R->in_elems = std::max<size_t>(R->in_elems, in_off[i] + (size_t)n);
         R->out_elems = std::max<size_t>(R->out_elems, out_off[i] + (size_t)n / 2 + 1);
     }
-    // Odd lengths > 1 pair up (two real signals, one complex transform) unless
-    // the pair's transform would be longer than the longest unpaired one of the
-    // batch: each task runs on one workgroup, so the longest task bounds the launch.
-    long long longest = 0;
-    for (size_t i = 0; i < count; ++i) longest = std::max(longest, transform_len((long long)lengths[i], false));
-    std::vector<std::pair<int, int>> units;
+    // odd lengths > 1 pair up: two real signals, one complex transform
     {
         std::vector<std::pair<long long, int>> odd;
         for (size_t i = 0; i < count; ++i)
             if (lengths[i] % 2 == 1 && lengths[i] > 1) odd.push_back({(long long)lengths[i], (int)i});
-            else units.push_back({(int)i, -1});
+            else R->units.push_back({(int)i, -1});
         std::stable_sort(odd.begin(), odd.end());
         for (size_t q = 0; q < odd.size(); ++q) {
-            if (q + 1 < odd.size() && odd[q + 1].first == odd[q].first &&
-                transform_len(odd[q].first, true) <= longest) {
-                units.push_back({odd[q].second, odd[q + 1].second});
+            if (q + 1 < odd.size() && odd[q + 1].first == odd[q].first) {
+                R->units.push_back({odd[q].second, odd[q + 1].second});
                 ++q;
             } else {
-                units.push_back({odd[q].second, -1});
+                R->units.push_back({odd[q].second, -1});
             }
         }
     }
-    std::vector<std::pair<double, size_t>> cost(units.size());
-    for (size_t u = 0; u < units.size(); ++u) {
-        const long long n = (long long)lengths[(size_t)units[u].first];
-        const long long M = (n % 2 == 0) ? n / 2 : n;
-        const long long L = transform_len(n, units[u].second >= 0);
-        cost[u] = {-(rg_smooth13(M) ? 1.0 : 2.0) * (double)L * std::log2((double)L + 1.0), u};
-    }
-    std::stable_sort(cost.begin(), cost.end());
-    R->order.resize(units.size());
-    for (size_t q = 0; q < units.size(); ++q) R->order[q] = units[cost[q].second];
-    if (R->dtasks.alloc(units.size() * sizeof(RaggedTask)) || R->counter.alloc(256)) {
-        err = {E_NOMEM, "ragged workspace allocation failed"};
-        return E_NOMEM;
-    }
+    if (R->counter.alloc(64 * kPhases * sizeof(int))) { err = {E_NOMEM, "ragged counter allocation failed"}; return E_NOMEM; }
     const int inst[6][2] = {{256, 0}, {256, 3}, {256, 4}, {512, 0}, {512, 4}, {1024, 0}};
     for (const auto &v : inst)
-        if (hipFuncSetAttribute(ragged_fn(v[0], v[1]),
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
+        if (hipFuncSetAttribute(ragged_fn(v[0], v[1]), hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)(lds_points_padded(8192) * sizeof(double2))) != hipSuccess) {
             (void)hipGetLastError();
             err = {E_HIP, "cannot raise the LDS limit of the ragged kernel"};
@@ -783,7 +921,7 @@ void ragged_destroy(Ragged *r) { delete r; }
 int ragged_configure(Ragged &R, int threads, int lds_max, int order, int waves, Error &err)
 {
     const bool pow2 = lds_max > 0 && (lds_max & (lds_max - 1)) == 0;
-    if ((threads != 256 && threads != 512 && threads != 1024) || !pow2 || lds_max < kBlueRow || lds_max > 8192 ||
+    if ((threads != 256 && threads != 512 && threads != 1024) || !pow2 || lds_max < 2048 || lds_max > 8192 ||
         lds_max > kPointsPerThread * threads || order < 0 || order > 3 ||
         (waves != 0 && waves != 4 && !(waves == 3 && threads == 256))) {
         err = {E_INVALID, "ragged configuration out of range (threads 256/512/1024; lds_points a power of two in "
@@ -800,26 +938,30 @@ int ragged_configure(Ragged &R, int threads, int lds_max, int order, int waves, 
     return build_tasks(R, threads, lds_max, waves, err);
 }
 
+// the reset and one persistent launch per non-empty phase
 static hipError_t ragged_launch(Ragged &R, const float *x, double2 *y, double fct, hipStream_t s)
 {
-    RaggedParams P;
-    P.x = x;
-    P.y = y;
-    P.tasks = (const RaggedTask *)R.dtasks.p;
-    P.ntasks = (int)R.tasks.size();
-    P.counter = (int *)R.counter.p;
-    P.scratch = (double2 *)R.scratch.p;
-    P.slot = R.slot;
-    P.spec = (double2 *)R.spec.p;
-    P.mode = 0;
-    P.order = R.rorder;
-    P.fct = fct;
-    P.osign = R.forward ? 1.0 : -1.0;
     hipLaunchKernelGGL(k_ragged_reset, dim3(1), dim3(64), 0, s, (int *)R.counter.p);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return launch_ragged(P, R.threads, R.waves, (unsigned)R.grid, (size_t)lds_points_padded(R.lds_max) * sizeof(double2),
-                         s);
+    const size_t lds = (size_t)lds_points_padded(R.lds_max) * sizeof(double2);
+    for (int q = 0; q < kPhases && e == hipSuccess; ++q) {
+        if (!R.phase_cnt[q]) continue;
+        RaggedParams P;
+        std::memset(&P, 0, sizeof(P));
+        P.x = x;
+        P.y = y;
+        P.tasks = (const RaggedTask *)R.dtasks.p + R.phase_off[q];
+        P.ntasks = R.phase_cnt[q];
+        P.counter = (int *)R.counter.p + 64 * q;
+        P.ws = (double2 *)R.ws.p;
+        P.spec = (double2 *)R.spec.p;
+        P.mode = 0;
+        P.order = R.rorder;
+        P.fct = fct;
+        P.osign = R.forward ? 1.0 : -1.0;
+        e = launch_ragged(P, R.threads, R.waves, (unsigned)R.phase_grid[q], lds, s);
+    }
+    return e;
 }
 
 static int check_ptrs(const Ragged &R, const float *x, double2 *y, Error &err)
@@ -938,9 +1080,9 @@ void ragged_info(const Ragged &R, int *threads, int *lds_max, int *order, int *w
     if (waves) *waves = R.waves;
     if (lds_max) *lds_max = R.lds_max;
     if (order) *order = R.rorder;
-    if (grid) *grid = R.grid;
+    if (grid) *grid = *std::max_element(R.phase_grid, R.phase_grid + kPhases);
     if (flops) *flops = R.flops;
-    if (scratch_bytes) *scratch_bytes = R.scratch.bytes;
+    if (scratch_bytes) *scratch_bytes = R.ws.bytes;
 }
 
 }  // namespace ifft

@@ -156,12 +156,15 @@ class Conv2d:
         dt = _lib.F32 if dtype == torch.float32 else _lib.F64
         _lib.check(_lib.lib().impulse_fft_conv2d_create(ctypes.byref(h), batch, height, width, kh, kw, dt))
         self.handle = h
+        self._destroy = _lib.lib().impulse_fft_conv2d_destroy
         self.dtype = dtype
 
     def __del__(self):
-        h = getattr(self, "handle", None)
-        if h is not None and h.value:
-            _lib.lib().impulse_fft_conv2d_destroy(h)
+        # the destroy entry point is bound at creation: at interpreter exit the
+        # module globals may already be gone
+        h, destroy = getattr(self, "handle", None), getattr(self, "_destroy", None)
+        if h is not None and h.value and destroy is not None:
+            destroy(h)
             self.handle = None
 
     @property

@@ -83,11 +83,13 @@ class _Plan1D:
         self.pocket = getattr(L, f"impulse_make_{self._kind}_plan")(self.length)
         if not self.pocket:
             raise RuntimeError(f"make_{self._kind}_plan failed: {L.impulse_fft_last_error().decode()}")
+        # bound at creation: at interpreter exit the module globals may be gone
+        self._destroy = getattr(L, f"impulse_destroy_{self._kind}_plan")
 
     def __del__(self):
-        p = getattr(self, "pocket", None)
-        if p:
-            getattr(lib(), f"impulse_destroy_{self._kind}_plan")(p)
+        p, destroy = getattr(self, "pocket", None), getattr(self, "_destroy", None)
+        if p and destroy is not None:
+            destroy(p)
             self.pocket = None
 
     def call(self, data, forward, norm_value):
@@ -326,12 +328,14 @@ class Plan:
                                                 strides(stride_in), strides(stride_out), len(axes), sizes(axes),
                                                 1 if forward else 0))
         self.handle = h
+        self._destroy = lib().impulse_fft_plan_destroy
         self.kind = kind
 
     def __del__(self):
-        h = getattr(self, "handle", None)
-        if h is not None and h.value:
-            lib().impulse_fft_plan_destroy(h)
+        # bound at creation: at interpreter exit the module globals may be gone
+        h, destroy = getattr(self, "handle", None), getattr(self, "_destroy", None)
+        if h is not None and h.value and destroy is not None:
+            destroy(h)
             self.handle = None
 
     def execute(self, in_ptr, out_ptr, fct=1.0, stream=0):
@@ -549,11 +553,13 @@ class FftpackPlan:
                                                     int(bool(forward)), _lib.F64, len(shape), sizes(shape),
                                                     strides(stride_in), strides(stride_out), len(axes), sizes(axes)))
         self.handle = h
+        self._destroy = lib().impulse_fft_plan_destroy
 
     def __del__(self):
-        h = getattr(self, "handle", None)
-        if h is not None and h.value:
-            lib().impulse_fft_plan_destroy(h)
+        # bound at creation: at interpreter exit the module globals may be gone
+        h, destroy = getattr(self, "handle", None), getattr(self, "_destroy", None)
+        if h is not None and h.value and destroy is not None:
+            destroy(h)
             self.handle = None
 
     def execute(self, in_ptr, out_ptr, fct=1.0, stream=0):

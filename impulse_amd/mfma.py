@@ -26,11 +26,14 @@ class MfmaFFT:
         h = ctypes.c_void_p()
         check(lib().impulse_fft_mfma_create(ctypes.byref(h), self.length, self.batch, PRECISIONS[precision]))
         self.handle = h
+        self._destroy = lib().impulse_fft_mfma_destroy
 
     def __del__(self):
-        h = getattr(self, "handle", None)
-        if h is not None and h.value:
-            lib().impulse_fft_mfma_destroy(h)
+        # the destroy entry point is bound at creation: at interpreter exit the
+        # module globals may already be gone
+        h, destroy = getattr(self, "handle", None), getattr(self, "_destroy", None)
+        if h is not None and h.value and destroy is not None:
+            destroy(h)
             self.handle = None
 
     @property

@@ -48,11 +48,14 @@ class RaggedBatch:
         check(lib().impulse_fft_ragged_create(ctypes.byref(h), len(self.lengths), sizes(self.lengths),
                                               sizes(self.in_offsets), sizes(self.out_offsets), int(bool(forward))))
         self.handle = h
+        self._destroy = lib().impulse_fft_ragged_destroy
 
     def __del__(self):
-        h = getattr(self, "handle", None)
-        if h is not None and h.value:
-            lib().impulse_fft_ragged_destroy(h)
+        # the destroy entry point is bound at creation: at interpreter exit the
+        # module globals may already be gone
+        h, destroy = getattr(self, "handle", None), getattr(self, "_destroy", None)
+        if h is not None and h.value and destroy is not None:
+            destroy(h)
             self.handle = None
 
     def _check_tensors(self, x, y):

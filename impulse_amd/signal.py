@@ -31,15 +31,17 @@ class UpfirdnPlan:
         check(lib().impulse_fft_upfirdn_create(ctypes.byref(p), int(batch), int(nt), hh.ctypes.data, hh.size,
                                                int(up), int(down), int(algorithm)))
         self.handle = p
+        self._destroy = lib().impulse_fft_upfirdn_destroy
         self.batch, self.nt, self.up, self.down = int(batch), int(nt), int(up), int(down)
         n, a, L, b = ctypes.c_size_t(), ctypes.c_int(), ctypes.c_size_t(), ctypes.c_size_t()
         check(lib().impulse_fft_upfirdn_info(p, ctypes.byref(n), ctypes.byref(a), ctypes.byref(L), ctypes.byref(b)))
         self.nout, self.algorithm, self.fft_length, self.blocks = n.value, a.value, L.value, b.value
 
     def __del__(self):
-        h = getattr(self, "handle", None)
-        if h is not None and h.value:
-            lib().impulse_fft_upfirdn_destroy(h)
+        # bound at creation: at interpreter exit the module globals may be gone
+        h, destroy = getattr(self, "handle", None), getattr(self, "_destroy", None)
+        if h is not None and h.value and destroy is not None:
+            destroy(h)
             self.handle = None
 
     def execute(self, t, y=None):
@@ -86,14 +88,28 @@ def upfirdn(t, h, up=1, down=1, algorithm=_lib.UPFIRDN_AUTO):
     second (the Matlab order the reference documents)."""
     if _is_torch(t):
         import torch
-        hh = (h.detach().cpu().numpy() if _is_torch(h) else np.asarray(h)).astype(np.float64).ravel()
         if t.dim() != 1:
             raise ValueError("upfirdn takes a rank-1 signal")
+        hn = h.detach().cpu().numpy() if _is_torch(h) else np.asarray(h)
+        if np.iscomplexobj(hn):
+            # a complex filter runs as its real and imaginary taps, as on the host path
+            hr, hi = np.ascontiguousarray(hn.real, np.float64), np.ascontiguousarray(hn.imag, np.float64)
+            if np.any(hi):
+                return upfirdn(t, hr, up, down, algorithm) + 1j * upfirdn(t, hi, up, down, algorithm)
+            hn = hr
+        hh = np.ascontiguousarray(hn, dtype=np.float64).ravel()
         if t.is_complex():
             parts = torch.stack([t.real, t.imag]).to(torch.float64).contiguous()
             y = _plan(t.shape[0], hh, up, down, 2, algorithm).execute(parts)
-            return torch.complex(y[0], y[1])
+            yc = torch.complex(y[0], y[1])
+            return yc if t.dtype == torch.complex128 else yc.to(t.dtype)
         y = _plan(t.shape[0], hh, up, down, 1, algorithm).execute(t.to(torch.float64).contiguous())[0]
+        if not t.dtype.is_floating_point:
+            # integer signal: an integer filter gives integer outputs (rounded, not truncated);
+            # a real filter gives float64, as on the host path
+            if np.issubdtype(hn.dtype, np.integer):
+                return torch.round(y).to(torch.promote_types(t.dtype, torch.from_numpy(np.zeros(1, hn.dtype)).dtype))
+            return y
         return y if t.dtype == torch.float64 else y.to(t.dtype)
     t = np.asarray(t)
     h = np.asarray(h)

@@ -132,6 +132,34 @@ def test_batch_and_device_tensors():
     assert rel_l2(rd, S.resample(t[0], up=3, down=2)) < TOL
 
 
+def test_device_tensor_complex_filter_and_integers():
+    """A complex filter on the torch path runs as its real and imaginary taps (the host path's
+    rule); integer signals with integer filters round, and keep an integer dtype."""
+    import torch
+    rng = np.random.default_rng(9)
+    t = rng.uniform(-1, 1, 3000)
+    h = rng.standard_normal(41) + 1j * rng.standard_normal(41)
+    want = oracle.upfirdn(t, h.real, 3, 2) + 1j * oracle.upfirdn(t, h.imag, 3, 2)
+    for algo in ALGOS:
+        y = S.upfirdn(torch.from_numpy(t).cuda(), torch.from_numpy(h), 3, 2, algorithm=algo)
+        assert y.is_complex() and y.is_cuda
+        assert rel_l2(y.cpu().numpy(), want) < TOL
+        # complex signal, complex filter: (a + ib) * (c + id) by the four real products
+        tc = torch.complex(torch.from_numpy(t), torch.from_numpy(-0.5 * t)).cuda()
+        yc = S.upfirdn(tc, h, 3, 2, algorithm=algo).cpu().numpy()
+        assert rel_l2(yc, want * (1 - 0.5j)) < TOL
+    # integer KATs on device tensors: exact integers, integer dtype
+    for (ti, hi, up, down), expect in KATS[:3] + KATS[4:5]:
+        yi = S.upfirdn(torch.tensor(ti, dtype=torch.int64).cuda(), np.asarray(hi, np.int64), up, down)
+        assert yi.dtype == torch.int64
+        assert yi.cpu().tolist() == expect
+    # negative sums: rounding, not truncation toward zero, and the int32 dtype kept
+    ti = torch.tensor([3, -7, 11, 5], dtype=torch.int32).cuda()
+    yi = S.upfirdn(ti, np.array([2, -1, 4], np.int32), 2, 1)
+    assert yi.dtype == torch.int32
+    assert yi.cpu().tolist() == [6, -3, -2, 7, -6, -11, 54, -5, 20]
+
+
 T = np.array([1.0, 1.2, -0.3, -1.0, 2.0])
 E32 = [1.0006061736, 1.2269863677, 1.0003755977, -0.3001818521, -1.4444776098, 0.1147783685, 2.0012123471,
        0.9175789829]

@@ -118,6 +118,16 @@ def measured_traffic(kernel, threads):
     return int(sum(vals) / len(vals)) if vals and len(vals) == len(threads) else None
 
 
+def form_traffic(form):
+    """HBM bytes of one call of a form (ragged execute, convolution forward / backward) from the
+    committed rocprofv3 --pmc summary, scaled like the drop-in's, or None."""
+    try:
+        with open(os.path.join(HERE, "profiles", "pmc_traffic.json")) as f:
+            return json.load(f).get("forms", {}).get(form)
+    except (OSError, ValueError):
+        return None
+
+
 def torch_dtype(torch, s):
     return {"c128": torch.complex128, "f64": torch.float64}[s]
 
@@ -360,7 +370,19 @@ def ragged_form(torch, reps, loop_sample=600):
                            "projected_ms": round(flops / (loop_gflops * 1e9) * 1e3, 1),
                            "speedup": round(gf / loop_gflops, 1)},
         "max_rel_l2_vs_fp64_plan": worst,
+        **ragged_traffic(comp, ms),
     }
+
+
+def ragged_traffic(comp, ms):
+    """The profiled execute's HBM traffic against the compulsory bytes (counters committed in
+    profiles/pmc_traffic.json), or nothing if no summary is there."""
+    tr = form_traffic("ragged")
+    if not tr or not tr.get("hbm_bytes_per_execute"):
+        return {}
+    t = tr["hbm_bytes_per_execute"]
+    return {"traffic": {"hbm_bytes": t, "compulsory_bytes": comp, "over_compulsory": round(t / comp, 2),
+                        "hbm_GBs": round(t / (ms * 1e-3) / 1e9, 1), "profiled_config": tr.get("config")}}
 
 
 def mfma_form(torch, reps):
@@ -457,10 +479,27 @@ def conv_form(torch, reps):
     # backward (recomputing spectra) r2c-adjoint, c2r-adjoints for both gradients
     n = P * Q
     fl_r = 2.5 * n * math.log2(n)
-    return {"form": "conv2d", "workload": "16 x 2048^2 f32 images * 63x63 kernel, 'same', FP64 arithmetic",
-            "grid": [P, Q], "forward_ms": round(fwd, 3), "backward_ms": round(bwd, 3),
-            "forward_gflops": round((B + 1 + B) * fl_r / (fwd * 1e-3) / 1e9, 1),
-            "note": "backward recomputes the image and kernel spectra instead of storing them"}
+    # compulsory bytes: forward reads x and k, writes y; backward reads x, k and dy, writes dx and dk
+    img, ker = B * H * H * 4, K * K * 4
+    comp_f, comp_b = 2 * img + ker, 3 * img + 2 * ker
+    out = {"form": "conv2d", "workload": "16 x 2048^2 f32 images * 63x63 kernel, 'same', FP64 arithmetic",
+           "grid": [P, Q], "forward_ms": round(fwd, 3), "backward_ms": round(bwd, 3),
+           "forward_gflops": round((B + 1 + B) * fl_r / (fwd * 1e-3) / 1e9, 1),
+           "forward_compulsory_GBs": round(comp_f / (fwd * 1e-3) / 1e9, 1),
+           "forward_hbm_roofline_frac": round(comp_f / (fwd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "backward_compulsory_GBs": round(comp_b / (bwd * 1e-3) / 1e9, 1),
+           "backward_hbm_roofline_frac": round(comp_b / (bwd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "note": "backward recomputes the image and kernel spectra instead of storing them"}
+    tr = form_traffic("conv2d")
+    if tr:
+        # counters of the profiled build (profiles/pmc_traffic.json): HBM bytes per call and the
+        # rate they imply at this run's time
+        for kind, ms, comp in (("forward", fwd, comp_f), ("backward", bwd, comp_b)):
+            t = tr.get(kind, {}).get("hbm_bytes_per_call")
+            if t:
+                out[f"{kind}_traffic"] = {"hbm_bytes": t, "over_compulsory": round(t / comp, 2),
+                                          "hbm_GBs": round(t / (ms * 1e-3) / 1e9, 1)}
+    return out
 
 
 def _timed(torch, fn, reps):

@@ -157,4 +157,55 @@ __device__ __noinline__ void lds_fft(double2 *lds, int L, int count, double s, i
     if (ns == 0) __syncthreads();
 }
 
+// The same transform with every stage inlined into the caller: no call frames,
+// so no per-lane stack in scratch memory.  A call's frame (return address and
+// the callee-saved registers, 150-250 bytes a lane) lives in scratch, and with
+// thousands of resident waves those frames outgrow the L2 and are written back
+// on every stage -- more traffic than the transform's data in a streaming
+// kernel.  PPT bounds the points a thread owns (count L <= PPT blockDim.x), so a
+// radix-R stage takes ceil(PPT / R) butterflies per thread.
+template <int PPT>
+__host__ __device__ constexpr int nb_ppt(int R) { return (PPT + R - 1) / R; }
+
+// SMALL: only radices 2, 3, 4, 5, 7 and 8 are compiled in, for 7-smooth lengths
+// with the radix-8 order (order bit 1); the largest stage then holds at most 14
+// values a thread, which keeps a 1,024-thread workgroup within 128 registers.
+// The caller guarantees the length (no other radix may occur).
+template <int PPT, bool SMALL, typename TW>
+__device__ __forceinline__ void lds_fft_inline(double2 *lds, int L, int count, double s, int order, TW tw)
+{
+    int ns;
+    unsigned long long codes = radix_codes(L, order, ns);
+    const int NT = blockDim.x / count;
+    const int t = threadIdx.x / NT, i = threadIdx.x % NT;
+    const int base = t * L;
+    int p = 1;
+    for (int st = 0; st < ns; ++st, codes >>= 4) {
+        const int code = (int)(codes & 15);
+        if constexpr (SMALL) {
+            switch (code) {
+                case 1: rstage_body<2, nb_ppt<PPT>(2)>(lds, base, L, p, i, NT, s, tw); p *= 2; break;
+                case 2: rstage_body<3, nb_ppt<PPT>(3)>(lds, base, L, p, i, NT, s, tw); p *= 3; break;
+                case 3: rstage_body<4, nb_ppt<PPT>(4)>(lds, base, L, p, i, NT, s, tw); p *= 4; break;
+                case 4: rstage_body<5, nb_ppt<PPT>(5)>(lds, base, L, p, i, NT, s, tw); p *= 5; break;
+                case 5: rstage_body<7, nb_ppt<PPT>(7)>(lds, base, L, p, i, NT, s, tw); p *= 7; break;
+                default: rstage_body<8, nb_ppt<PPT>(8)>(lds, base, L, p, i, NT, s, tw); p *= 8; break;
+            }
+        } else {
+            switch (code) {
+                case 1: rstage_body<2, nb_ppt<PPT>(2)>(lds, base, L, p, i, NT, s, tw); p *= 2; break;
+                case 2: rstage_body<3, nb_ppt<PPT>(3)>(lds, base, L, p, i, NT, s, tw); p *= 3; break;
+                case 3: rstage_body<4, nb_ppt<PPT>(4)>(lds, base, L, p, i, NT, s, tw); p *= 4; break;
+                case 4: rstage_body<5, nb_ppt<PPT>(5)>(lds, base, L, p, i, NT, s, tw); p *= 5; break;
+                case 5: rstage_body<7, nb_ppt<PPT>(7)>(lds, base, L, p, i, NT, s, tw); p *= 7; break;
+                case 6: rstage_body<8, nb_ppt<PPT>(8)>(lds, base, L, p, i, NT, s, tw); p *= 8; break;
+                case 7: rstage_body<11, nb_ppt<PPT>(11)>(lds, base, L, p, i, NT, s, tw); p *= 11; break;
+                case 8: rstage_body<13, nb_ppt<PPT>(13)>(lds, base, L, p, i, NT, s, tw); p *= 13; break;
+                default: rstage_body<16, nb_ppt<PPT>(16)>(lds, base, L, p, i, NT, s, tw); p *= 16; break;
+            }
+        }
+    }
+    if (ns == 0) __syncthreads();
+}
+
 }  // namespace ifft

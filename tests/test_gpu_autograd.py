@@ -128,6 +128,42 @@ def test_conv_matches_definition(shape, ks):
     assert np.linalg.norm(y32.cpu().numpy() - ref32) <= 1e-6 * np.linalg.norm(ref32)
 
 
+def _direct_adjoints(x, k, g):
+    """gx[p, q] = sum_ab g[p - cy + a, q - cx + b] k[a, b] and gk[a, b] = sum g[i, j] x[i + cy - a, j + cx - b]:
+    the gradients of sum(y * g) for the 'same' convolution y, as direct sums."""
+    B, H, W = x.shape
+    KH, KW = k.shape
+    cy, cx = (KH - 1) // 2, (KW - 1) // 2
+    gx, gk = np.zeros_like(x), np.zeros_like(k)
+    for a in range(KH):
+        for b in range(KW):
+            di, dj = cy - a, cx - b
+            i0, i1 = max(0, -di), min(H, H - di)
+            j0, j1 = max(0, -dj), min(W, W - dj)
+            gx[:, i0 + di:i1 + di, j0 + dj:j1 + dj] += g[:, i0:i1, j0:j1] * k[a, b]
+            gk[a, b] = np.sum(g[:, i0:i1, j0:j1] * x[:, i0 + di:i1 + di, j0 + dj:j1 + dj])
+    return gx, gk
+
+
+# the last two shapes' grids (columns or rows beyond 2,560 points) are beyond the fused LDS
+# kernels and take the plan-based path
+@pytest.mark.parametrize("shape,ks", [((2, 20, 24), (5, 7)), ((3, 17, 16), (4, 3)), ((1, 2050, 8), (63, 3)),
+                                      ((2, 4, 2048), (3, 63)), ((1, 4700, 6), (9, 3)), ((1, 3, 5200), (3, 3))])
+def test_conv_forward_and_backward_match_direct_sums(shape, ks):
+    print(f"case conv direct sums {shape} {ks}", flush=True)
+    x = synth.uniform(shape, 0, 1, seed=17)
+    k = synth.normal(ks, seed=18)
+    g = synth.normal(shape, seed=19)
+    xt, kt = _t(x).requires_grad_(True), _t(k).requires_grad_(True)
+    y = AG.fft_conv2d(xt, kt)
+    torch.sum(y * _t(g)).backward()
+    rx, rk = _direct_adjoints(x, k, g)
+    assert np.linalg.norm(y.detach().cpu().numpy() - _direct_conv_same(x, k)) <= 1e-12 * np.linalg.norm(
+        _direct_conv_same(x, k))
+    assert np.linalg.norm(xt.grad.cpu().numpy() - rx) <= 1e-12 * np.linalg.norm(rx)
+    assert np.linalg.norm(kt.grad.cpu().numpy() - rk) <= 1e-12 * np.linalg.norm(rk)
+
+
 def test_gradcheck_conv():
     x = torch.rand(2, 12, 11, dtype=torch.float64, device="cuda", requires_grad=True)
     k = torch.randn(5, 3, dtype=torch.float64, device="cuda", requires_grad=True)
@@ -175,6 +211,31 @@ def test_conv_full_size_finite_differences():
           f"h=1e-2 {worst[1e-2]:.3e}")
     assert worst[1e-6] <= 2e-3
     assert worst[1e-2] <= 1e-7
+
+
+def test_conv_full_size_float32_io():
+    """The C5 workload with float32 images, kernel and gradients (the bench's dtype) against the
+    float64 path on the same values: FP64 arithmetic, so only the float32 rounding of the outputs
+    differs (rel-L2 bar 1e-6)."""
+    B, H, K = 16, 2048, 63
+    print("case conv 16x2048^2 k63 float32 io", flush=True)
+    x32 = synth.uniform_device(B * H * H, 0.0, 1.0, seed=11, dtype=torch.float32).view(B, H, H)
+    k32 = _t(synth.normal((K, K), seed=12, sigma=1.0 / K).astype(np.float32))
+    g32 = synth.uniform_device(B * H * H, -1.0, 1.0, seed=13, dtype=torch.float32).view(B, H, H)
+    c32 = AG._conv(B, H, H, K, K, torch.float32)
+    c64 = AG._conv(B, H, H, K, K, torch.float64)
+    y32, gx32, gk32 = torch.empty_like(x32), torch.empty_like(x32), torch.empty_like(k32)
+    c32.forward(x32, k32, y32)
+    c32.backward(x32, k32, g32, gx32, gk32)
+    x64, k64, g64 = x32.double(), k32.double(), g32.double()
+    y64, gx64, gk64 = torch.empty_like(x64), torch.empty_like(x64), torch.empty_like(k64)
+    c64.forward(x64, k64, y64)
+    c64.backward(x64, k64, g64, gx64, gk64)
+    torch.cuda.synchronize()
+    for a, b in ((y32, y64), (gx32, gx64), (gk32, gk64)):
+        assert a.dtype == torch.float32
+        err = float(torch.linalg.vector_norm(a.double() - b) / torch.linalg.vector_norm(b))
+        assert err <= 1e-6, err
 
 
 def test_gradients_through_conjugate_views():

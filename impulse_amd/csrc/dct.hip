@@ -323,7 +323,9 @@ constexpr int kR2rFusedMax = 4096;   // longest N/2 the fused kernel takes (512 
 #define DCT_FUSED_WPE 3
 #endif
 
-template <int TYPE, int THREADS>
+// SMALL: a 7-smooth M, transformed by inlined radix-2..8 stages (no call frames in
+// scratch memory); other lengths call the per-radix stage functions.
+template <int TYPE, int THREADS, bool SMALL>
 __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(DCT_FUSED_WPE)))
 k_r2r_fused(const R2rParams P, int count)
 {
@@ -341,7 +343,8 @@ k_r2r_fused(const R2rParams P, int count)
             for (int m = i; m < M; m += NT) zl[lpad(t * M + m)] = half_pre<TYPE>(P, x, m);
         }
         __syncthreads();
-        lds_fft(zl, M, count, s, 0, TwTable{P.ftw, M});   // ends with a barrier
+        if constexpr (SMALL) lds_fft_inline<kPointsPerThread, true>(zl, M, count, s, 2, TwTable{P.ftw, M});
+        else lds_fft(zl, M, count, s, 0, TwTable{P.ftw, M});   // both end with a barrier
         if (valid) {
             double *yl = P.dst + line_off(P.out, line);
             auto Zat = [&](long long m) { return zl[lpad(t * M + (int)m)]; };
@@ -358,6 +361,13 @@ static bool smooth13_r2r(long long n)
     return n == 1;
 }
 
+static bool smooth7_r2r(long long n)
+{
+    for (int p : {2, 3, 5, 7})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
 // workgroup size of the fused kernel for N/2 = M, and lines per workgroup: the
 // largest power of two <= 64 with count M <= 8 threads
 static int fused_threads(long long M) { return M <= 2048 ? 256 : 512; }
@@ -369,23 +379,31 @@ static int fused_count(long long M)
     return c;
 }
 
-template <int THREADS>
-static hipError_t launch_fused_t(int type, const R2rParams &Q, unsigned grid, size_t lds, int count, hipStream_t s)
+template <int THREADS, bool SMALL>
+static hipError_t launch_fused_ts(int type, const R2rParams &Q, unsigned grid, size_t lds, int count, hipStream_t s)
 {
-    static bool attr_done = false;
-    if (!attr_done) {
-        for (const void *f : {(const void *)k_r2r_fused<2, THREADS>, (const void *)k_r2r_fused<3, THREADS>,
-                              (const void *)k_r2r_fused<4, THREADS>})
-            if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024) != hipSuccess)
-                (void)hipGetLastError();
-        attr_done = true;
-    }
     switch (type) {
-        case 2: hipLaunchKernelGGL((k_r2r_fused<2, THREADS>), dim3(grid), dim3(THREADS), lds, s, Q, count); break;
-        case 3: hipLaunchKernelGGL((k_r2r_fused<3, THREADS>), dim3(grid), dim3(THREADS), lds, s, Q, count); break;
-        default: hipLaunchKernelGGL((k_r2r_fused<4, THREADS>), dim3(grid), dim3(THREADS), lds, s, Q, count); break;
+        case 2: hipLaunchKernelGGL((k_r2r_fused<2, THREADS, SMALL>), dim3(grid), dim3(THREADS), lds, s, Q, count); break;
+        case 3: hipLaunchKernelGGL((k_r2r_fused<3, THREADS, SMALL>), dim3(grid), dim3(THREADS), lds, s, Q, count); break;
+        default: hipLaunchKernelGGL((k_r2r_fused<4, THREADS, SMALL>), dim3(grid), dim3(THREADS), lds, s, Q, count); break;
     }
     return hipGetLastError();
+}
+
+template <int THREADS>
+static hipError_t launch_fused_t(int type, bool small, const R2rParams &Q, unsigned grid, size_t lds, int count,
+                                 hipStream_t s)
+{
+    static std::once_flag attr_once;
+    std::call_once(attr_once, [] {
+        for (const void *f : {(const void *)k_r2r_fused<2, THREADS, false>, (const void *)k_r2r_fused<3, THREADS, false>,
+                              (const void *)k_r2r_fused<4, THREADS, false>, (const void *)k_r2r_fused<2, THREADS, true>,
+                              (const void *)k_r2r_fused<3, THREADS, true>, (const void *)k_r2r_fused<4, THREADS, true>})
+            if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024) != hipSuccess)
+                (void)hipGetLastError();
+    });
+    return small ? launch_fused_ts<THREADS, true>(type, Q, grid, lds, count, s)
+                 : launch_fused_ts<THREADS, false>(type, Q, grid, lds, count, s);
 }
 
 struct R2r {
@@ -573,8 +591,9 @@ int r2r_execute(Plan &P, const void *in, void *out, double fct, hipStream_t s, E
             const long long groups = (A.lines + count - 1) / count;
             const size_t lds = (size_t)lds_points_padded((int)(count * A.L)) * sizeof(double2);
             const unsigned grid = (unsigned)std::min<long long>(groups, 1 << 20);
-            hipError_t e = fused_threads(A.L) == 256 ? launch_fused_t<256>(R.type, Q, grid, lds, count, s)
-                                                     : launch_fused_t<512>(R.type, Q, grid, lds, count, s);
+            const bool small = smooth7_r2r(A.L);
+            hipError_t e = fused_threads(A.L) == 256 ? launch_fused_t<256>(R.type, small, Q, grid, lds, count, s)
+                                                     : launch_fused_t<512>(R.type, small, Q, grid, lds, count, s);
             if (e != hipSuccess) { err = {E_HIP, std::string("DCT/DST launch: ") + hipGetErrorString(e)}; return E_HIP; }
             continue;
         }

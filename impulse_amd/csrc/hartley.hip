@@ -183,7 +183,9 @@ __global__ void k_rfp_genuine(const GenParams G)
 // X[k], X[M-k] = (Z[k] + conj Z[M-k]) / 2 -+ i/2 W_N^{+-k} (Z[k] - conj Z[M-k]),
 // and written as halfcomplex or Hartley values; `count` lines per workgroup.
 constexpr int kRfpFusedMax = 4096;
-template <int MODE, int THREADS>
+// SMALL: a 7-smooth M, transformed by inlined radix-2..8 stages (no call frames in scratch
+// memory); other lengths call the per-radix stage functions.
+template <int MODE, int THREADS, bool SMALL>
 __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(3)))
 k_rfp_fused(const RfpParams P, int count, int flip, double fct)
 {
@@ -205,7 +207,8 @@ k_rfp_fused(const RfpParams P, int count, int flip, double fct)
             }
         }
         __syncthreads();
-        lds_fft(zl, M, count, -1.0, 0, TwTable{P.ftw, M});   // ends with a barrier
+        if constexpr (SMALL) lds_fft_inline<kPointsPerThread, true>(zl, M, count, -1.0, 2, TwTable{P.ftw, M});
+        else lds_fft(zl, M, count, -1.0, 0, TwTable{P.ftw, M});   // both end with a barrier
         if (valid) {
             double *y = P.rdst + rfp_line_off(P.lout, line);
             const long long sa = P.sa_out;
@@ -257,22 +260,37 @@ static int rfp_fused_count(long long M)
     return c;
 }
 
-template <int THREADS>
-static hipError_t launch_rfp_fused(int mode, const RfpParams &Q, unsigned grid, size_t lds, int count, int flip,
-                                   double fct, hipStream_t s)
+template <int THREADS, bool SMALL>
+static hipError_t launch_rfp_fused_s(int mode, const RfpParams &Q, unsigned grid, size_t lds, int count, int flip,
+                                     double fct, hipStream_t s)
 {
-    static bool attr_done = false;
-    if (!attr_done) {
-        for (const void *f : {(const void *)k_rfp_fused<0, THREADS>, (const void *)k_rfp_fused<1, THREADS>})
+    if (mode == 0)
+        hipLaunchKernelGGL((k_rfp_fused<0, THREADS, SMALL>), dim3(grid), dim3(THREADS), lds, s, Q, count, flip, fct);
+    else
+        hipLaunchKernelGGL((k_rfp_fused<1, THREADS, SMALL>), dim3(grid), dim3(THREADS), lds, s, Q, count, flip, fct);
+    return hipGetLastError();
+}
+
+template <int THREADS>
+static hipError_t launch_rfp_fused(int mode, bool small, const RfpParams &Q, unsigned grid, size_t lds, int count,
+                                   int flip, double fct, hipStream_t s)
+{
+    static std::once_flag attr_once;
+    std::call_once(attr_once, [] {
+        for (const void *f : {(const void *)k_rfp_fused<0, THREADS, false>, (const void *)k_rfp_fused<1, THREADS, false>,
+                              (const void *)k_rfp_fused<0, THREADS, true>, (const void *)k_rfp_fused<1, THREADS, true>})
             if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024) != hipSuccess)
                 (void)hipGetLastError();
-        attr_done = true;
-    }
-    if (mode == 0)
-        hipLaunchKernelGGL((k_rfp_fused<0, THREADS>), dim3(grid), dim3(THREADS), lds, s, Q, count, flip, fct);
-    else
-        hipLaunchKernelGGL((k_rfp_fused<1, THREADS>), dim3(grid), dim3(THREADS), lds, s, Q, count, flip, fct);
-    return hipGetLastError();
+    });
+    return small ? launch_rfp_fused_s<THREADS, true>(mode, Q, grid, lds, count, flip, fct, s)
+                 : launch_rfp_fused_s<THREADS, false>(mode, Q, grid, lds, count, flip, fct, s);
+}
+
+static bool smooth7_rfp(long long n)
+{
+    for (int p : {2, 3, 5, 7})
+        while (n % p == 0) n /= p;
+    return n == 1;
 }
 
 struct Rfp {
@@ -538,8 +556,10 @@ int rfp_execute(Plan &P, const void *in, void *out, double fct, hipStream_t s, E
             Q.ftw = (const double2 *)A.tab->p;
             const int mode = R.kind == 7 ? 0 : 1;
             const int flip = R.kind == 7 && R.forward && !R.r2h;
-            hipError_t e = rfp_fused_threads(M) == 256 ? launch_rfp_fused<256>(mode, Q, grid, lds, count, flip, f, s)
-                                                       : launch_rfp_fused<512>(mode, Q, grid, lds, count, flip, f, s);
+            const bool small = smooth7_rfp(M);
+            hipError_t e = rfp_fused_threads(M) == 256
+                               ? launch_rfp_fused<256>(mode, small, Q, grid, lds, count, flip, f, s)
+                               : launch_rfp_fused<512>(mode, small, Q, grid, lds, count, flip, f, s);
             RFP_TRY_HIP(e);
             continue;
         }

@@ -84,7 +84,8 @@ struct RaggedParams {
     double2 *y;
     const RaggedTask *tasks;
     int ntasks;
-    int *counter;
+    int *counter;              // one counter per queue, 64 ints apart
+    int qoff[8], qcnt[8];      // the phase's eight queues (one per XCD): first task and task count
     double2 *ws;               // workspaces of the split transforms
     double2 *scratch;          // spectrum mode: per workgroup, `slot` complex points
     long long slot;
@@ -348,7 +349,7 @@ __device__ void self_split(const RaggedTask &T, double2 *lds, const double2 *ws,
 
 __global__ void k_ragged_reset(int *counter)
 {
-    if (threadIdx.x < kPhases) counter[64 * threadIdx.x] = 0;
+    if (threadIdx.x < 8 * kPhases) counter[64 * threadIdx.x] = 0;
 }
 
 // the chirp kernel's spectrum of one Bluestein length into the table, natural
@@ -391,19 +392,38 @@ __device__ void spectrum_task(const RaggedTask &T, const RaggedParams &P, double
 // One instance per workgroup size and register budget: WPE = 4 caps the
 // registers at 128 per lane (four waves per SIMD, some spills), WPE = 0 lets
 // the compiler take what the workgroup size allows (1024 threads: 128).  One
-// launch per phase: the workgroups claim the phase's tasks in order.
+// launch per phase.  A phase's tasks sit in eight queues, one per XCD, each
+// holding whole signals: workgroup b runs on XCD b mod 8 and claims from that
+// queue first, so the tiles of one signal run on one XCD at about the same
+// time and their short strided runs merge in that XCD's L2.  A workgroup whose
+// queue is empty takes tasks from the next queues in turn; every workgroup
+// leaves once all eight are empty.
 template <int MAXT, int WPE>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ? WPE : 1))) k_ragged(const RaggedParams P)
 {
     extern __shared__ double2 lds[];
     __shared__ int s_task;
     const int tid = threadIdx.x, nthr = blockDim.x;
+    const int home = (int)(blockIdx.x & 7);
+    int visited = 0;   // queues found empty so far (thread 0's count is the one that matters)
     for (;;) {
-        if (tid == 0) s_task = atomicAdd(P.counter, 1);
+        if (tid == 0) {
+            int t = -1;
+            while (visited < 8) {
+                const int q = (home + visited) & 7;
+                const int k = atomicAdd(P.counter + 64 * q, 1);
+                if (k < P.qcnt[q]) {
+                    t = P.qoff[q] + k;
+                    break;
+                }
+                ++visited;
+            }
+            s_task = t;
+        }
         __syncthreads();
         const int task = __builtin_amdgcn_readfirstlane(s_task);   // uniform: the task record loads as scalars
         __syncthreads();
-        if (task >= P.ntasks) break;
+        if (task < 0) break;
         const RaggedTask T = P.tasks[task];
         if (P.mode == 1) {
             spectrum_task(T, P, lds, P.scratch + (long long)blockIdx.x * P.slot);
@@ -520,6 +540,7 @@ struct Ragged {
     // derived for the configuration: the tasks of the phases back to back
     std::vector<RaggedTask> tasks;
     int phase_off[kPhases] = {0, 0, 0, 0}, phase_cnt[kPhases] = {0, 0, 0, 0}, phase_grid[kPhases] = {0, 0, 0, 0};
+    int queue_off[kPhases][8] = {}, queue_cnt[kPhases][8] = {};   // per phase: the XCD queues within it
     DevBuf dtasks, counter, ws;
     // chirp-kernel spectra, one per distinct Bluestein length, natural order
     std::vector<std::pair<long long, long long>> spec_index;   // (2 n + pair, offset), sorted
@@ -719,9 +740,14 @@ static int resident_grid(const Ragged &R, int threads, int waves, size_t lds)
 
 static int build_tasks(Ragged &R, int threads, int lds_max, int waves, Error &err)
 {
-    std::vector<RaggedTask> ph[kPhases];
+    // each unit's tasks per phase, then the units dealt to eight queues per phase (largest unit
+    // first, to the queue with the least work), each queue ordered by decreasing task cost
+    std::vector<std::vector<RaggedTask>> unit_ph[kPhases];
+    for (auto &v : unit_ph) v.resize(R.units.size());
     long long ws = 0;
-    for (const auto &u : R.units) {
+    for (size_t ui = 0; ui < R.units.size(); ++ui) {
+        const auto &u = R.units[ui];
+        std::vector<RaggedTask> ph[kPhases];
         const size_t i = (size_t)u.first;
         const bool pair = u.second >= 0;
         const long long n = (long long)R.n[i];
@@ -740,15 +766,35 @@ static int build_tasks(Ragged &R, int threads, int lds_max, int waves, Error &er
                                       " has no tile split for this configuration"};
             return E_UNSUPPORTED;
         }
+        for (int q = 0; q < kPhases; ++q) unit_ph[q][ui].swap(ph[q]);
     }
     std::vector<RaggedTask> all;
     int off[kPhases], cnt[kPhases];
+    int qoff[kPhases][8], qcnt[kPhases][8];
     for (int q = 0; q < kPhases; ++q) {
-        std::stable_sort(ph[q].begin(), ph[q].end(),
-                         [](const RaggedTask &a, const RaggedTask &b) { return task_cost(a) > task_cost(b); });
+        std::vector<std::pair<double, size_t>> order;   // (cost of the unit's tasks in this phase, unit)
+        for (size_t ui = 0; ui < unit_ph[q].size(); ++ui) {
+            double c = 0.0;
+            for (const auto &t : unit_ph[q][ui]) c += task_cost(t);
+            if (!unit_ph[q][ui].empty()) order.push_back({c, ui});
+        }
+        std::stable_sort(order.begin(), order.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+        std::vector<RaggedTask> queue[8];
+        double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (const auto &o : order) {
+            const int k = (int)(std::min_element(load, load + 8) - load);
+            load[k] += o.first;
+            queue[k].insert(queue[k].end(), unit_ph[q][o.second].begin(), unit_ph[q][o.second].end());
+        }
         off[q] = (int)all.size();
-        cnt[q] = (int)ph[q].size();
-        all.insert(all.end(), ph[q].begin(), ph[q].end());
+        for (int k = 0; k < 8; ++k) {
+            std::stable_sort(queue[k].begin(), queue[k].end(),
+                             [](const RaggedTask &a, const RaggedTask &b) { return task_cost(a) > task_cost(b); });
+            qoff[q][k] = (int)all.size() - off[q];
+            qcnt[q][k] = (int)queue[k].size();
+            all.insert(all.end(), queue[k].begin(), queue[k].end());
+        }
+        cnt[q] = (int)all.size() - off[q];
     }
     if (all.size() > (size_t)INT_MAX) { err = {E_UNSUPPORTED, "too many ragged tasks"}; return E_UNSUPPORTED; }
     const size_t wsb = (size_t)std::max<long long>(ws, 1) * sizeof(double2);
@@ -768,6 +814,10 @@ static int build_tasks(Ragged &R, int threads, int lds_max, int waves, Error &er
         R.phase_off[q] = off[q];
         R.phase_cnt[q] = cnt[q];
         R.phase_grid[q] = std::max(1, std::min(res, cnt[q]));
+        for (int k = 0; k < 8; ++k) {
+            R.queue_off[q][k] = qoff[q][k];
+            R.queue_cnt[q][k] = qcnt[q][k];
+        }
     }
     R.tasks.swap(all);
     R.threads = threads;
@@ -838,6 +888,8 @@ static int build_spectra(Ragged &R, Error &err)
     RaggedParams P{};
     P.tasks = (const RaggedTask *)dt.p;
     P.ntasks = (int)st.size();
+    P.qoff[0] = 0;
+    P.qcnt[0] = (int)st.size();   // one queue; the other seven are empty
     P.counter = (int *)R.counter.p;
     P.scratch = (double2 *)scr.p;
     P.slot = slot;
@@ -899,7 +951,7 @@ int ragged_create(Ragged **out, size_t count, const size_t *lengths, const size_
             }
         }
     }
-    if (R->counter.alloc(64 * kPhases * sizeof(int))) { err = {E_NOMEM, "ragged counter allocation failed"}; return E_NOMEM; }
+    if (R->counter.alloc(64 * 8 * kPhases * sizeof(int))) { err = {E_NOMEM, "ragged counter allocation failed"}; return E_NOMEM; }
     const int inst[6][2] = {{256, 0}, {256, 3}, {256, 4}, {512, 0}, {512, 4}, {1024, 0}};
     for (const auto &v : inst)
         if (hipFuncSetAttribute(ragged_fn(v[0], v[1]), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -952,7 +1004,11 @@ static hipError_t ragged_launch(Ragged &R, const float *x, double2 *y, double fc
         P.y = y;
         P.tasks = (const RaggedTask *)R.dtasks.p + R.phase_off[q];
         P.ntasks = R.phase_cnt[q];
-        P.counter = (int *)R.counter.p + 64 * q;
+        P.counter = (int *)R.counter.p + 64 * 8 * q;
+        for (int k = 0; k < 8; ++k) {
+            P.qoff[k] = R.queue_off[q][k];
+            P.qcnt[k] = R.queue_cnt[q][k];
+        }
         P.ws = (double2 *)R.ws.p;
         P.spec = (double2 *)R.spec.p;
         P.mode = 0;

@@ -83,8 +83,10 @@ def test_golden_nd(golden_r2r):
         assert rel_l2(F.dst(v[::-1], [0, 1, 2], t, 1.0, t == 3), g[f"nd_dst{t}_012"]) < TOL, t
 
 
-LENGTHS = [1, 2, 3, 4, 5, 6, 7, 9, 15, 16, 17, 31, 33, 63, 64, 65, 100, 127, 128, 129, 243, 255, 257, 500,
-           1000, 1023, 1024, 1025, 2047, 3001, 4096, 4097]
+# even lengths with a 7-smooth half (fused kernel, inlined radix-2..8 stages: 6, 16, 64, 100, 128, 500,
+# 1000, 1024, 4096) and with 11 or 13 in the half (fused kernel, per-radix stage calls: 22, 52, 286)
+LENGTHS = [1, 2, 3, 4, 5, 6, 7, 9, 15, 16, 17, 22, 31, 33, 52, 63, 64, 65, 100, 127, 128, 129, 243, 255, 257,
+           286, 500, 1000, 1023, 1024, 1025, 2047, 3001, 4096, 4097]
 
 
 @pytest.mark.parametrize("name,t", KINDS)

@@ -59,7 +59,9 @@ def test_golden(golden_r2r):
     assert rel_l2(F.r2r_genuine_hartley(v[:, ::-1, :], [2, 0, 1], 2.0), g["nd_genuine_201"]) < TOL
 
 
-LENGTHS = [1, 2, 3, 4, 5, 6, 7, 9, 16, 17, 31, 64, 97, 100, 127, 256, 1000, 1009, 4096, 4097]
+# even lengths take the fused kernel: a 7-smooth half (inlined radix-2..8 stages) or one with 11 or 13
+# (per-radix stage calls: 22, 52, 286)
+LENGTHS = [1, 2, 3, 4, 5, 6, 7, 9, 16, 17, 22, 31, 52, 64, 97, 100, 127, 256, 286, 1000, 1009, 4096, 4097]
 
 
 @pytest.mark.parametrize("r2h,fw", FLAGS)

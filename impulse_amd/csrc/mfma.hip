@@ -29,8 +29,10 @@
 //          stage); the weighted sums are combined in FP64.  complex64 in,
 //          complex128 out.
 //
-// Lengths: 256 (one pass) and 65536 = 256 x 256 (four-step: column FFTs
-// with the W65536^(t k) twiddle, then row FFTs with a transposed store).  A
+// Lengths: 256 (one pass) and 256 n2 for n2 = 2 .. 8192 (four-step: 256-point
+// column FFTs with the W_L^(t k) twiddle, then the n2-point row FFTs with a
+// transposed store -- on the matrix cores for n2 = 256, by the engine's FP64
+// kernels otherwise).  A
 // workgroup of four wavefronts owns a tile of 16 sub-transforms in LDS; each
 // wavefront computes four of them.
 #include <cmath>
@@ -72,10 +74,12 @@ struct MfmaParams {
     const void *src;
     void *dst;
     long long batch;                // transforms of length L
-    int L;                          // 256 or 65536
+    int L;                          // 256 n2
+    int n2, ln2;                    // columns of the column pass (L / 256) and log2 of it
     double sign;                    // -1 forward, +1 backward
     double fct;                     // applied by the last pass
-    const double2 *tw;              // 512 FP64 twiddles (forward sign): W_256^m, then W_65536^m, m < 256
+    const double2 *tw;              // 512 FP64 twiddles (forward sign): W_256^m, then W_L^m, m < 256
+    const double2 *tw2;             // W_n2^q, q < n2 (forward sign), for n2 != 256
 };
 
 __device__ __forceinline__ int lidx(int n) { return n + (n >> 4); }
@@ -279,20 +283,29 @@ k_mfma256(const MfmaParams P)
     static_assert(PER * (NT / 64) == TILE, "the tile splits evenly over the wavefronts");
     using E = typename Elem<VAR>::T;
     __shared__ E lds[TILE * kRow];
-    __shared__ double2 tws[512];   // W_256^m and W_65536^m (m < 256), forward sign
+    __shared__ double2 tws[512];   // W_256^m and W_L^m (m < 256), forward sign
     const int tid = threadIdx.x;
     // the tiles of one signal run on one XCD, so its 2 KB rows are fetched into one L2
     const long long tile = xcd_span_block(blockIdx.x, gridDim.x);
     const long long L = P.L;
-    long long base = 0;   // first element of the signal (COL/ROW) or of transform 0 of the tile (SINGLE)
+    long long base = 0;   // first element of the signal (ROW) or of transform 0 of the tile (SINGLE)
     int t0 = 0, ntr = TILE;
     if (PASS == MP_SINGLE) {
         base = tile * TILE * 256;
         ntr = (int)min((long long)TILE, P.batch - tile * TILE);
-    } else {
+    } else if (PASS == MP_ROW) {
         base = (tile / TPS) * L;
         t0 = (int)(tile % TPS) * TILE;
     }
+    // the column pass: the tile's transforms are the batch's columns gc = tile TILE + c (batch n2 of
+    // them; column t of signal s holds points s L + t + n2 n, n < 256), so a tile may span signals
+    const long long ncols = P.batch * (long long)P.n2;
+    auto col_at = [&](int c, long long &off, int &t) {   // first point of column c of the tile, its t
+        const long long gc = tile * TILE + c;
+        t = (int)(gc & (P.n2 - 1));
+        off = (gc >> P.ln2) * L + t;
+        return gc < ncols;
+    };
     for (int m = tid; m < (PASS == MP_COL ? 512 : 256); m += NT) tws[m] = P.tw[m];
     auto put = [&](int c, int n, float2 v) {
         if constexpr (VAR == MF_F32) lds[c * kRow + lidx(n)] = v;
@@ -300,11 +313,15 @@ k_mfma256(const MfmaParams P)
     };
     // ---- load (16-byte accesses: complex64 pairs, or one complex128) ----
     if (PASS == MP_COL) {
-        // row n of the tile: TILE consecutive complex64 at t0 + 256 n
-        const float4 *x = (const float4 *)((const float2 *)P.src + base + t0);
+        // row n of the tile: columns 2p, 2p + 1 are neighbours in one signal (n2 even), one
+        // 16-byte load of two complex64
+        const float2 *x = (const float2 *)P.src;
         for (int e = tid; e < TILE * 128; e += NT) {
             const int p = e % (TILE / 2), n = e / (TILE / 2);
-            const float4 v = x[128LL * n + p];
+            long long off;
+            int t;
+            float4 v = float4{0.f, 0.f, 0.f, 0.f};   // columns past the batch compute zeros
+            if (col_at(2 * p, off, t)) v = *(const float4 *)(x + off + (long long)P.n2 * n);
             put(2 * p, n, make_float2(v.x, v.y));
             put(2 * p + 1, n, make_float2(v.z, v.w));
         }
@@ -335,17 +352,20 @@ k_mfma256(const MfmaParams P)
     const int l = tid & 63, w = tid >> 6, g = l >> 4, j = l & 15;
     const double sign = P.sign;
     const bool last = PASS != MP_COL;
-    // W_256^(b k1) for this lane's four (b, k1) pairs; the four-step twiddle
-    // W_65536^(t k) = W_65536^(t k mod 256) W_256^(t k / 256), both factors from LDS, product in FP64
+    // W_256^(b k1) for this lane's four (b, k1) pairs
     double2 w1[4];
     if constexpr (VAR == MF_F32) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) w1[r] = tw_sign(tws[(4 * g + r) * j], sign);
     }
+    // W_L^(t k) = W_L^(m mod 256) W_n2^(m / 256), m = t k < L: the first factor from LDS, the second
+    // from LDS for n2 = 256 (W_256) and from the W_n2 table otherwise; product in FP64
     auto step_tw = [&](int t, int k) -> double2 {
-        const int m = t * k;   // < 65536
-        return tw_sign(cmul(tws[256 + (m & 255)], tws[m >> 8]), sign);
+        const int m = t * k;
+        const double2 hi = P.n2 == 256 ? tws[m >> 8] : P.tw2[m >> 8];
+        return tw_sign(cmul(tws[256 + (m & 255)], hi), sign);
     };
+    auto col_t = [&](int c) { return (int)((tile * TILE + c) & (P.n2 - 1)); };
     if constexpr (VAR == MF_F32) {
         const F32Consts C = f32_consts(sign);
         for (int u = 0; u < PER; ++u) {
@@ -361,7 +381,7 @@ k_mfma256(const MfmaParams P)
             for (int r = 0; r < 4; ++r) {
                 const int k = 4 * g + r + 16 * j;
                 float2 v = a[r];
-                if (PASS == MP_COL) v = f32_twiddle(v, step_tw(t0 + c, k));
+                if (PASS == MP_COL) v = f32_twiddle(v, step_tw(col_t(c), k));
                 if (last && P.fct != 1.0) v = make_float2((float)(v.x * P.fct), (float)(v.y * P.fct));
                 lds[c * kRow + lidx(k)] = v;
             }
@@ -382,7 +402,7 @@ k_mfma256(const MfmaParams P)
             for (int r = 0; r < 4; ++r) {
                 const int k = 4 * g + r + 16 * j;
                 double2 v = a[r];
-                if (PASS == MP_COL) v = cmul(v, step_tw(t0 + c, k));
+                if (PASS == MP_COL) v = cmul(v, step_tw(col_t(c), k));
                 if (last) v = cscale(v, P.fct);
                 lds[c * kRow + lidx(k)] = v;
             }
@@ -399,6 +419,17 @@ k_mfma256(const MfmaParams P)
                 const float2 a = lds[c * kRow + lidx(kk)], b = lds[c * kRow + lidx(kk + 1)];
                 y4[e] = make_float4(a.x, a.y, b.x, b.y);
             }
+        } else if (PASS == MP_COL) {
+            // element (c, k) of column t of signal s goes to s L + t + n2 k (pairs of neighbours)
+            float2 *yc = (float2 *)P.dst;
+            for (int e = tid; e < TILE * 128; e += NT) {
+                const int p = e % (TILE / 2), k = e / (TILE / 2);
+                long long off;
+                int t;
+                if (!col_at(2 * p, off, t)) continue;
+                const float2 a = lds[2 * p * kRow + lidx(k)], b = lds[(2 * p + 1) * kRow + lidx(k)];
+                *(float4 *)(yc + off + (long long)P.n2 * k) = make_float4(a.x, a.y, b.x, b.y);
+            }
         } else {
             // element (c, k) goes to t0 + c + 256 k: TILE consecutive complex64 per k
             float4 *y4 = (float4 *)(y + t0);
@@ -412,6 +443,14 @@ k_mfma256(const MfmaParams P)
         double2 *y = (double2 *)P.dst + base;
         if (PASS == MP_SINGLE) {
             for (int e = tid; e < ntr * 256; e += NT) y[e] = lds[(e >> 8) * kRow + lidx(e & 255)];
+        } else if (PASS == MP_COL) {
+            double2 *yc = (double2 *)P.dst;
+            for (int e = tid; e < TILE * 256; e += NT) {
+                const int c = e % TILE, k = e / TILE;
+                long long off;
+                int t;
+                if (col_at(c, off, t)) yc[off + (long long)P.n2 * k] = lds[c * kRow + lidx(k)];
+            }
         } else {
             for (int e = tid; e < TILE * 256; e += NT) {
                 const int c = e % TILE, k = e / TILE;
@@ -428,7 +467,9 @@ k_mfma256(const MfmaParams P)
 struct Mfma {
     int precision = 0;
     long long L = 0, batch = 0;
-    DevBuf tw, tmp;
+    int n2 = 1, ln2 = 0;                      // L = 256 n2
+    DevBuf tw, tw2, tmp;
+    std::unique_ptr<Plan> rows[2];            // n2 != 256: the n2-point row transforms (backward, forward)
 };
 
 static int upload(DevBuf &b, const void *p, size_t n)
@@ -441,12 +482,16 @@ static int upload(DevBuf &b, const void *p, size_t n)
     return OK;
 }
 
+// L = 256 n2 as a four-step transform: the 256-point column transforms (stride n2) on the
+// matrix cores with the twiddle W_L^(t k1), then the n2-point row transforms -- on the matrix
+// cores for n2 = 256, otherwise by the engine's FP64 kernels (float32 arrays for the F32
+// precision, float64 for BF16X) with the transposed store X[k1 + 256 k2].
 int mfma_create(Mfma **out, size_t length, size_t batch, int precision, Error &err)
 {
     if (!out) { err = {E_INVALID, "null handle pointer"}; return E_INVALID; }
     *out = nullptr;
-    if (length != 256 && length != 65536) {
-        err = {E_UNSUPPORTED, "matrix-core FFT lengths are 256 and 65536"};
+    if (length < 256 || length > ((size_t)1 << 21) || (length & (length - 1))) {
+        err = {E_UNSUPPORTED, "matrix-core FFT lengths are the powers of two from 256 to 2^21"};
         return E_UNSUPPORTED;
     }
     if (batch == 0 || batch > (size_t)1 << 40) { err = {E_INVALID, "batch out of range"}; return E_INVALID; }
@@ -457,19 +502,35 @@ int mfma_create(Mfma **out, size_t length, size_t batch, int precision, Error &e
     M->precision = precision;
     M->L = (long long)length;
     M->batch = (long long)batch;
-    // W_256^m then W_65536^m for m < 256, from the long-double host generator
-    std::vector<double2> t256, t64k;
+    M->n2 = (int)(length / 256);
+    while ((1 << M->ln2) < M->n2) ++M->ln2;
+    // W_256^m then W_L^m for m < 256, and W_n2^q for q < n2, from the long-double host generator
+    std::vector<double2> t256, tL;
     twiddle_table(t256, 256, 256, 1);
-    twiddle_table(t64k, 65536, 256, 1);
-    t256.insert(t256.end(), t64k.begin(), t64k.end());
+    twiddle_table(tL, (uint64_t)length, 256, 1);
+    t256.insert(t256.end(), tL.begin(), tL.end());
     rc = upload(M->tw, t256.data(), t256.size() * sizeof(double2));
-    if (rc == OK && length == 65536) {
-        const size_t es = precision == MF_F32 ? sizeof(float2) : sizeof(double2);
-        if (M->tmp.alloc((size_t)length * batch * es)) rc = E_NOMEM;
+    if (rc == OK && M->n2 > 1 && M->n2 != 256) {
+        std::vector<double2> t2;
+        twiddle_table(t2, (uint64_t)M->n2, (uint64_t)M->n2, 1);
+        rc = upload(M->tw2, t2.data(), t2.size() * sizeof(double2));
     }
+    const size_t es = precision == MF_F32 ? sizeof(float2) : sizeof(double2);
+    if (rc == OK && length > 256 && M->tmp.alloc((size_t)length * batch * es)) rc = E_NOMEM;
     if (rc != OK) {
         err = {rc, "matrix-core FFT table or workspace allocation failed"};
         return rc;
+    }
+    if (M->n2 > 1 && M->n2 != 256) {
+        // [batch, 256 rows k1, n2 points t] -> X[k1 + 256 k2], byte strides
+        const size_t shape[3] = {batch, 256, (size_t)M->n2};
+        const ptrdiff_t e = (ptrdiff_t)es, Lb = (ptrdiff_t)length * e;
+        const ptrdiff_t sin[3] = {Lb, (ptrdiff_t)M->n2 * e, e}, sout[3] = {Lb, e, 256 * e};
+        const size_t axes[1] = {2};
+        for (int fw = 0; fw < 2; ++fw) {
+            rc = plan_create(M->rows[fw], 0, precision == MF_F32 ? 1 : 0, 3, shape, sin, sout, 1, axes, fw, err);
+            if (rc != OK) return rc;
+        }
     }
     *out = M.release();
     return OK;
@@ -483,7 +544,9 @@ template <int VAR>
 static hipError_t launch_var(const MfmaParams &P, int pass, long long batch, hipStream_t s)
 {
     constexpr int TILE = tile_of<VAR>();
-    const long long grid = pass == MP_SINGLE ? (batch + TILE - 1) / TILE : batch * (256 / TILE);
+    const long long grid = pass == MP_SINGLE ? (batch + TILE - 1) / TILE
+                         : pass == MP_COL   ? (batch * (long long)P.n2 + TILE - 1) / TILE
+                                            : batch * (256 / TILE);
     if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
     const dim3 g((unsigned)grid), b(threads_of<VAR>());
     if (pass == MP_COL) hipLaunchKernelGGL((k_mfma256<VAR, MP_COL>), g, b, 0, s, P);
@@ -502,9 +565,12 @@ int mfma_execute(Mfma &m, const void *in, void *out, int forward, double fct, hi
     MfmaParams P{};
     P.batch = m.batch;
     P.L = (int)m.L;
+    P.n2 = m.n2;
+    P.ln2 = m.ln2;
     P.sign = forward ? -1.0 : 1.0;
     P.fct = fct;
     P.tw = (const double2 *)m.tw.p;
+    P.tw2 = (const double2 *)m.tw2.p;
     auto go = [&](int pass, const void *src, void *dst) {
         P.src = src;
         P.dst = dst;
@@ -516,12 +582,13 @@ int mfma_execute(Mfma &m, const void *in, void *out, int forward, double fct, hi
         e = go(MP_SINGLE, in, out);
     } else {
         e = go(MP_COL, in, m.tmp.p);
-        if (e == hipSuccess) e = go(MP_ROW, m.tmp.p, out);
+        if (e == hipSuccess && m.n2 == 256) e = go(MP_ROW, m.tmp.p, out);
     }
     if (e != hipSuccess) {
         err = {E_HIP, std::string("matrix-core FFT launch: ") + hipGetErrorString(e)};
         return E_HIP;
     }
+    if (m.L > 256 && m.n2 != 256) return plan_execute(*m.rows[forward ? 1 : 0], m.tmp.p, out, fct, s, err);
     return OK;
 }
 

@@ -1,7 +1,8 @@
 """Matrix-core complex FFTs over the C ABI (impulse_fft_mfma_*).
 
-The radix-16 DFT stages of 256- and 65536-point complex64 transforms run as
-complex 16x16x16 matrix products on the MFMA units, in one of two precisions:
+The radix-16 DFT stages of the 256-point factors of complex64 transforms
+(lengths: the powers of two from 256 to 2^21) run as complex 16x16x16 matrix
+products on the MFMA units, in one of two precisions:
 
 * ``"f32"``   -- FP32 MFMA with FP32 accumulation; complex64 output.
 * ``"bf16x"`` -- bf16 slices accumulated exactly by the MFMA units and
@@ -19,7 +20,7 @@ PRECISIONS = {"f32": _lib.MFMA_F32, "bf16x": _lib.MFMA_BF16X}
 
 
 class MfmaFFT:
-    """`batch` contiguous complex64 transforms of `length` (256 or 65536)."""
+    """`batch` contiguous complex64 transforms of `length` (a power of two, 256 .. 2^21)."""
 
     def __init__(self, length, batch, precision="bf16x"):
         self.length, self.batch, self.precision = int(length), int(batch), precision

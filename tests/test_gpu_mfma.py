@@ -1,4 +1,5 @@
-"""Matrix-core complex FFTs (impulse_fft_mfma_*) against the FP64 oracle.
+"""Matrix-core complex FFTs (impulse_fft_mfma_*) against the FP64 oracle, over
+the powers of two from 256 to 2^20.
 
 Input is complex64; the reference is the oracle's c2c of the same values
 widened to complex128.  Bars (relative L2 per transform), stated here:
@@ -36,8 +37,12 @@ def _rel_rows(a, b):
     return np.linalg.norm(a - b, axis=-1) / np.linalg.norm(b, axis=-1)
 
 
+# 256: one pass; 65536: both four-step passes on the matrix cores; the other powers of two: the
+# matrix-core column pass (tiles spanning several signals when length/256 < the tile) and FP64
+# row transforms; odd batches leave a partial last tile
 @pytest.mark.parametrize("prec", ["bf16x", "f32"])
-@pytest.mark.parametrize("length,batch", [(256, 37), (256, 16), (65536, 3)])
+@pytest.mark.parametrize("length,batch", [(256, 37), (256, 16), (65536, 3), (512, 7), (2048, 5), (8192, 3),
+                                          (16384, 2), (131072, 2), (1 << 20, 1)])
 @pytest.mark.parametrize("forward", [True, False])
 def test_against_oracle(prec, length, batch, forward):
     print(f"case mfma {prec} L={length} batch={batch} forward={forward}", flush=True)
@@ -77,8 +82,9 @@ def test_zero_and_tiny_scale():
 
 
 def test_rejects_bad_arguments():
-    with pytest.raises(_lib.FFTError):
-        MfmaFFT(512, 4, "f32")
+    for bad in (128, 768, 1 << 22):   # below 256, not a power of two, above 2^21
+        with pytest.raises(_lib.FFTError):
+            MfmaFFT(bad, 4, "f32")
     with pytest.raises(_lib.FFTError):
         MfmaFFT(256, 0, "f32")
     p = MfmaFFT(256, 2, "f32")

@@ -36,6 +36,7 @@
 // workgroup of four wavefronts owns a tile of 16 sub-transforms in LDS; each
 // wavefront computes four of them.
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "butterflies.h"
@@ -49,7 +50,10 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef unsigned u4 __attribute__((ext_vector_type(4)));
 
 enum { MF_F32 = 0, MF_BF16X = 1 };
-enum { MP_COL = 0, MP_ROW = 1, MP_SINGLE = 2 };
+// MP_FUSED: the column pass of L = 256 n2 for n2 <= 16, with the n2-point row DFTs of the
+// tile's whole signals done in LDS before the store (one pass, no workspace)
+enum { MP_COL = 0, MP_ROW = 1, MP_SINGLE = 2, MP_FUSED = 3 };
+constexpr int kFusedMaxN2 = 16;
 constexpr int kSlices = 5;
 constexpr int kRow = 273;           // LDS row stride (elements): 256 + 256/16 padding + 1
 // sub-transforms per workgroup (four per wavefront): 32 for FP32 data (256-byte
@@ -274,6 +278,33 @@ __device__ __forceinline__ void bx_stage(double2 (&a)[4], const BxConsts &C)
 // the kernel
 // ---------------------------------------------------------------------------
 
+__device__ __forceinline__ double2 to_d2(float2 v) { return make_double2((double)v.x, (double)v.y); }
+__device__ __forceinline__ double2 to_d2(double2 v) { return v; }
+__device__ __forceinline__ void put_out(float2 *y, long long i, double2 v) { y[i] = make_float2((float)v.x, (float)v.y); }
+__device__ __forceinline__ void put_out(double2 *y, long long i, double2 v) { y[i] = v; }
+
+// The n2-point row DFTs of a fused tile: its columns are the n2 columns t of ns whole signals
+// (column g n2 + t of the tile holds the twiddled 256-point transform of column t of signal
+// s0 + g).  X[k1 + 256 k2] = sum_t T[t][k1] W_n2^(t k2) in FP64, scaled by fct; one job per
+// (signal, k1), k1 fastest, so the stores run over 256 consecutive outputs.
+template <int N2, typename E, typename O>
+__device__ __forceinline__ void fused_row_dfts(const E *lds, O *y, long long s0, long long batch, long long L,
+                                               int ns, double sign, double fct, int tid, int NT)
+{
+    for (int q = tid; q < ns * 256; q += NT) {
+        const int g = q >> 8, k1 = q & 255;
+        const long long sg = s0 + g;
+        if (sg >= batch) break;   // q only grows, so every later job is past the batch too
+        double2 v[N2];
+#pragma unroll
+        for (int t = 0; t < N2; ++t) v[t] = to_d2(lds[(g * N2 + t) * kRow + lidx(k1)]);
+        dft<N2>(v, sign);
+        O *ys = y + sg * L + k1;
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) put_out(ys, 256LL * k2, cscale(v[k2], fct));
+    }
+}
+
 template <int VAR, int PASS>
 __global__ void __launch_bounds__(threads_of<VAR>()) __attribute__((amdgpu_waves_per_eu(wpe_of<VAR>())))
 k_mfma256(const MfmaParams P)
@@ -306,18 +337,23 @@ k_mfma256(const MfmaParams P)
         off = (gc >> P.ln2) * L + t;
         return gc < ncols;
     };
-    for (int m = tid; m < (PASS == MP_COL ? 512 : 256); m += NT) tws[m] = P.tw[m];
+    constexpr bool COLS = PASS == MP_COL || PASS == MP_FUSED;   // the column pass of L = 256 n2
+    for (int m = tid; m < (COLS ? 512 : 256); m += NT) tws[m] = P.tw[m];
     auto put = [&](int c, int n, float2 v) {
         if constexpr (VAR == MF_F32) lds[c * kRow + lidx(n)] = v;
         else lds[c * kRow + lidx(n)] = make_double2((double)v.x, (double)v.y);
     };
     // ---- load (16-byte accesses: complex64 pairs, or one complex128) ----
-    if (PASS == MP_COL) {
+    if (COLS) {
         // row n of the tile: columns 2p, 2p + 1 are neighbours in one signal (n2 even), one
-        // 16-byte load of two complex64
+        // 16-byte load of two complex64.  Consecutive e walk a signal's memory in order: the
+        // PW pairs of a row, then the rows, then (n2 < TILE) the tile's next signal.
         const float2 *x = (const float2 *)P.src;
+        constexpr int LT = TILE == 32 ? 5 : 4;
+        static_assert(1 << LT == TILE, "the tile is 16 or 32 columns");
+        const int lpw = min(P.ln2, LT) - 1;   // log2 of the PW = min(n2, TILE) / 2 pairs of a row
         for (int e = tid; e < TILE * 128; e += NT) {
-            const int p = e % (TILE / 2), n = e / (TILE / 2);
+            const int p = ((e >> (lpw + 8)) << lpw) + (e & ((1 << lpw) - 1)), n = (e >> lpw) & 255;
             long long off;
             int t;
             float4 v = float4{0.f, 0.f, 0.f, 0.f};   // columns past the batch compute zeros
@@ -351,7 +387,7 @@ k_mfma256(const MfmaParams P)
     // ---- compute: wave w owns transforms PER w .. PER w + PER - 1 ----
     const int l = tid & 63, w = tid >> 6, g = l >> 4, j = l & 15;
     const double sign = P.sign;
-    const bool last = PASS != MP_COL;
+    const bool last = PASS == MP_ROW || PASS == MP_SINGLE;
     // W_256^(b k1) for this lane's four (b, k1) pairs
     double2 w1[4];
     if constexpr (VAR == MF_F32) {
@@ -381,7 +417,7 @@ k_mfma256(const MfmaParams P)
             for (int r = 0; r < 4; ++r) {
                 const int k = 4 * g + r + 16 * j;
                 float2 v = a[r];
-                if (PASS == MP_COL) v = f32_twiddle(v, step_tw(col_t(c), k));
+                if (COLS) v = f32_twiddle(v, step_tw(col_t(c), k));
                 if (last && P.fct != 1.0) v = make_float2((float)(v.x * P.fct), (float)(v.y * P.fct));
                 lds[c * kRow + lidx(k)] = v;
             }
@@ -402,7 +438,7 @@ k_mfma256(const MfmaParams P)
             for (int r = 0; r < 4; ++r) {
                 const int k = 4 * g + r + 16 * j;
                 double2 v = a[r];
-                if (PASS == MP_COL) v = cmul(v, step_tw(col_t(c), k));
+                if (COLS) v = cmul(v, step_tw(col_t(c), k));
                 if (last) v = cscale(v, P.fct);
                 lds[c * kRow + lidx(k)] = v;
             }
@@ -410,7 +446,18 @@ k_mfma256(const MfmaParams P)
     }
     __syncthreads();
     // ---- store ----
-    if constexpr (VAR == MF_F32) {
+    if constexpr (PASS == MP_FUSED) {
+        using O = typename std::conditional<VAR == MF_F32, float2, double2>::type;
+        O *y = (O *)P.dst;
+        const int ns = TILE / P.n2;
+        const long long s0 = tile * ns;
+        switch (P.n2) {
+            case 2: fused_row_dfts<2>(lds, y, s0, P.batch, L, ns, sign, P.fct, tid, NT); break;
+            case 4: fused_row_dfts<4>(lds, y, s0, P.batch, L, ns, sign, P.fct, tid, NT); break;
+            case 8: fused_row_dfts<8>(lds, y, s0, P.batch, L, ns, sign, P.fct, tid, NT); break;
+            default: fused_row_dfts<16>(lds, y, s0, P.batch, L, ns, sign, P.fct, tid, NT); break;
+        }
+    } else if constexpr (VAR == MF_F32) {
         float2 *y = (float2 *)P.dst + base;
         if (PASS == MP_SINGLE) {
             float4 *y4 = (float4 *)y;
@@ -516,12 +563,13 @@ int mfma_create(Mfma **out, size_t length, size_t batch, int precision, Error &e
         rc = upload(M->tw2, t2.data(), t2.size() * sizeof(double2));
     }
     const size_t es = precision == MF_F32 ? sizeof(float2) : sizeof(double2);
-    if (rc == OK && length > 256 && M->tmp.alloc((size_t)length * batch * es)) rc = E_NOMEM;
+    const bool fused = M->n2 > 1 && M->n2 <= kFusedMaxN2;
+    if (rc == OK && length > 256 && !fused && M->tmp.alloc((size_t)length * batch * es)) rc = E_NOMEM;
     if (rc != OK) {
         err = {rc, "matrix-core FFT table or workspace allocation failed"};
         return rc;
     }
-    if (M->n2 > 1 && M->n2 != 256) {
+    if (M->n2 > kFusedMaxN2 && M->n2 != 256) {
         // [batch, 256 rows k1, n2 points t] -> X[k1 + 256 k2], byte strides
         const size_t shape[3] = {batch, 256, (size_t)M->n2};
         const ptrdiff_t e = (ptrdiff_t)es, Lb = (ptrdiff_t)length * e;
@@ -545,11 +593,12 @@ static hipError_t launch_var(const MfmaParams &P, int pass, long long batch, hip
 {
     constexpr int TILE = tile_of<VAR>();
     const long long grid = pass == MP_SINGLE ? (batch + TILE - 1) / TILE
-                         : pass == MP_COL   ? (batch * (long long)P.n2 + TILE - 1) / TILE
+                         : pass == MP_COL || pass == MP_FUSED ? (batch * (long long)P.n2 + TILE - 1) / TILE
                                             : batch * (256 / TILE);
     if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
     const dim3 g((unsigned)grid), b(threads_of<VAR>());
-    if (pass == MP_COL) hipLaunchKernelGGL((k_mfma256<VAR, MP_COL>), g, b, 0, s, P);
+    if (pass == MP_FUSED) hipLaunchKernelGGL((k_mfma256<VAR, MP_FUSED>), g, b, 0, s, P);
+    else if (pass == MP_COL) hipLaunchKernelGGL((k_mfma256<VAR, MP_COL>), g, b, 0, s, P);
     else if (pass == MP_ROW) hipLaunchKernelGGL((k_mfma256<VAR, MP_ROW>), g, b, 0, s, P);
     else hipLaunchKernelGGL((k_mfma256<VAR, MP_SINGLE>), g, b, 0, s, P);
     return hipGetLastError();
@@ -578,8 +627,11 @@ int mfma_execute(Mfma &m, const void *in, void *out, int forward, double fct, hi
                                      : launch_var<MF_BF16X>(P, pass, m.batch, s);
     };
     hipError_t e;
+    const bool fused = m.n2 > 1 && m.n2 <= kFusedMaxN2;
     if (m.L == 256) {
         e = go(MP_SINGLE, in, out);
+    } else if (fused) {
+        e = go(MP_FUSED, in, out);
     } else {
         e = go(MP_COL, in, m.tmp.p);
         if (e == hipSuccess && m.n2 == 256) e = go(MP_ROW, m.tmp.p, out);
@@ -588,7 +640,7 @@ int mfma_execute(Mfma &m, const void *in, void *out, int forward, double fct, hi
         err = {E_HIP, std::string("matrix-core FFT launch: ") + hipGetErrorString(e)};
         return E_HIP;
     }
-    if (m.L > 256 && m.n2 != 256) return plan_execute(*m.rows[forward ? 1 : 0], m.tmp.p, out, fct, s, err);
+    if (m.L > 256 && !fused && m.n2 != 256) return plan_execute(*m.rows[forward ? 1 : 0], m.tmp.p, out, fct, s, err);
     return OK;
 }
 

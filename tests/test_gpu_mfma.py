@@ -37,12 +37,12 @@ def _rel_rows(a, b):
     return np.linalg.norm(a - b, axis=-1) / np.linalg.norm(b, axis=-1)
 
 
-# 256: one pass; 65536: both four-step passes on the matrix cores; the other powers of two: the
-# matrix-core column pass (tiles spanning several signals when length/256 < the tile) and FP64
-# row transforms; odd batches leave a partial last tile
+# 256: one pass; 512 .. 4096: one pass with the length/256-point row DFTs in LDS (tiles of
+# several whole signals); 65536: both four-step passes on the matrix cores; the other powers of
+# two: the matrix-core column pass and FP64 row transforms; odd batches leave a partial last tile
 @pytest.mark.parametrize("prec", ["bf16x", "f32"])
-@pytest.mark.parametrize("length,batch", [(256, 37), (256, 16), (65536, 3), (512, 7), (2048, 5), (8192, 3),
-                                          (16384, 2), (131072, 2), (1 << 20, 1)])
+@pytest.mark.parametrize("length,batch", [(256, 37), (256, 16), (65536, 3), (512, 7), (1024, 9), (2048, 5),
+                                          (4096, 5), (8192, 3), (16384, 2), (131072, 2), (1 << 20, 1)])
 @pytest.mark.parametrize("forward", [True, False])
 def test_against_oracle(prec, length, batch, forward):
     print(f"case mfma {prec} L={length} batch={batch} forward={forward}", flush=True)

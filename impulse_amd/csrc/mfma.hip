@@ -31,10 +31,11 @@
 //
 // Lengths: 256 (one pass) and 256 n2 for n2 = 2 .. 8192 (four-step: 256-point
 // column FFTs with the W_L^(t k) twiddle, then the n2-point row FFTs with a
-// transposed store -- on the matrix cores for n2 = 256, by the engine's FP64
-// kernels otherwise).  A
+// transposed store -- in the same launch for n2 <= 16, on the matrix cores for
+// n2 = 256, by an FP64 LDS row kernel otherwise).  A
 // workgroup of four wavefronts owns a tile of 16 sub-transforms in LDS; each
 // wavefront computes four of them.
+#include <algorithm>
 #include <cmath>
 #include <type_traits>
 #include <vector>
@@ -42,6 +43,7 @@
 #include "butterflies.h"
 #include "engine.h"
 #include "fft_common.h"
+#include "lds_fft.h"
 
 namespace ifft {
 
@@ -508,6 +510,38 @@ k_mfma256(const MfmaParams P)
 }
 
 // ---------------------------------------------------------------------------
+// the row pass of L = 256 n2 for n2 = 32 .. 8192, n2 != 256 (FP64 LDS stages)
+// ---------------------------------------------------------------------------
+
+// The column pass leaves row k1 of signal s as n2 contiguous points t at s L + n2 k1.  A
+// workgroup takes R = NT kPointsPerThread / n2 consecutive rows of one signal -- one contiguous
+// run of R n2 points -- transforms them in LDS (radix-8 powers of two, W_n2 from the table)
+// and stores X[k1 + 256 k2] k1 fastest, in runs of R consecutive outputs.
+// Four waves per SIMD (128 registers, 13 of them spilled) measured faster than the
+// compiler's 142 registers at three, and 512 threads faster than 256 (longer store runs).
+template <int VAR, int NT>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) k_mfma_rows(const MfmaParams P)
+{
+    constexpr int PTS = NT * kPointsPerThread;
+    __shared__ double2 lds[lds_points_padded(PTS)];
+    using I = typename Elem<VAR>::T;   // the workspace: complex64 (F32) or complex128 (BF16X)
+    using O = typename std::conditional<VAR == MF_F32, float2, double2>::type;
+    const int n2 = P.n2, lr = __builtin_ctz((unsigned)PTS) - P.ln2, R = 1 << lr;
+    const long long blk = xcd_span_block(blockIdx.x, gridDim.x);
+    const long long sg = blk >> (8 - lr);   // 256 / R workgroups per signal
+    const int r0 = (int)(blk & ((256 >> lr) - 1)) << lr;
+    const I *x = (const I *)P.src + sg * (long long)P.L + (long long)r0 * n2;
+    for (int e = threadIdx.x; e < PTS; e += NT) lds[lpad(e)] = to_d2(x[e]);
+    __syncthreads();
+    lds_fft_inline<kPointsPerThread, true>(lds, n2, R, P.sign, 2, TwTable{P.tw2, n2});
+    O *y = (O *)P.dst + sg * (long long)P.L + r0;
+    for (int e = threadIdx.x; e < PTS; e += NT) {
+        const int r = e & (R - 1), k2 = e >> lr;
+        put_out(y, r + 256LL * k2, cscale(lds[lpad(r * n2 + k2)], P.fct));
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 
@@ -516,7 +550,6 @@ struct Mfma {
     long long L = 0, batch = 0;
     int n2 = 1, ln2 = 0;                      // L = 256 n2
     DevBuf tw, tw2, tmp;
-    std::unique_ptr<Plan> rows[2];            // n2 != 256: the n2-point row transforms (backward, forward)
 };
 
 static int upload(DevBuf &b, const void *p, size_t n)
@@ -530,9 +563,9 @@ static int upload(DevBuf &b, const void *p, size_t n)
 }
 
 // L = 256 n2 as a four-step transform: the 256-point column transforms (stride n2) on the
-// matrix cores with the twiddle W_L^(t k1), then the n2-point row transforms -- on the matrix
-// cores for n2 = 256, otherwise by the engine's FP64 kernels (float32 arrays for the F32
-// precision, float64 for BF16X) with the transposed store X[k1 + 256 k2].
+// matrix cores with the twiddle W_L^(t k1), then the n2-point row transforms with the
+// transposed store X[k1 + 256 k2] -- in the same launch for n2 <= 16, on the matrix cores for
+// n2 = 256, otherwise by the FP64 LDS row kernel.
 int mfma_create(Mfma **out, size_t length, size_t batch, int precision, Error &err)
 {
     if (!out) { err = {E_INVALID, "null handle pointer"}; return E_INVALID; }
@@ -568,17 +601,6 @@ int mfma_create(Mfma **out, size_t length, size_t batch, int precision, Error &e
     if (rc != OK) {
         err = {rc, "matrix-core FFT table or workspace allocation failed"};
         return rc;
-    }
-    if (M->n2 > kFusedMaxN2 && M->n2 != 256) {
-        // [batch, 256 rows k1, n2 points t] -> X[k1 + 256 k2], byte strides
-        const size_t shape[3] = {batch, 256, (size_t)M->n2};
-        const ptrdiff_t e = (ptrdiff_t)es, Lb = (ptrdiff_t)length * e;
-        const ptrdiff_t sin[3] = {Lb, (ptrdiff_t)M->n2 * e, e}, sout[3] = {Lb, e, 256 * e};
-        const size_t axes[1] = {2};
-        for (int fw = 0; fw < 2; ++fw) {
-            rc = plan_create(M->rows[fw], 0, precision == MF_F32 ? 1 : 0, 3, shape, sin, sout, 1, axes, fw, err);
-            if (rc != OK) return rc;
-        }
     }
     *out = M.release();
     return OK;
@@ -634,13 +656,33 @@ int mfma_execute(Mfma &m, const void *in, void *out, int forward, double fct, hi
         e = go(MP_FUSED, in, out);
     } else {
         e = go(MP_COL, in, m.tmp.p);
-        if (e == hipSuccess && m.n2 == 256) e = go(MP_ROW, m.tmp.p, out);
+        if (e == hipSuccess && m.n2 == 256) {
+            e = go(MP_ROW, m.tmp.p, out);
+        } else if (e == hipSuccess) {
+            // R n2 = 4096 points per workgroup (8192 for n2 = 8192); 256 / R workgroups a signal
+            P.src = m.tmp.p;
+            P.dst = out;
+            const int NT = std::max(512, m.n2 / kPointsPerThread), R = NT * kPointsPerThread / m.n2;
+            const long long grid = m.batch * (256 / R);
+            if (grid > 0x7fffffffLL) {
+                e = hipErrorInvalidValue;
+            } else {
+                const dim3 g((unsigned)grid);
+                if (m.precision == MF_F32) {
+                    if (NT == 1024) hipLaunchKernelGGL((k_mfma_rows<MF_F32, 1024>), g, dim3(1024), 0, s, P);
+                    else hipLaunchKernelGGL((k_mfma_rows<MF_F32, 512>), g, dim3(512), 0, s, P);
+                } else {
+                    if (NT == 1024) hipLaunchKernelGGL((k_mfma_rows<MF_BF16X, 1024>), g, dim3(1024), 0, s, P);
+                    else hipLaunchKernelGGL((k_mfma_rows<MF_BF16X, 512>), g, dim3(512), 0, s, P);
+                }
+                e = hipGetLastError();
+            }
+        }
     }
     if (e != hipSuccess) {
         err = {E_HIP, std::string("matrix-core FFT launch: ") + hipGetErrorString(e)};
         return E_HIP;
     }
-    if (m.L > 256 && !fused && m.n2 != 256) return plan_execute(*m.rows[forward ? 1 : 0], m.tmp.p, out, fct, s, err);
     return OK;
 }
 

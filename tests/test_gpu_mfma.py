@@ -92,6 +92,15 @@ def test_rejects_bad_arguments():
         p(torch.zeros(256, dtype=torch.complex64, device="cuda"))
 
 
+def test_workspace_only_for_two_pass_lengths():
+    # 256 and the one-pass lengths (512 .. 4096) hold no intermediate; from 8192 on, one
+    # intermediate of the output type
+    for L in (256, 512, 4096):
+        assert MfmaFFT(L, 3, "f32").workspace_bytes == 0
+    assert MfmaFFT(8192, 3, "f32").workspace_bytes == 8192 * 3 * 8
+    assert MfmaFFT(8192, 3, "bf16x").workspace_bytes == 8192 * 3 * 16
+
+
 @pytest.mark.parametrize("prec", ["bf16x", "f32"])
 def test_full_size_8192x65536(prec):
     print(f"case mfma full size {prec}", flush=True)

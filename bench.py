@@ -309,7 +309,8 @@ def ragged_form(torch, reps, loop_sample=600):
     """Ragged batch of 20,000 real float32 signals (lengths log-uniform in
     [64, 262144], 70 % 2^a3^b5^c7^d, 30 % with a prime factor > 1000, values
     uniform[-1, 1]) in one hipGraph launch, autotuned, against a loop of FP64
-    per-signal r2c plans timed on a random sample of the same signals."""
+    per-signal r2c plans timed on a random sample of the same signals; then the
+    same batch with the radix-16 stages on the matrix cores (exact bf16 slices)."""
     from impulse_amd import _lib, ragged, synth
     from impulse_amd import fft as F
 
@@ -359,6 +360,31 @@ def ragged_form(torch, reps, loop_sample=600):
         d = torch.linalg.vector_norm(y[o:o + n // 2 + 1] - ys) / torch.linalg.vector_norm(ys)
         worst = max(worst, float(d))
     gf = flops / (ms * 1e-3) / 1e9
+
+    # the matrix-core precision (radix-16 stages as exact bf16-slice MFMA products), tuned on its
+    # own; its bins against the same FP64 per-signal plans
+    b.set_precision("bf16x")
+    tune_mf = b.tune(x, y, reps=2)
+    info_mf = b.info()
+    for _ in range(2):
+        b.execute(x, y)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        b.execute(x, y)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_mf = e0.elapsed_time(e1) / reps
+    worst_mf = 0.0
+    for pl, xs, ys, i in jobs:
+        o, n = int(b.out_offsets[i]), int(lengths[i])
+        d = torch.linalg.vector_norm(y[o:o + n // 2 + 1] - ys) / torch.linalg.vector_norm(ys)
+        worst_mf = max(worst_mf, float(d))
+    b.set_precision("fp64")
+    mf = {"ms": round(ms_mf, 3), "gflops": round(flops / (ms_mf * 1e-3) / 1e9, 1),
+          "vs_fp64_ragged": round(ms / ms_mf, 3), "max_rel_l2_vs_fp64_plan": worst_mf,
+          "tuned": {k: info_mf[k] for k in ("threads", "lds_points", "radix_order", "waves_per_simd")},
+          "tune_best_ms": round(tune_mf, 3)}
     return {
         "form": "ragged", "signals": len(lengths), "samples": total, "gflop": round(flops / 1e9, 3),
         "ms": round(ms, 3), "gflops": round(gf, 1), "compulsory_GBs": round(comp / (ms * 1e-3) / 1e9, 1),
@@ -371,6 +397,7 @@ def ragged_form(torch, reps, loop_sample=600):
                            "speedup": round(gf / loop_gflops, 1)},
         "max_rel_l2_vs_fp64_plan": worst,
         **ragged_traffic(comp, ms),
+        "mfma_bf16x": mf,
     }
 
 
@@ -741,6 +768,11 @@ def main():
     forms = [] if args.no_forms else [ragged_form(torch, args.form_reps), mfma_form(torch, args.form_reps),
                                        conv_form(torch, args.form_reps), dct_form(torch, args.form_reps),
                                        upfirdn_form(torch, args.form_reps), fftpack_form(torch, args.form_reps)]
+    for f in forms:
+        if f.get("form") == "ragged":
+            # the north-star comparison: ragged-batch throughput against the FP64 drop-in's
+            f["vs_fp64_dropin_gflops"] = round(f["gflops"] / value, 3)
+            f["mfma_bf16x"]["vs_fp64_dropin_gflops"] = round(f["mfma_bf16x"]["gflops"] / value, 3)
     vendor = None if args.no_vendor else vendor_fft(torch, plist, bufs, ms_per_step, args.steps)
     cpu = None if args.no_cpu else cpu_baseline(plist, args.cpu_seconds)
     out = {

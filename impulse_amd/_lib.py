@@ -29,6 +29,7 @@ EXPORTS = (
     "impulse_fft_conv2d_grid", "impulse_fft_conv2d_destroy",
     "impulse_fft_ragged_create", "impulse_fft_ragged_execute", "impulse_fft_ragged_tune",
     "impulse_fft_ragged_configure", "impulse_fft_ragged_info", "impulse_fft_ragged_destroy",
+    "impulse_fft_ragged_set_precision",
     "impulse_fft_mfma_create", "impulse_fft_mfma_execute", "impulse_fft_mfma_workspace_bytes",
     "impulse_fft_mfma_destroy",
     "impulse_fft_dct", "impulse_fft_dst", "impulse_fft_r2r_plan_create",
@@ -86,6 +87,7 @@ def _sig(L):
     L.impulse_fft_ragged_execute.argtypes = [_vp, _vp, _vp, d, _vp]
     L.impulse_fft_ragged_tune.argtypes = [_vp, _vp, _vp, _vp, i, ctypes.POINTER(d)]
     L.impulse_fft_ragged_configure.argtypes = [_vp, i, i, i, i]
+    L.impulse_fft_ragged_set_precision.argtypes = [_vp, i]
     ip = ctypes.POINTER(i)
     L.impulse_fft_ragged_info.argtypes = [_vp, ip, ip, ip, ip, ip, ctypes.POINTER(d), _szp]
     L.impulse_fft_ragged_destroy.argtypes = [_vp]

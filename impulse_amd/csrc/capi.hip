@@ -623,6 +623,14 @@ int impulse_fft_ragged_configure(impulse_fft_ragged batch, int threads, int lds_
     return rc == ifft::OK ? rc : fail(err);
 }
 
+int impulse_fft_ragged_set_precision(impulse_fft_ragged batch, int precision)
+{
+    if (!batch || !batch->r) return fail(ifft::E_INVALID, "batch is NULL");
+    ifft::Error err{0, ""};
+    int rc = ifft::ragged_set_precision(*batch->r, precision, err);
+    return rc == ifft::OK ? rc : fail(err);
+}
+
 int impulse_fft_ragged_info(impulse_fft_ragged batch, int *threads, int *lds_points, int *radix_order,
                             int *waves_per_simd, int *grid, double *flops, size_t *scratch_bytes)
 {

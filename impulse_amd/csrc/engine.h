@@ -171,6 +171,7 @@ int ragged_create(Ragged **out, size_t count, const size_t *lengths, const size_
 int ragged_execute(Ragged &R, const float *x, double2 *y, double fct, hipStream_t s, bool graph, Error &err);
 int ragged_tune(Ragged &R, const float *x, double2 *y, hipStream_t s, int reps, double *best_ms, Error &err);
 int ragged_configure(Ragged &R, int threads, int lds_max, int order, int waves, Error &err);
+int ragged_set_precision(Ragged &R, int precision, Error &err);
 void ragged_info(const Ragged &R, int *threads, int *lds_max, int *order, int *waves, int *grid, double *flops,
                  size_t *scratch_bytes);
 void ragged_destroy(Ragged *r);

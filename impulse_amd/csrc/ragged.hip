@@ -44,6 +44,7 @@
 #include "engine.h"
 #include "fft_common.h"
 #include "lds_fft.h"
+#include "mfma_stage.h"
 
 namespace ifft {
 
@@ -90,6 +91,7 @@ struct RaggedParams {
     double2 *scratch;          // spectrum mode: per workgroup, `slot` complex points
     long long slot;
     double2 *spec;             // table of chirp-kernel spectra (unnormalized, natural order)
+    const u4 *bx;              // non-null: radix-16 stages on the matrix cores (B operand table of both signs)
     int mode;                  // 0: transform the signals; 1: fill the spectrum table
     int order;                 // radix order of LDS stages (radix_codes): bit 0 odd radices first, bit 1 radix-8 powers of two
     double fct;                // output scale
@@ -158,6 +160,92 @@ __device__ __forceinline__ double2 chirp(long long j, long long M, double inv2m)
     return make_double2(c, -s);
 }
 
+// A radix-16 Stockham stage with the data movement of rstage<16>, whose 16-point DFTs run as
+// exact bf16-slice products on the matrix cores (bx_stage).  The NT threads of a transform are
+// whole wavefronts; wavefront w takes the butterfly groups [16 grp, 16 grp + 16) for grp = w and
+// w + NT / 64 (N <= 8 NT leaves at most two groups a wavefront).  Lane (g, j) loads inputs
+// q = 4 g + e of butterfly 16 grp + j times W_16p^(q k), k = b mod p (FP64), and after the
+// product stores output j of butterflies 16 grp + 4 g + r.
+__device__ __noinline__ void rstage_bx16(double2 *lds, int base, int N, int p, int i, int NT, double s,
+                                         const u4 *bx)
+{
+    const int nb = N / 16, NW = NT >> 6, w = i >> 6, g = (i >> 4) & 3, j = i & 15;
+    const int G = (nb + 15) >> 4;   // butterfly groups of the transform
+    double2 v[2][4];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int b = 16 * (w + NW * u) + j;
+        if (b < nb) {
+            const int k = b % p;
+            double2 t = make_double2(1.0, 0.0), w1 = make_double2(1.0, 0.0);
+            if (k) {
+                t = twid32(4 * g * k, 16 * p, s);
+                w1 = twid32(k, 16 * p, s);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                v[u][e] = cmul(lds[lpad(base + b + (4 * g + e) * nb)], t);
+                t = cmul(t, w1);
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[u][e] = make_double2(0.0, 0.0);
+        }
+    }
+    __syncthreads();
+    const BxConsts C = bx_consts_load(bx, s);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int grp = w + NW * u;
+        if (grp >= G) break;   // the same for the whole wavefront
+        bx_stage(v[u], C);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int b = 16 * grp + 4 * g + r;
+            if (b < nb) {
+                const int k = b % p;
+                lds[lpad(base + (b - k) * 16 + k + j * p)] = v[u][r];
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// lds_fft with the radix-16 stages on the matrix cores (the other radices in FP64 as there)
+__device__ __noinline__ void lds_fft_bx(double2 *lds, int L, int count, double s, int order, const u4 *bx)
+{
+    int ns;
+    unsigned long long codes = radix_codes(L, order, ns);
+    const int NT = blockDim.x / count;
+    const int t = threadIdx.x / NT, i = threadIdx.x % NT;
+    const int base = t * L;
+    const TwSincos tw{};
+    int p = 1;
+    for (int st = 0; st < ns; ++st, codes >>= 4) {
+        switch ((int)(codes & 15)) {
+            case 1: rstage<2, nb_rg(2)>(lds, base, L, p, i, NT, s, tw); p *= 2; break;
+            case 2: rstage<3, nb_rg(3)>(lds, base, L, p, i, NT, s, tw); p *= 3; break;
+            case 3: rstage<4, nb_rg(4)>(lds, base, L, p, i, NT, s, tw); p *= 4; break;
+            case 4: rstage<5, nb_rg(5)>(lds, base, L, p, i, NT, s, tw); p *= 5; break;
+            case 5: rstage<7, nb_rg(7)>(lds, base, L, p, i, NT, s, tw); p *= 7; break;
+            case 6: rstage<8, nb_rg(8)>(lds, base, L, p, i, NT, s, tw); p *= 8; break;
+            case 7: rstage<11, nb_rg(11)>(lds, base, L, p, i, NT, s, tw); p *= 11; break;
+            case 8: rstage<13, nb_rg(13)>(lds, base, L, p, i, NT, s, tw); p *= 13; break;
+            default: rstage_bx16(lds, base, L, p, i, NT, s, bx); p *= 16; break;
+        }
+    }
+    if (ns == 0) __syncthreads();
+}
+
+// The LDS transform of the ragged kernel: `count` transforms of length L (blockDim.x / count
+// threads each).  With the matrix-core precision (bx given) and transforms that span whole
+// wavefronts, the radix-16 stages run on the matrix cores; otherwise every stage is FP64.
+__device__ __forceinline__ void rg_fft(double2 *lds, int L, int count, double s, int order, const u4 *bx)
+{
+    if (bx && ((blockDim.x / count) & 63) == 0) lds_fft_bx(lds, L, count, s, order, bx);
+    else lds_fft(lds, L, count, s, order, TwSincos{});
+}
+
 // Hand global scratch to the other threads of the workgroup.  A barrier's
 // workgroup-scope release/acquire covers global memory too: the waves of a
 // workgroup share one CU and its write-through L1 (no threadgroup split).  A
@@ -172,7 +260,7 @@ __device__ __forceinline__ void wg_sync_global() { __syncthreads(); }
 // accesses are `width` points wide.
 template <class LOAD>
 __device__ void col_batch(LOAD load, long long L, int L1, int L2, int c0, int cnt, int width, double s, double2 *lds,
-                          double2 *out, int order)
+                          double2 *out, int order, const u4 *bx)
 {
     const int tid = threadIdx.x, nthr = blockDim.x;
     for (int e = tid; e < width * L2; e += nthr) {
@@ -180,7 +268,7 @@ __device__ void col_batch(LOAD load, long long L, int L1, int L2, int c0, int cn
         lds[lpad(t * L2 + n2)] = t < cnt ? load((long long)(c0 + t) + (long long)L1 * n2) : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    lds_fft(lds, L2, width, s, order, TwSincos{});
+    rg_fft(lds, L2, width, s, order, bx);
     {
         // t = tid % width is fixed per thread (width divides nthr); k2 advances by nthr / width
         const int t = tid % width, step = nthr / width;
@@ -197,13 +285,13 @@ __device__ void col_batch(LOAD load, long long L, int L1, int L2, int c0, int cn
 
 // Rows [r0, r0 + cnt) (cnt <= width) of `buf` (rows of L1 contiguous points)
 // transformed in place with sign s.
-__device__ void row_batch(int L1, int r0, int cnt, int width, double s, double2 *lds, double2 *buf, int order)
+__device__ void row_batch(int L1, int r0, int cnt, int width, double s, double2 *lds, double2 *buf, int order, const u4 *bx)
 {
     const int tid = threadIdx.x, nthr = blockDim.x;
     double2 *row = buf + (long long)r0 * L1;
     for (int e = tid; e < width * L1; e += nthr) lds[lpad(e)] = e < cnt * L1 ? row[e] : make_double2(0.0, 0.0);
     __syncthreads();
-    lds_fft(lds, L1, width, s, order, TwSincos{});
+    rg_fft(lds, L1, width, s, order, bx);
     for (int e = tid; e < cnt * L1; e += nthr) row[e] = lds[lpad(e)];
     __syncthreads();
 }
@@ -212,20 +300,20 @@ __device__ void row_batch(int L1, int r0, int cnt, int width, double s, double2 
 // in place: forward FFT, times the chirp kernel's spectrum (natural order: bin
 // k2 + L2 k1, read in runs of cnt bins), inverse FFT, times W_L^(+a k2).
 __device__ void brows_batch(long long L, int L1, int L2, int r0, int cnt, int width, double2 *lds, double2 *ws,
-                            const double2 *spec, int order)
+                            const double2 *spec, int order, const u4 *bx)
 {
     const int tid = threadIdx.x, nthr = blockDim.x;
     double2 *row = ws + (long long)r0 * L1;
     for (int e = tid; e < width * L1; e += nthr) lds[lpad(e)] = e < cnt * L1 ? row[e] : make_double2(0.0, 0.0);
     __syncthreads();
-    lds_fft(lds, L1, width, -1.0, order, TwSincos{});
+    rg_fft(lds, L1, width, -1.0, order, bx);
     for (int e = tid; e < cnt * L1; e += nthr) {
         const int t = e % cnt, k1 = e / cnt;
         const int q = lpad(t * L1 + k1);
         lds[q] = cmul(lds[q], spec[(long long)(r0 + t) + (long long)L2 * k1]);
     }
     __syncthreads();
-    lds_fft(lds, L1, width, 1.0, order, TwSincos{});
+    rg_fft(lds, L1, width, 1.0, order, bx);
     for (int e = tid; e < cnt * L1; e += nthr) {
         const int t = e / L1, a = e - t * L1;
         row[e] = cmul(lds[lpad(e)], twid((long long)a * (r0 + t), L, 1.0));
@@ -238,7 +326,7 @@ __device__ void brows_batch(long long L, int L1, int L2, int r0, int cnt, int wi
 // for the tile's own columns (so an in-place emit touches only its columns).
 template <class EMIT>
 __device__ void icols_batch(EMIT emit, int L1, int L2, int c0, int cnt, int width, double2 *lds, const double2 *ws,
-                            int order)
+                            int order, const u4 *bx)
 {
     const int tid = threadIdx.x, nthr = blockDim.x;
     for (int e = tid; e < width * L2; e += nthr) {
@@ -246,7 +334,7 @@ __device__ void icols_batch(EMIT emit, int L1, int L2, int c0, int cnt, int widt
         lds[lpad(t * L2 + k2)] = t < cnt ? ws[(long long)(c0 + t) + (long long)L1 * k2] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    lds_fft(lds, L2, width, 1.0, order, TwSincos{});
+    rg_fft(lds, L2, width, 1.0, order, bx);
     for (int e = tid; e < width * L2; e += nthr) {
         const int t = e % width, b = e / width;
         if (t < cnt) emit((long long)(c0 + t) + (long long)L1 * b, lds[lpad(t * L2 + b)]);
@@ -288,7 +376,7 @@ __device__ __forceinline__ void emit_bin(const RaggedTask &T, long long k, doubl
 // of the band's 2R rows is finished here; the stores run over R consecutive
 // bins (k2 fastest).
 __device__ void rows_split(const RaggedTask &T, double2 *lds, const double2 *ws, double2 *y, double2 *y2, double f,
-                           double fo, int order)
+                           double fo, int order, const u4 *bx)
 {
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int L1 = T.L1, L2 = T.L2, r0 = T.first, R = T.count, W = T.width;
@@ -302,7 +390,7 @@ __device__ void rows_split(const RaggedTask &T, double2 *lds, const double2 *ws,
         lds[lpad(e)] = v;
     }
     __syncthreads();
-    lds_fft(lds, L1, W, -1.0, order, TwSincos{});
+    rg_fft(lds, L1, W, -1.0, order, bx);
     for (int e = tid; e < 2 * R * L1; e += nthr) {
         const int t = e % (2 * R), k1 = e / (2 * R);
         const int row = t < R ? r0 + t : L2 - r0 - (t - R);
@@ -317,7 +405,7 @@ __device__ void rows_split(const RaggedTask &T, double2 *lds, const double2 *ws,
 // (L1 - k1) mod L1) in slot 0, and for even L2 row L2/2 (mirror point
 // L1 - 1 - k1) in slot 1; for even n also X[M] = Re Z[0] - Im Z[0].
 __device__ void self_split(const RaggedTask &T, double2 *lds, const double2 *ws, double2 *y, double2 *y2, double f,
-                           double fo, int order)
+                           double fo, int order, const u4 *bx)
 {
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int L1 = T.L1, L2 = T.L2;
@@ -327,7 +415,7 @@ __device__ void self_split(const RaggedTask &T, double2 *lds, const double2 *ws,
         lds[lpad(e)] = (t == 0 || mid) ? ws[(long long)(t ? L2 / 2 : 0) * L1 + k1] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    lds_fft(lds, L1, 2, -1.0, order, TwSincos{});
+    rg_fft(lds, L1, 2, -1.0, order, bx);
     for (int e = tid; e < 2 * L1; e += nthr) {
         const int t = e & 1, k1 = e >> 1;
         if (t == 1 && !mid) continue;
@@ -346,6 +434,9 @@ __device__ void self_split(const RaggedTask &T, double2 *lds, const double2 *ws,
 // ---------------------------------------------------------------------------
 // the kernels
 // ---------------------------------------------------------------------------
+
+// the B operand table of the matrix-core radix-16 stages (one wavefront)
+__global__ void k_bx_table(u4 *tab) { bx_consts_store(tab); }
 
 __global__ void k_ragged_reset(int *counter)
 {
@@ -378,9 +469,9 @@ __device__ void spectrum_task(const RaggedTask &T, const RaggedParams &P, double
     }
     // width = columns per batch, count = rows per batch
     for (int c0 = 0; c0 < T.L1; c0 += T.width)
-        col_batch(bload, n2, T.L1, T.L2, c0, min(T.width, T.L1 - c0), T.width, -1.0, lds, S0, P.order);
+        col_batch(bload, n2, T.L1, T.L2, c0, min(T.width, T.L1 - c0), T.width, -1.0, lds, S0, P.order, nullptr);
     for (int r0 = 0; r0 < T.L2; r0 += T.count)
-        row_batch(T.L1, r0, min(T.count, T.L2 - r0), T.count, -1.0, lds, S0, P.order);
+        row_batch(T.L1, r0, min(T.count, T.L2 - r0), T.count, -1.0, lds, S0, P.order, nullptr);
     // row layout (S0[k2 L1 + k1] = bin k2 + L2 k1) to natural order
     for (long long m = tid; m < n2; m += nthr) {
         const long long k2 = m / T.L1, k1 = m - k2 * T.L1;
@@ -398,11 +489,12 @@ __device__ void spectrum_task(const RaggedTask &T, const RaggedParams &P, double
 // time and their short strided runs merge in that XCD's L2.  A workgroup whose
 // queue is empty takes tasks from the next queues in turn; every workgroup
 // leaves once all eight are empty.
-template <int MAXT, int WPE>
+template <int MAXT, int WPE, bool MF>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ? WPE : 1))) k_ragged(const RaggedParams P)
 {
     extern __shared__ double2 lds[];
     __shared__ int s_task;
+    const u4 *bx = MF ? P.bx : nullptr;   // the FP64 instances carry no matrix-core stage
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int home = (int)(blockIdx.x & 7);
     int visited = 0;   // queues found empty so far (thread 0's count is the one that matters)
@@ -444,19 +536,19 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
         const double f = P.fct, fo = P.fct * P.osign;
         switch (T.kind) {
             case RG_COLS:
-                col_batch(zin, T.L, T.L1, T.L2, T.first, T.count, T.width, -1.0, lds, ws, P.order);
+                col_batch(zin, T.L, T.L1, T.L2, T.first, T.count, T.width, -1.0, lds, ws, P.order, bx);
                 break;
             case RG_BCOLS: {
                 auto aload = [&](long long m) -> double2 {
                     return m < M ? cmul(zin(m), chirp(m, M, inv2m)) : make_double2(0.0, 0.0);
                 };
-                col_batch(aload, T.L, T.L1, T.L2, T.first, T.count, T.width, -1.0, lds, ws, P.order);
+                col_batch(aload, T.L, T.L1, T.L2, T.first, T.count, T.width, -1.0, lds, ws, P.order, bx);
                 break;
             }
-            case RG_ROWS: rows_split(T, lds, ws, y, y2, f, fo, P.order); break;
-            case RG_SELF: self_split(T, lds, ws, y, y2, f, fo, P.order); break;
+            case RG_ROWS: rows_split(T, lds, ws, y, y2, f, fo, P.order, bx); break;
+            case RG_SELF: self_split(T, lds, ws, y, y2, f, fo, P.order, bx); break;
             case RG_BROWS:
-                brows_batch(T.L, T.L1, T.L2, T.first, T.count, T.width, lds, ws, P.spec + T.spec_off, P.order);
+                brows_batch(T.L, T.L1, T.L2, T.first, T.count, T.width, lds, ws, P.spec + T.spec_off, P.order, bx);
                 break;
             case RG_ICOLS:
                 if (half || pair) {
@@ -464,7 +556,7 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
                     auto keep = [&](long long k, double2 v) {
                         if (k < M) ws[k] = v;
                     };
-                    icols_batch(keep, T.L1, T.L2, T.first, T.count, T.width, lds, ws, P.order);
+                    icols_batch(keep, T.L1, T.L2, T.first, T.count, T.width, lds, ws, P.order, bx);
                 } else {
                     const long long K = n / 2 + 1;
                     const double sc = 1.0 / (double)T.L;
@@ -474,7 +566,7 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
                             y[k] = make_double2(f * z.x, fo * z.y);
                         }
                     };
-                    icols_batch(emit, T.L1, T.L2, T.first, T.count, T.width, lds, ws, P.order);
+                    icols_batch(emit, T.L1, T.L2, T.first, T.count, T.width, lds, ws, P.order, bx);
                 }
                 break;
             case RG_SPLIT: {
@@ -495,17 +587,17 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
                 if (!T.blue) {
                     for (long long m = tid; m < L; m += nthr) lds[lpad((int)m)] = zin(m);
                     __syncthreads();
-                    lds_fft(lds, (int)L, 1, -1.0, P.order, TwSincos{});
+                    rg_fft(lds, (int)L, 1, -1.0, P.order, bx);
                 } else {
                     // A = FFT(z w); C = A B; c = IFFT(C) (unnormalized; 1/n2 applied with the output chirp)
                     const double2 *B = P.spec + T.spec_off;
                     for (long long m = tid; m < L; m += nthr)
                         lds[lpad((int)m)] = m < M ? cmul(zin(m), chirp(m, M, inv2m)) : make_double2(0.0, 0.0);
                     __syncthreads();
-                    lds_fft(lds, (int)L, 1, -1.0, P.order, TwSincos{});
+                    rg_fft(lds, (int)L, 1, -1.0, P.order, bx);
                     for (long long m = tid; m < L; m += nthr) lds[lpad((int)m)] = cmul(lds[lpad((int)m)], B[m]);
                     __syncthreads();
-                    lds_fft(lds, (int)L, 1, 1.0, P.order, TwSincos{});
+                    rg_fft(lds, (int)L, 1, 1.0, P.order, bx);
                 }
                 const double bscale = T.blue ? 1.0 / (double)L : 1.0;
                 auto Zl = [&](long long k) -> double2 {
@@ -537,6 +629,8 @@ struct Ragged {
     int forward = 1;
     // configuration
     int threads = 0, lds_max = 0, rorder = 0, waves = 0;
+    int precision = 0;   // 0: FP64 stages; 1: radix-16 stages as exact bf16-slice MFMA products
+    DevBuf bx;           // their B operand table (precision 1)
     // derived for the configuration: the tasks of the phases back to back
     std::vector<RaggedTask> tasks;
     int phase_off[kPhases] = {0, 0, 0, 0}, phase_cnt[kPhases] = {0, 0, 0, 0}, phase_grid[kPhases] = {0, 0, 0, 0};
@@ -563,25 +657,39 @@ struct Ragged {
     ~Ragged() { drop_graph(); }
 };
 
-static const void *ragged_fn(int threads, int waves)
+template <bool MF>
+static const void *ragged_fn_t(int threads, int waves)
 {
     if (threads == 256)
-        return waves == 4 ? (const void *)k_ragged<256, 4>
-                          : waves == 3 ? (const void *)k_ragged<256, 3> : (const void *)k_ragged<256, 0>;
-    if (threads == 512) return waves == 4 ? (const void *)k_ragged<512, 4> : (const void *)k_ragged<512, 0>;
-    return (const void *)k_ragged<1024, 0>;
+        return waves == 4 ? (const void *)k_ragged<256, 4, MF>
+                          : waves == 3 ? (const void *)k_ragged<256, 3, MF> : (const void *)k_ragged<256, 0, MF>;
+    if (threads == 512) return waves == 4 ? (const void *)k_ragged<512, 4, MF> : (const void *)k_ragged<512, 0, MF>;
+    return (const void *)k_ragged<1024, 0, MF>;
 }
 
+static const void *ragged_fn(int threads, int waves, bool mf)
+{
+    return mf ? ragged_fn_t<true>(threads, waves) : ragged_fn_t<false>(threads, waves);
+}
+
+template <bool MF>
+static void launch_ragged_t(const RaggedParams &P, int threads, int waves, dim3 g, size_t lds, hipStream_t s)
+{
+    const dim3 b(threads);
+    if (threads == 256 && waves == 4) hipLaunchKernelGGL((k_ragged<256, 4, MF>), g, b, lds, s, P);
+    else if (threads == 256 && waves == 3) hipLaunchKernelGGL((k_ragged<256, 3, MF>), g, b, lds, s, P);
+    else if (threads == 256) hipLaunchKernelGGL((k_ragged<256, 0, MF>), g, b, lds, s, P);
+    else if (threads == 512 && waves == 4) hipLaunchKernelGGL((k_ragged<512, 4, MF>), g, b, lds, s, P);
+    else if (threads == 512) hipLaunchKernelGGL((k_ragged<512, 0, MF>), g, b, lds, s, P);
+    else hipLaunchKernelGGL((k_ragged<1024, 0, MF>), g, b, lds, s, P);
+}
+
+// the matrix-core instances when P.bx is set
 static hipError_t launch_ragged(const RaggedParams &P, int threads, int waves, unsigned grid, size_t lds,
                                 hipStream_t s)
 {
-    const dim3 g(grid), b(threads);
-    if (threads == 256 && waves == 4) hipLaunchKernelGGL((k_ragged<256, 4>), g, b, lds, s, P);
-    else if (threads == 256 && waves == 3) hipLaunchKernelGGL((k_ragged<256, 3>), g, b, lds, s, P);
-    else if (threads == 256) hipLaunchKernelGGL((k_ragged<256, 0>), g, b, lds, s, P);
-    else if (threads == 512 && waves == 4) hipLaunchKernelGGL((k_ragged<512, 4>), g, b, lds, s, P);
-    else if (threads == 512) hipLaunchKernelGGL((k_ragged<512, 0>), g, b, lds, s, P);
-    else hipLaunchKernelGGL((k_ragged<1024, 0>), g, b, lds, s, P);
+    if (P.bx) launch_ragged_t<true>(P, threads, waves, dim3(grid), lds, s);
+    else launch_ragged_t<false>(P, threads, waves, dim3(grid), lds, s);
     return hipGetLastError();
 }
 
@@ -730,7 +838,7 @@ static int resident_grid(const Ragged &R, int threads, int waves, size_t lds)
         (void)hipGetLastError();
         ncu = 256;
     }
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, ragged_fn(threads, waves), threads, lds) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, ragged_fn(threads, waves, false), threads, lds) != hipSuccess ||
         per <= 0) {
         (void)hipGetLastError();
         per = 1;
@@ -954,12 +1062,13 @@ int ragged_create(Ragged **out, size_t count, const size_t *lengths, const size_
     if (R->counter.alloc(64 * 8 * kPhases * sizeof(int))) { err = {E_NOMEM, "ragged counter allocation failed"}; return E_NOMEM; }
     const int inst[6][2] = {{256, 0}, {256, 3}, {256, 4}, {512, 0}, {512, 4}, {1024, 0}};
     for (const auto &v : inst)
-        if (hipFuncSetAttribute(ragged_fn(v[0], v[1]), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)(lds_points_padded(8192) * sizeof(double2))) != hipSuccess) {
-            (void)hipGetLastError();
-            err = {E_HIP, "cannot raise the LDS limit of the ragged kernel"};
-            return E_HIP;
-        }
+        for (int mf = 0; mf < 2; ++mf)
+            if (hipFuncSetAttribute(ragged_fn(v[0], v[1], mf != 0), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(lds_points_padded(8192) * sizeof(double2))) != hipSuccess) {
+                (void)hipGetLastError();
+                err = {E_HIP, "cannot raise the LDS limit of the ragged kernel"};
+                return E_HIP;
+            }
     rc = build_spectra(*R, err);
     if (rc != OK) return rc;
     rc = build_tasks(*R, 1024, 8192, 0, err);
@@ -990,6 +1099,28 @@ int ragged_configure(Ragged &R, int threads, int lds_max, int order, int waves, 
     return build_tasks(R, threads, lds_max, waves, err);
 }
 
+int ragged_set_precision(Ragged &R, int precision, Error &err)
+{
+    if (precision != 0 && precision != 1) {
+        err = {E_INVALID, "ragged precision is 0 (FP64) or 1 (radix-16 stages on the matrix cores)"};
+        return E_INVALID;
+    }
+    if (precision == 1 && !R.bx.p) {
+        if (R.bx.alloc((size_t)kBxTableWords * sizeof(u4))) { err = {E_NOMEM, "table allocation failed"}; return E_NOMEM; }
+        hipLaunchKernelGGL(k_bx_table, dim3(1), dim3(64), 0, nullptr, (u4 *)R.bx.p);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            err = {E_HIP, std::string("ragged table: ") + hipGetErrorString(e)};
+            return E_HIP;
+        }
+    }
+    R.precision = precision;
+    R.drop_graph();
+    return OK;
+}
+
 // the reset and one persistent launch per non-empty phase
 static hipError_t ragged_launch(Ragged &R, const float *x, double2 *y, double fct, hipStream_t s)
 {
@@ -1012,7 +1143,9 @@ static hipError_t ragged_launch(Ragged &R, const float *x, double2 *y, double fc
         P.ws = (double2 *)R.ws.p;
         P.spec = (double2 *)R.spec.p;
         P.mode = 0;
-        P.order = R.rorder;
+        // the matrix-core precision plans powers of two as radix-16 stages (order bit 1 clear)
+        P.order = R.precision ? (R.rorder & ~2) : R.rorder;
+        P.bx = R.precision ? (const u4 *)R.bx.p : nullptr;
         P.fct = fct;
         P.osign = R.forward ? 1.0 : -1.0;
         e = launch_ragged(P, R.threads, R.waves, (unsigned)R.phase_grid[q], lds, s);

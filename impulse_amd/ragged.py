@@ -49,6 +49,7 @@ class RaggedBatch:
                                               sizes(self.in_offsets), sizes(self.out_offsets), int(bool(forward))))
         self.handle = h
         self._destroy = lib().impulse_fft_ragged_destroy
+        self.precision = "fp64"
 
     def __del__(self):
         # the destroy entry point is bound at creation: at interpreter exit the
@@ -90,6 +91,17 @@ class RaggedBatch:
         or 3 with 256 threads."""
         check(lib().impulse_fft_ragged_configure(self.handle, int(threads), int(lds_points), int(radix_order),
                                                  int(waves_per_simd)))
+
+    PRECISIONS = {"fp64": 0, "bf16x": 1}
+
+    def set_precision(self, precision):
+        """Arithmetic of the LDS stages (impulse_fft_ragged_set_precision): "fp64" (every stage in
+        FP64, the default) or "bf16x" (radix-16 stages of transforms spanning whole wavefronts as
+        exact bf16-slice products on the matrix cores, combined in FP64; ~1e-12 relative)."""
+        if precision not in self.PRECISIONS:
+            raise ValueError(f"precision is one of {sorted(self.PRECISIONS)}")
+        check(lib().impulse_fft_ragged_set_precision(self.handle, self.PRECISIONS[precision]))
+        self.precision = precision
 
     def info(self):
         t, l, o, w, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -325,6 +325,21 @@ int impulse_fft_ragged_tune(impulse_fft_ragged batch, const float *in, void *out
  * threads, 3 (at most 168). */
 int impulse_fft_ragged_configure(impulse_fft_ragged batch, int threads, int lds_points, int radix_order,
                                  int waves_per_simd);
+/* Arithmetic of the LDS stages (no PocketFFT counterpart):
+ *   IMPULSE_FFT_RAGGED_FP64        every stage in FP64 (the default);
+ *   IMPULSE_FFT_RAGGED_MFMA_BF16X  the radix-16 stages of transforms that span
+ *       whole wavefronts (64 or more threads a transform) run as exact
+ *       bf16-slice products on the MFMA units -- the values of a stage cut into
+ *       five 8-bit slices, products accumulated exactly in FP32 and combined in
+ *       FP64 (error ~1e-12 relative) -- with powers of two planned as radix-16
+ *       stages whatever the radix order; twiddles, the Bluestein chirp and its
+ *       spectrum, and the other radices stay FP64.
+ * Applies from the next execute (the graph is re-captured). */
+typedef enum impulse_fft_ragged_precision {
+    IMPULSE_FFT_RAGGED_FP64 = 0,
+    IMPULSE_FFT_RAGGED_MFMA_BF16X = 1
+} impulse_fft_ragged_precision;
+int impulse_fft_ragged_set_precision(impulse_fft_ragged batch, int precision);
 /* Current configuration, persistent grid size, nominal flops of one execute
  * (2.5 n log2 n per signal) and device scratch bytes.  Any pointer may be NULL. */
 int impulse_fft_ragged_info(impulse_fft_ragged batch, int *threads, int *lds_points, int *radix_order,

@@ -160,12 +160,24 @@ __device__ __forceinline__ double2 chirp(long long j, long long M, double inv2m)
     return make_double2(c, -s);
 }
 
+// Stage functions per kernel instance.  A device function is compiled for the
+// register budget of every kernel that calls it, so stage functions shared by
+// all instances would get the 1,024-thread instance's 128 registers, and the
+// larger radices spill on every call.  The tag KI = RgInst<threads, waves,
+// precision> makes the FP64 ones distinct per instance (C4 at 256 threads,
+// three waves: 27.8 -> 25.9 ms).
+template <int MAXT, int WPE, bool MF>
+struct RgInst {};
+template <class KI>
+struct TwK : TwSincos {};
+
 // A radix-16 Stockham stage with the data movement of rstage<16>, whose 16-point DFTs run as
 // exact bf16-slice products on the matrix cores (bx_stage).  The NT threads of a transform are
 // whole wavefronts; wavefront w takes the butterfly groups [16 grp, 16 grp + 16) for grp = w and
 // w + NT / 64 (N <= 8 NT leaves at most two groups a wavefront).  Lane (g, j) loads inputs
 // q = 4 g + e of butterfly 16 grp + j times W_16p^(q k), k = b mod p (FP64), and after the
 // product stores output j of butterflies 16 grp + 4 g + r.
+template <class KI>
 __device__ __noinline__ void rstage_bx16(double2 *lds, int base, int N, int p, int i, int NT, double s,
                                          const u4 *bx)
 {
@@ -212,6 +224,7 @@ __device__ __noinline__ void rstage_bx16(double2 *lds, int base, int N, int p, i
 }
 
 // lds_fft with the radix-16 stages on the matrix cores (the other radices in FP64 as there)
+template <class KI>
 __device__ __noinline__ void lds_fft_bx(double2 *lds, int L, int count, double s, int order, const u4 *bx)
 {
     int ns;
@@ -219,7 +232,7 @@ __device__ __noinline__ void lds_fft_bx(double2 *lds, int L, int count, double s
     const int NT = blockDim.x / count;
     const int t = threadIdx.x / NT, i = threadIdx.x % NT;
     const int base = t * L;
-    const TwSincos tw{};
+    const TwK<KI> tw{};
     int p = 1;
     for (int st = 0; st < ns; ++st, codes >>= 4) {
         switch ((int)(codes & 15)) {
@@ -231,7 +244,7 @@ __device__ __noinline__ void lds_fft_bx(double2 *lds, int L, int count, double s
             case 6: rstage<8, nb_rg(8)>(lds, base, L, p, i, NT, s, tw); p *= 8; break;
             case 7: rstage<11, nb_rg(11)>(lds, base, L, p, i, NT, s, tw); p *= 11; break;
             case 8: rstage<13, nb_rg(13)>(lds, base, L, p, i, NT, s, tw); p *= 13; break;
-            default: rstage_bx16(lds, base, L, p, i, NT, s, bx); p *= 16; break;
+            default: rstage_bx16<KI>(lds, base, L, p, i, NT, s, bx); p *= 16; break;
         }
     }
     if (ns == 0) __syncthreads();
@@ -240,10 +253,11 @@ __device__ __noinline__ void lds_fft_bx(double2 *lds, int L, int count, double s
 // The LDS transform of the ragged kernel: `count` transforms of length L (blockDim.x / count
 // threads each).  With the matrix-core precision (bx given) and transforms that span whole
 // wavefronts, the radix-16 stages run on the matrix cores; otherwise every stage is FP64.
+template <class KI>
 __device__ __forceinline__ void rg_fft(double2 *lds, int L, int count, double s, int order, const u4 *bx)
 {
-    if (bx && ((blockDim.x / count) & 63) == 0) lds_fft_bx(lds, L, count, s, order, bx);
-    else lds_fft(lds, L, count, s, order, TwSincos{});
+    if (bx && ((blockDim.x / count) & 63) == 0) lds_fft_bx<KI>(lds, L, count, s, order, bx);
+    else lds_fft(lds, L, count, s, order, TwK<KI>{});
 }
 
 // Hand global scratch to the other threads of the workgroup.  A barrier's
@@ -258,7 +272,7 @@ __device__ __forceinline__ void wg_sync_global() { __syncthreads(); }
 // n1 + L1 n2 (load(m) gives point m).  Each column is transformed with sign s,
 // multiplied by W_L^(s n1 k2) and stored at out[n1 + L1 k2].  The strided
 // accesses are `width` points wide.
-template <class LOAD>
+template <class KI, class LOAD>
 __device__ void col_batch(LOAD load, long long L, int L1, int L2, int c0, int cnt, int width, double s, double2 *lds,
                           double2 *out, int order, const u4 *bx)
 {
@@ -268,7 +282,7 @@ __device__ void col_batch(LOAD load, long long L, int L1, int L2, int c0, int cn
         lds[lpad(t * L2 + n2)] = t < cnt ? load((long long)(c0 + t) + (long long)L1 * n2) : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    rg_fft(lds, L2, width, s, order, bx);
+    rg_fft<KI>(lds, L2, width, s, order, bx);
     {
         // t = tid % width is fixed per thread (width divides nthr); k2 advances by nthr / width
         const int t = tid % width, step = nthr / width;
@@ -285,13 +299,14 @@ __device__ void col_batch(LOAD load, long long L, int L1, int L2, int c0, int cn
 
 // Rows [r0, r0 + cnt) (cnt <= width) of `buf` (rows of L1 contiguous points)
 // transformed in place with sign s.
+template <class KI>
 __device__ void row_batch(int L1, int r0, int cnt, int width, double s, double2 *lds, double2 *buf, int order, const u4 *bx)
 {
     const int tid = threadIdx.x, nthr = blockDim.x;
     double2 *row = buf + (long long)r0 * L1;
     for (int e = tid; e < width * L1; e += nthr) lds[lpad(e)] = e < cnt * L1 ? row[e] : make_double2(0.0, 0.0);
     __syncthreads();
-    rg_fft(lds, L1, width, s, order, bx);
+    rg_fft<KI>(lds, L1, width, s, order, bx);
     for (int e = tid; e < cnt * L1; e += nthr) row[e] = lds[lpad(e)];
     __syncthreads();
 }
@@ -299,6 +314,7 @@ __device__ void row_batch(int L1, int r0, int cnt, int width, double s, double2 
 // Bluestein row tile: rows [r0, r0 + cnt) of the workspace (row k2, point k1)
 // in place: forward FFT, times the chirp kernel's spectrum (natural order: bin
 // k2 + L2 k1, read in runs of cnt bins), inverse FFT, times W_L^(+a k2).
+template <class KI>
 __device__ void brows_batch(long long L, int L1, int L2, int r0, int cnt, int width, double2 *lds, double2 *ws,
                             const double2 *spec, int order, const u4 *bx)
 {
@@ -306,14 +322,14 @@ __device__ void brows_batch(long long L, int L1, int L2, int r0, int cnt, int wi
     double2 *row = ws + (long long)r0 * L1;
     for (int e = tid; e < width * L1; e += nthr) lds[lpad(e)] = e < cnt * L1 ? row[e] : make_double2(0.0, 0.0);
     __syncthreads();
-    rg_fft(lds, L1, width, -1.0, order, bx);
+    rg_fft<KI>(lds, L1, width, -1.0, order, bx);
     for (int e = tid; e < cnt * L1; e += nthr) {
         const int t = e % cnt, k1 = e / cnt;
         const int q = lpad(t * L1 + k1);
         lds[q] = cmul(lds[q], spec[(long long)(r0 + t) + (long long)L2 * k1]);
     }
     __syncthreads();
-    rg_fft(lds, L1, width, 1.0, order, bx);
+    rg_fft<KI>(lds, L1, width, 1.0, order, bx);
     for (int e = tid; e < cnt * L1; e += nthr) {
         const int t = e / L1, a = e - t * L1;
         row[e] = cmul(lds[lpad(e)], twid((long long)a * (r0 + t), L, 1.0));
@@ -324,7 +340,7 @@ __device__ void brows_batch(long long L, int L1, int L2, int r0, int cnt, int wi
 // Inverse column tile: columns [c0, c0 + cnt) of the workspace (length L2,
 // stride L1) transformed with sign +1; emit(k, v) receives c[k], k = a + L1 b,
 // for the tile's own columns (so an in-place emit touches only its columns).
-template <class EMIT>
+template <class KI, class EMIT>
 __device__ void icols_batch(EMIT emit, int L1, int L2, int c0, int cnt, int width, double2 *lds, const double2 *ws,
                             int order, const u4 *bx)
 {
@@ -334,7 +350,7 @@ __device__ void icols_batch(EMIT emit, int L1, int L2, int c0, int cnt, int widt
         lds[lpad(t * L2 + k2)] = t < cnt ? ws[(long long)(c0 + t) + (long long)L1 * k2] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    rg_fft(lds, L2, width, 1.0, order, bx);
+    rg_fft<KI>(lds, L2, width, 1.0, order, bx);
     for (int e = tid; e < width * L2; e += nthr) {
         const int t = e % width, b = e / width;
         if (t < cnt) emit((long long)(c0 + t) + (long long)L1 * b, lds[lpad(t * L2 + b)]);
@@ -375,6 +391,7 @@ __device__ __forceinline__ void emit_bin(const RaggedTask &T, long long k, doubl
 // of a bin k = k2 + L2 k1 is point L1 - 1 - k1 of the mirror row, so every bin
 // of the band's 2R rows is finished here; the stores run over R consecutive
 // bins (k2 fastest).
+template <class KI>
 __device__ void rows_split(const RaggedTask &T, double2 *lds, const double2 *ws, double2 *y, double2 *y2, double f,
                            double fo, int order, const u4 *bx)
 {
@@ -390,7 +407,7 @@ __device__ void rows_split(const RaggedTask &T, double2 *lds, const double2 *ws,
         lds[lpad(e)] = v;
     }
     __syncthreads();
-    rg_fft(lds, L1, W, -1.0, order, bx);
+    rg_fft<KI>(lds, L1, W, -1.0, order, bx);
     for (int e = tid; e < 2 * R * L1; e += nthr) {
         const int t = e % (2 * R), k1 = e / (2 * R);
         const int row = t < R ? r0 + t : L2 - r0 - (t - R);
@@ -404,6 +421,7 @@ __device__ void rows_split(const RaggedTask &T, double2 *lds, const double2 *ws,
 // The self-paired rows of a smooth split: row 0 (Z[M - L2 k1] is its point
 // (L1 - k1) mod L1) in slot 0, and for even L2 row L2/2 (mirror point
 // L1 - 1 - k1) in slot 1; for even n also X[M] = Re Z[0] - Im Z[0].
+template <class KI>
 __device__ void self_split(const RaggedTask &T, double2 *lds, const double2 *ws, double2 *y, double2 *y2, double f,
                            double fo, int order, const u4 *bx)
 {
@@ -415,7 +433,7 @@ __device__ void self_split(const RaggedTask &T, double2 *lds, const double2 *ws,
         lds[lpad(e)] = (t == 0 || mid) ? ws[(long long)(t ? L2 / 2 : 0) * L1 + k1] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    rg_fft(lds, L1, 2, -1.0, order, bx);
+    rg_fft<KI>(lds, L1, 2, -1.0, order, bx);
     for (int e = tid; e < 2 * L1; e += nthr) {
         const int t = e & 1, k1 = e >> 1;
         if (t == 1 && !mid) continue;
@@ -445,6 +463,7 @@ __global__ void k_ragged_reset(int *counter)
 
 // the chirp kernel's spectrum of one Bluestein length into the table, natural
 // order (create time; the split path goes through the workgroup's scratch `S0`)
+template <class KI>
 __device__ void spectrum_task(const RaggedTask &T, const RaggedParams &P, double2 *lds, double2 *S0)
 {
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -462,16 +481,16 @@ __device__ void spectrum_task(const RaggedTask &T, const RaggedParams &P, double
     if (T.L1 == 0) {
         for (long long m = tid; m < n2; m += nthr) lds[lpad((int)m)] = bload(m);
         __syncthreads();
-        lds_fft(lds, (int)n2, 1, -1.0, P.order, TwSincos{});
+        lds_fft(lds, (int)n2, 1, -1.0, P.order, TwK<KI>{});
         for (long long m = tid; m < n2; m += nthr) B[m] = lds[lpad((int)m)];
         __syncthreads();
         return;
     }
     // width = columns per batch, count = rows per batch
     for (int c0 = 0; c0 < T.L1; c0 += T.width)
-        col_batch(bload, n2, T.L1, T.L2, c0, min(T.width, T.L1 - c0), T.width, -1.0, lds, S0, P.order, nullptr);
+        col_batch<KI>(bload, n2, T.L1, T.L2, c0, min(T.width, T.L1 - c0), T.width, -1.0, lds, S0, P.order, nullptr);
     for (int r0 = 0; r0 < T.L2; r0 += T.count)
-        row_batch(T.L1, r0, min(T.count, T.L2 - r0), T.count, -1.0, lds, S0, P.order, nullptr);
+        row_batch<KI>(T.L1, r0, min(T.count, T.L2 - r0), T.count, -1.0, lds, S0, P.order, nullptr);
     // row layout (S0[k2 L1 + k1] = bin k2 + L2 k1) to natural order
     for (long long m = tid; m < n2; m += nthr) {
         const long long k2 = m / T.L1, k1 = m - k2 * T.L1;
@@ -495,6 +514,10 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
     extern __shared__ double2 lds[];
     __shared__ int s_task;
     const u4 *bx = MF ? P.bx : nullptr;   // the FP64 instances carry no matrix-core stage
+    // the matrix-core instances share one set (the 128-register budget): with each instance's
+    // own budget their stage functions clobber more of the kernel's registers, and the kernel
+    // spills around the calls instead (C4: 32.2 ms shared against 38.5-40.9 ms per instance)
+    using KI = RgInst<MF ? 1024 : MAXT, MF ? 0 : WPE, MF>;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int home = (int)(blockIdx.x & 7);
     int visited = 0;   // queues found empty so far (thread 0's count is the one that matters)
@@ -518,7 +541,7 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
         if (task < 0) break;
         const RaggedTask T = P.tasks[task];
         if (P.mode == 1) {
-            spectrum_task(T, P, lds, P.scratch + (long long)blockIdx.x * P.slot);
+            spectrum_task<KI>(T, P, lds, P.scratch + (long long)blockIdx.x * P.slot);
             continue;
         }
         const long long n = T.n, M = T.M;
@@ -536,19 +559,19 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
         const double f = P.fct, fo = P.fct * P.osign;
         switch (T.kind) {
             case RG_COLS:
-                col_batch(zin, T.L, T.L1, T.L2, T.first, T.count, T.width, -1.0, lds, ws, P.order, bx);
+                col_batch<KI>(zin, T.L, T.L1, T.L2, T.first, T.count, T.width, -1.0, lds, ws, P.order, bx);
                 break;
             case RG_BCOLS: {
                 auto aload = [&](long long m) -> double2 {
                     return m < M ? cmul(zin(m), chirp(m, M, inv2m)) : make_double2(0.0, 0.0);
                 };
-                col_batch(aload, T.L, T.L1, T.L2, T.first, T.count, T.width, -1.0, lds, ws, P.order, bx);
+                col_batch<KI>(aload, T.L, T.L1, T.L2, T.first, T.count, T.width, -1.0, lds, ws, P.order, bx);
                 break;
             }
-            case RG_ROWS: rows_split(T, lds, ws, y, y2, f, fo, P.order, bx); break;
-            case RG_SELF: self_split(T, lds, ws, y, y2, f, fo, P.order, bx); break;
+            case RG_ROWS: rows_split<KI>(T, lds, ws, y, y2, f, fo, P.order, bx); break;
+            case RG_SELF: self_split<KI>(T, lds, ws, y, y2, f, fo, P.order, bx); break;
             case RG_BROWS:
-                brows_batch(T.L, T.L1, T.L2, T.first, T.count, T.width, lds, ws, P.spec + T.spec_off, P.order, bx);
+                brows_batch<KI>(T.L, T.L1, T.L2, T.first, T.count, T.width, lds, ws, P.spec + T.spec_off, P.order, bx);
                 break;
             case RG_ICOLS:
                 if (half || pair) {
@@ -556,7 +579,7 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
                     auto keep = [&](long long k, double2 v) {
                         if (k < M) ws[k] = v;
                     };
-                    icols_batch(keep, T.L1, T.L2, T.first, T.count, T.width, lds, ws, P.order, bx);
+                    icols_batch<KI>(keep, T.L1, T.L2, T.first, T.count, T.width, lds, ws, P.order, bx);
                 } else {
                     const long long K = n / 2 + 1;
                     const double sc = 1.0 / (double)T.L;
@@ -566,7 +589,7 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
                             y[k] = make_double2(f * z.x, fo * z.y);
                         }
                     };
-                    icols_batch(emit, T.L1, T.L2, T.first, T.count, T.width, lds, ws, P.order, bx);
+                    icols_batch<KI>(emit, T.L1, T.L2, T.first, T.count, T.width, lds, ws, P.order, bx);
                 }
                 break;
             case RG_SPLIT: {
@@ -587,17 +610,17 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE ?
                 if (!T.blue) {
                     for (long long m = tid; m < L; m += nthr) lds[lpad((int)m)] = zin(m);
                     __syncthreads();
-                    rg_fft(lds, (int)L, 1, -1.0, P.order, bx);
+                    rg_fft<KI>(lds, (int)L, 1, -1.0, P.order, bx);
                 } else {
                     // A = FFT(z w); C = A B; c = IFFT(C) (unnormalized; 1/n2 applied with the output chirp)
                     const double2 *B = P.spec + T.spec_off;
                     for (long long m = tid; m < L; m += nthr)
                         lds[lpad((int)m)] = m < M ? cmul(zin(m), chirp(m, M, inv2m)) : make_double2(0.0, 0.0);
                     __syncthreads();
-                    rg_fft(lds, (int)L, 1, -1.0, P.order, bx);
+                    rg_fft<KI>(lds, (int)L, 1, -1.0, P.order, bx);
                     for (long long m = tid; m < L; m += nthr) lds[lpad((int)m)] = cmul(lds[lpad((int)m)], B[m]);
                     __syncthreads();
-                    rg_fft(lds, (int)L, 1, 1.0, P.order, bx);
+                    rg_fft<KI>(lds, (int)L, 1, 1.0, P.order, bx);
                 }
                 const double bscale = T.blue ? 1.0 / (double)L : 1.0;
                 auto Zl = [&](long long k) -> double2 {

@@ -148,6 +148,14 @@ __device__ __forceinline__ double2 bload2(__amdgpu_buffer_rsrc_t r, int off)
     return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
 }
 
+// a transform's input; NT = true marks the load non-temporal (streamed past the L2), for
+// passes whose lines no other workgroup reads
+template <bool NT>
+__device__ __forceinline__ double2 bload2_data(__amdgpu_buffer_rsrc_t r, int off)
+{
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, NT ? 2 : 0));
+}
+
 __device__ __forceinline__ void bstore2(__amdgpu_buffer_rsrc_t r, int off, double2 v)
 {
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned int, v),

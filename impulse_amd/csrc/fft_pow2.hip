@@ -200,13 +200,18 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
     double2 v[16];
     // ---- stage 0 (radix 16, p = 1): inputs x[i + q N/16] from global memory
     if constexpr (FAST) {
+        // The r2c row-pair pass reads whole rows that no other workgroup touches: its loads are
+        // non-temporal, which keeps the L2 for merging its short strided stores (pass time
+        // 0.337 -> 0.330 ms at r2c 2^22 x16).  The column passes read 32- to 128-byte pieces of
+        // lines their neighbours read too, and lose 4-17 % with the same hint.
+        constexpr bool NTLOAD = IO == IO_R2C_ROWPAIR;
         // byte offsets inside the source window fit 31 bits (checked at plan time)
         const __amdgpu_buffer_rsrc_t rin = make_rsrc((const char *)P.src + P.src_lo, P.src_span);
         const int step = (int)(P.in.pst * P.in.sax * 16);
         const int base = (int)((T.ia + T.ip0 * P.in.sax) * 16 - P.src_lo) + i * step;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
-            const double2 x = bload2(rin, base + q * (N / 16) * step);
+            const double2 x = bload2_data<NTLOAD>(rin, base + q * (N / 16) * step);
             v[q] = valid ? x : make_double2(0.0, 0.0);
         }
     } else {

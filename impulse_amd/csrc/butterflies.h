@@ -172,6 +172,18 @@ template <> __device__ __forceinline__ void dft<11>(double2 *v, double s) { dft_
 template <> __device__ __forceinline__ void dft<13>(double2 *v, double s) { dft_odd<13>(v, s); }
 template <> __device__ __forceinline__ void dft<16>(double2 *v, double s) { dft16(v, s); }
 
+// b mod p for 0 <= b < 2^13 and p >= 1, from rp ~ 1 / p in FP32 (the hardware
+// reciprocal, within 1 ulp): the quotient estimate b rp is within 2^-9 of b / p,
+// so one correction step gives the exact remainder.  About six VALU
+// instructions against the seventeen of the integer division.
+__device__ __forceinline__ int mod_small(int b, int p, float rp)
+{
+    const int q = (int)((float)b * rp);
+    int r = b - q * p;
+    r = r < 0 ? r + p : r;
+    return r >= p ? r - p : r;
+}
+
 // Butterflies a thread may hold per stage of radix R (register budget: 16 points).
 __host__ __device__ constexpr int nb_max(int R) { return R >= 16 ? 1 : 16 / R; }
 

@@ -29,6 +29,7 @@ __device__ __noinline__ void stage(double2 *__restrict__ buf, int N, int p, int 
 {
     const int nb = N / R;
     const int twstep = N / (p * R);
+    const float rp = __builtin_amdgcn_rcpf((float)p);
     double2 v[NB][R];
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
@@ -43,7 +44,7 @@ __device__ __noinline__ void stage(double2 *__restrict__ buf, int N, int p, int 
     for (int c = 0; c < NB; ++c) {
         const int b = i + c * NT;
         if (b < nb) {
-            const int k = b % p;
+            const int k = mod_small(b, p, rp);   // b < N / R <= 4096
             if (p > 1) {
 #pragma unroll
                 for (int q = 1; q < R; ++q) v[c][q] = cmul(v[c][q], tw_sign(tw[q * k * twstep], s));

@@ -44,31 +44,22 @@ struct TwTable {
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
 constexpr int lds_points_padded(int n) { return n + n / 16 + 1; }
 
-// b mod p for 0 <= b < 2^13 and p >= 1, from rp ~ 1 / p in FP32 (the hardware
-// reciprocal, within 1 ulp): the quotient estimate b rp is within 2^-9 of b / p,
-// so one correction step gives the exact remainder.  About six VALU
-// instructions against the seventeen of the integer division.
-__device__ __forceinline__ int mod_small(int b, int p, float rp)
-{
-    const int q = (int)((float)b * rp);
-    int r = b - q * p;
-    r = r < 0 ? r + p : r;
-    return r >= p ? r - p : r;
-}
-
 // Stockham stage on one transform in LDS: butterflies b = i, i + NT, ...
 // The twiddle roots are generated (or their table loads issued) before the
 // values are loaded, so that work does not hold the R NB values live.
-template <int R, int NB, typename TW>
+// SCALAR: read N, p and NT (the same for every thread of the workgroup: the transforms of one
+// LDS batch share their length) as scalars.  Inlined into a kernel that keeps the stage loop
+// in scalar registers this took the fused DCT rows from 0.118 to 0.083 ms and the convolution
+// from 1.96 to 1.80 ms; in the stage calls of the ragged kernel it cost 1.4 %, so they skip it.
+template <int R, int NB, typename TW, bool SCALAR = false>
 __device__ __forceinline__ void rstage_body(double2 *lds, int base, int N, int p, int i, int NT, double s,
                                             const TW &tw)
 {
-    // N, p and NT are the same for every thread of the workgroup (the transforms of one LDS
-    // batch share their length); reading them as scalars keeps their arithmetic off the VALU
-    // when the stage is a call of its own
-    N = __builtin_amdgcn_readfirstlane(N);
-    p = __builtin_amdgcn_readfirstlane(p);
-    NT = __builtin_amdgcn_readfirstlane(NT);
+    if constexpr (SCALAR) {
+        N = __builtin_amdgcn_readfirstlane(N);
+        p = __builtin_amdgcn_readfirstlane(p);
+        NT = __builtin_amdgcn_readfirstlane(NT);
+    }
     const int nb = N / R;
     const float rp = __builtin_amdgcn_rcpf((float)p);
     int kk[NB];
@@ -205,24 +196,24 @@ __device__ __forceinline__ void lds_fft_inline(double2 *lds, int L, int count, d
         const int code = (int)(codes & 15);
         if constexpr (SMALL) {
             switch (code) {
-                case 1: rstage_body<2, nb_ppt<PPT>(2)>(lds, base, L, p, i, NT, s, tw); p *= 2; break;
-                case 2: rstage_body<3, nb_ppt<PPT>(3)>(lds, base, L, p, i, NT, s, tw); p *= 3; break;
-                case 3: rstage_body<4, nb_ppt<PPT>(4)>(lds, base, L, p, i, NT, s, tw); p *= 4; break;
-                case 4: rstage_body<5, nb_ppt<PPT>(5)>(lds, base, L, p, i, NT, s, tw); p *= 5; break;
-                case 5: rstage_body<7, nb_ppt<PPT>(7)>(lds, base, L, p, i, NT, s, tw); p *= 7; break;
-                default: rstage_body<8, nb_ppt<PPT>(8)>(lds, base, L, p, i, NT, s, tw); p *= 8; break;
+                case 1: rstage_body<2, nb_ppt<PPT>(2), TW, true>(lds, base, L, p, i, NT, s, tw); p *= 2; break;
+                case 2: rstage_body<3, nb_ppt<PPT>(3), TW, true>(lds, base, L, p, i, NT, s, tw); p *= 3; break;
+                case 3: rstage_body<4, nb_ppt<PPT>(4), TW, true>(lds, base, L, p, i, NT, s, tw); p *= 4; break;
+                case 4: rstage_body<5, nb_ppt<PPT>(5), TW, true>(lds, base, L, p, i, NT, s, tw); p *= 5; break;
+                case 5: rstage_body<7, nb_ppt<PPT>(7), TW, true>(lds, base, L, p, i, NT, s, tw); p *= 7; break;
+                default: rstage_body<8, nb_ppt<PPT>(8), TW, true>(lds, base, L, p, i, NT, s, tw); p *= 8; break;
             }
         } else {
             switch (code) {
-                case 1: rstage_body<2, nb_ppt<PPT>(2)>(lds, base, L, p, i, NT, s, tw); p *= 2; break;
-                case 2: rstage_body<3, nb_ppt<PPT>(3)>(lds, base, L, p, i, NT, s, tw); p *= 3; break;
-                case 3: rstage_body<4, nb_ppt<PPT>(4)>(lds, base, L, p, i, NT, s, tw); p *= 4; break;
-                case 4: rstage_body<5, nb_ppt<PPT>(5)>(lds, base, L, p, i, NT, s, tw); p *= 5; break;
-                case 5: rstage_body<7, nb_ppt<PPT>(7)>(lds, base, L, p, i, NT, s, tw); p *= 7; break;
-                case 6: rstage_body<8, nb_ppt<PPT>(8)>(lds, base, L, p, i, NT, s, tw); p *= 8; break;
-                case 7: rstage_body<11, nb_ppt<PPT>(11)>(lds, base, L, p, i, NT, s, tw); p *= 11; break;
-                case 8: rstage_body<13, nb_ppt<PPT>(13)>(lds, base, L, p, i, NT, s, tw); p *= 13; break;
-                default: rstage_body<16, nb_ppt<PPT>(16)>(lds, base, L, p, i, NT, s, tw); p *= 16; break;
+                case 1: rstage_body<2, nb_ppt<PPT>(2), TW, true>(lds, base, L, p, i, NT, s, tw); p *= 2; break;
+                case 2: rstage_body<3, nb_ppt<PPT>(3), TW, true>(lds, base, L, p, i, NT, s, tw); p *= 3; break;
+                case 3: rstage_body<4, nb_ppt<PPT>(4), TW, true>(lds, base, L, p, i, NT, s, tw); p *= 4; break;
+                case 4: rstage_body<5, nb_ppt<PPT>(5), TW, true>(lds, base, L, p, i, NT, s, tw); p *= 5; break;
+                case 5: rstage_body<7, nb_ppt<PPT>(7), TW, true>(lds, base, L, p, i, NT, s, tw); p *= 7; break;
+                case 6: rstage_body<8, nb_ppt<PPT>(8), TW, true>(lds, base, L, p, i, NT, s, tw); p *= 8; break;
+                case 7: rstage_body<11, nb_ppt<PPT>(11), TW, true>(lds, base, L, p, i, NT, s, tw); p *= 11; break;
+                case 8: rstage_body<13, nb_ppt<PPT>(13), TW, true>(lds, base, L, p, i, NT, s, tw); p *= 13; break;
+                default: rstage_body<16, nb_ppt<PPT>(16), TW, true>(lds, base, L, p, i, NT, s, tw); p *= 16; break;
             }
         }
     }

@@ -1,5 +1,5 @@
 """Matrix-core complex FFTs (impulse_fft_mfma_*) against the FP64 oracle, over
-the powers of two from 256 to 2^20.
+the powers of two from 256 to 2^21.
 
 Input is complex64; the reference is the oracle's c2c of the same values
 widened to complex128.  Bars (relative L2 per transform), stated here:
@@ -42,7 +42,8 @@ def _rel_rows(a, b):
 # two: the matrix-core column pass and FP64 row transforms; odd batches leave a partial last tile
 @pytest.mark.parametrize("prec", ["bf16x", "f32"])
 @pytest.mark.parametrize("length,batch", [(256, 37), (256, 16), (65536, 3), (512, 7), (1024, 9), (2048, 5),
-                                          (4096, 5), (8192, 3), (16384, 2), (131072, 2), (1 << 20, 1)])
+                                          (4096, 5), (8192, 3), (16384, 2), (131072, 2), (1 << 20, 1),
+                                          (1 << 21, 1)])
 @pytest.mark.parametrize("forward", [True, False])
 def test_against_oracle(prec, length, batch, forward):
     print(f"case mfma {prec} L={length} batch={batch} forward={forward}", flush=True)

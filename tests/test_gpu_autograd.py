@@ -249,7 +249,7 @@ def test_gradients_through_conjugate_views():
         G = torch.randn_like(Y)
         loss = (Y.conj() * G).real.sum()
         (gx,) = torch.autograd.grad(loss, x)
-        assert abs(float(loss) - float((x.detach() * gx).sum())) <= 1e-10 * float(x.norm() * gx.norm()), n
+        assert abs(float(loss.detach()) - float((x.detach() * gx).sum())) <= 1e-10 * float(x.detach().norm() * gx.norm()), n
         Z = torch.randn(3, n // 2 + 1, dtype=torch.complex128, device="cuda", requires_grad=True)
         z = AG.irfftn(Z.conj(), n, [1])          # a conjugate view as the input, too
         w = torch.randn_like(z)

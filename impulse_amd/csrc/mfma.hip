@@ -266,11 +266,13 @@ k_mfma256(const MfmaParams P)
 #pragma unroll
         for (int r = 0; r < 4; ++r) w1[r] = tw_sign(tws[(4 * g + r) * j], sign);
     }
-    // W_L^(t k) = W_L^(m mod 256) W_n2^(m / 256), m = t k < L: the first factor from LDS, the second
-    // from LDS for n2 = 256 (W_256) and from the W_n2 table otherwise; product in FP64
+    // W_L^(t k) = W_L^(m mod 256) W_n2^(m / 256), m = t k < L: the first factor from LDS; the second
+    // for n2 <= 256 is W_256^((m / 256) (256 / n2)), from LDS too, and from the W_n2 table otherwise;
+    // product in FP64
+    const int hstep = P.n2 <= 256 ? 256 >> P.ln2 : 0;
     auto step_tw = [&](int t, int k) -> double2 {
         const int m = t * k;
-        const double2 hi = P.n2 == 256 ? tws[m >> 8] : P.tw2[m >> 8];
+        const double2 hi = hstep ? tws[(m >> 8) * hstep] : P.tw2[m >> 8];
         return tw_sign(cmul(tws[256 + (m & 255)], hi), sign);
     };
     auto col_t = [&](int c) { return (int)((tile * TILE + c) & (P.n2 - 1)); };

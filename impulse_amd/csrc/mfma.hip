@@ -50,10 +50,12 @@ namespace ifft {
 
 
 enum { MF_F32 = 0, MF_BF16X = 1 };
-// MP_FUSED: the column pass of L = 256 n2 for n2 <= 16, with the n2-point row DFTs of the
-// tile's whole signals done in LDS before the store (one pass, no workspace)
+// MP_FUSED: the column pass of L = 256 n2 for n2 <= 16 (and n2 = 32 with FP32 data, whose
+// 32-column tile is one whole signal), with the n2-point row DFTs of the tile's whole signals
+// done in LDS before the store (one pass, no workspace)
 enum { MP_COL = 0, MP_ROW = 1, MP_SINGLE = 2, MP_FUSED = 3 };
 constexpr int kFusedMaxN2 = 16;
+__host__ __device__ constexpr bool fused_n2(int n2, int var) { return (n2 > 1 && n2 <= kFusedMaxN2) || (n2 == 32 && var == 0); }
 constexpr int kRow = 273;           // LDS row stride (elements): 256 + 256/16 padding + 1
 // sub-transforms per workgroup (four per wavefront): 32 for FP32 data (256-byte
 // segments in the strided passes), 16 for FP64 data (LDS budget)
@@ -159,6 +161,41 @@ __device__ __forceinline__ void put_out(double2 *y, long long i, double2 v) { y[
 // (column g n2 + t of the tile holds the twiddled 256-point transform of column t of signal
 // s0 + g).  X[k1 + 256 k2] = sum_t T[t][k1] W_n2^(t k2) in FP64, scaled by fct; one job per
 // (signal, k1), k1 fastest, so the stores run over 256 consecutive outputs.
+// n2 = 32 with FP32 data (the tile is one signal): the 32-point row DFTs as radix 8 x 4 in LDS.
+// Step 1, per (k1, t1): the 8-point DFT over t2 of T[t1 + 4 t2][k1], times W_32^(t1 q), back into
+// the same columns (column t1 + 4 q; each job reads and writes only its own eight); step 2, per
+// (k1, q): X[k1 + 256 (q + 8 r)] = sum_t1 W_4^(t1 r) Y[t1 + 4 q][k1].  W_32^j = W_256^(8 j) comes
+// from the LDS table tws (forward sign).
+template <typename O>
+__device__ __forceinline__ void fused_row_dft32(float2 *lds, const double2 *tws, O *y, long long sg, long long L,
+                                                double sign, double fct, int tid, int NT)
+{
+    for (int q = tid; q < 4 * 256; q += NT) {
+        const int k1 = q & 255, t1 = q >> 8;
+        double2 v[8];
+#pragma unroll
+        for (int t2 = 0; t2 < 8; ++t2) v[t2] = to_d2(lds[(t1 + 4 * t2) * kRow + lidx(k1)]);
+        dft<8>(v, sign);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const double2 w = v[k];
+            const double2 r = t1 * k ? cmul(w, tw_sign(tws[8 * t1 * k], sign)) : w;
+            lds[(t1 + 4 * k) * kRow + lidx(k1)] = make_float2((float)r.x, (float)r.y);
+        }
+    }
+    __syncthreads();
+    O *ys = y + sg * L;
+    for (int q = tid; q < 8 * 256; q += NT) {
+        const int k1 = q & 255, k2a = q >> 8;
+        double2 v[4];
+#pragma unroll
+        for (int t1 = 0; t1 < 4; ++t1) v[t1] = to_d2(lds[(t1 + 4 * k2a) * kRow + lidx(k1)]);
+        dft<4>(v, sign);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) put_out(ys, k1 + 256LL * (k2a + 8 * r), cscale(v[r], fct));
+    }
+}
+
 template <int N2, typename E, typename O>
 __device__ __forceinline__ void fused_row_dfts(const E *lds, O *y, long long s0, long long batch, long long L,
                                                int ns, double sign, double fct, int tid, int NT)
@@ -325,6 +362,12 @@ k_mfma256(const MfmaParams P)
         O *y = (O *)P.dst;
         const int ns = TILE / P.n2;
         const long long s0 = tile * ns;
+        if constexpr (VAR == MF_F32) {
+            if (P.n2 == 32) {   // one signal per tile
+                if (s0 < P.batch) fused_row_dft32(lds, tws, y, s0, L, sign, P.fct, tid, NT);
+                return;
+            }
+        }
         switch (P.n2) {
             case 2: fused_row_dfts<2>(lds, y, s0, P.batch, L, ns, sign, P.fct, tid, NT); break;
             case 4: fused_row_dfts<4>(lds, y, s0, P.batch, L, ns, sign, P.fct, tid, NT); break;
@@ -468,7 +511,7 @@ int mfma_create(Mfma **out, size_t length, size_t batch, int precision, Error &e
         rc = upload(M->tw2, t2.data(), t2.size() * sizeof(double2));
     }
     const size_t es = precision == MF_F32 ? sizeof(float2) : sizeof(double2);
-    const bool fused = M->n2 > 1 && M->n2 <= kFusedMaxN2;
+    const bool fused = fused_n2(M->n2, precision);
     if (rc == OK && length > 256 && !fused && M->tmp.alloc((size_t)length * batch * es)) rc = E_NOMEM;
     if (rc != OK) {
         err = {rc, "matrix-core FFT table or workspace allocation failed"};
@@ -521,7 +564,7 @@ int mfma_execute(Mfma &m, const void *in, void *out, int forward, double fct, hi
                                      : launch_var<MF_BF16X>(P, pass, m.batch, s);
     };
     hipError_t e;
-    const bool fused = m.n2 > 1 && m.n2 <= kFusedMaxN2;
+    const bool fused = fused_n2(m.n2, m.precision);
     if (m.L == 256) {
         e = go(MP_SINGLE, in, out);
     } else if (fused) {

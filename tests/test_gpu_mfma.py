@@ -37,9 +37,10 @@ def _rel_rows(a, b):
     return np.linalg.norm(a - b, axis=-1) / np.linalg.norm(b, axis=-1)
 
 
-# 256: one pass; 512 .. 4096: one pass with the length/256-point row DFTs in LDS (tiles of
-# several whole signals); 65536: both four-step passes on the matrix cores; the other powers of
-# two: the matrix-core column pass and FP64 row transforms; odd batches leave a partial last tile
+# 256: one pass; 512 .. 4096 (and 8192 for F32): one pass with the length/256-point row DFTs in
+# LDS (tiles of whole signals); 65536: both four-step passes on the matrix cores; the other
+# powers of two: the matrix-core column pass and FP64 row transforms; odd batches leave a
+# partial last tile
 @pytest.mark.parametrize("prec", ["bf16x", "f32"])
 @pytest.mark.parametrize("length,batch", [(256, 37), (256, 16), (65536, 3), (512, 7), (1024, 9), (2048, 5),
                                           (4096, 5), (8192, 3), (16384, 2), (131072, 2), (1 << 20, 1),
@@ -94,11 +95,12 @@ def test_rejects_bad_arguments():
 
 
 def test_workspace_only_for_two_pass_lengths():
-    # 256 and the one-pass lengths (512 .. 4096) hold no intermediate; from 8192 on, one
-    # intermediate of the output type
-    for L in (256, 512, 4096):
+    # 256 and the one-pass lengths (512 .. 4096, and 8192 for F32, whose 32-column tile is one
+    # whole signal) hold no intermediate; longer ones, one intermediate of the output type
+    for L in (256, 512, 4096, 8192):
         assert MfmaFFT(L, 3, "f32").workspace_bytes == 0
-    assert MfmaFFT(8192, 3, "f32").workspace_bytes == 8192 * 3 * 8
+    assert MfmaFFT(4096, 3, "bf16x").workspace_bytes == 0
+    assert MfmaFFT(16384, 3, "f32").workspace_bytes == 16384 * 3 * 8
     assert MfmaFFT(8192, 3, "bf16x").workspace_bytes == 8192 * 3 * 16
 
 

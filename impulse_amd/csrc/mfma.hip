@@ -31,8 +31,9 @@
 //
 // Lengths: 256 (one pass) and 256 n2 for n2 = 2 .. 8192 (four-step: 256-point
 // column FFTs with the W_L^(t k) twiddle, then the n2-point row FFTs with a
-// transposed store -- in the same launch for n2 <= 16, on the matrix cores for
-// n2 = 256, by an FP64 LDS row kernel otherwise).  A
+// transposed store -- in the same launch for n2 <= 16 (and n2 = 32 with FP32
+// data), on the matrix cores for n2 = 256, by an FP64 LDS row kernel
+// otherwise).  A
 // workgroup of four wavefronts owns a tile of 16 sub-transforms in LDS; each
 // wavefront computes four of them.
 #include <algorithm>
@@ -479,8 +480,8 @@ static int upload(DevBuf &b, const void *p, size_t n)
 
 // L = 256 n2 as a four-step transform: the 256-point column transforms (stride n2) on the
 // matrix cores with the twiddle W_L^(t k1), then the n2-point row transforms with the
-// transposed store X[k1 + 256 k2] -- in the same launch for n2 <= 16, on the matrix cores for
-// n2 = 256, otherwise by the FP64 LDS row kernel.
+// transposed store X[k1 + 256 k2] -- in the same launch for n2 <= 16 (and n2 = 32 with FP32
+// data), on the matrix cores for n2 = 256, otherwise by the FP64 LDS row kernel.
 int mfma_create(Mfma **out, size_t length, size_t batch, int precision, Error &err)
 {
     if (!out) { err = {E_INVALID, "null handle pointer"}; return E_INVALID; }

@@ -354,8 +354,8 @@ int impulse_fft_ragged_destroy(impulse_fft_ragged batch);
  * to 2^21) whose 256-point factors run as two radix-16 stages of 16x16x16
  * complex matrix products on the MFMA units: one pass for 256, a four-step
  * 256 x (length/256) otherwise, whose (length/256)-point row transforms run in
- * the same launch up to 4096, on the MFMA units for 65536 and in an FP64 LDS
- * row kernel for the other lengths.
+ * the same launch up to 4096 (8192 with IMPULSE_FFT_MFMA_F32), on the MFMA
+ * units for 65536 and in an FP64 LDS row kernel for the other lengths.
  * Nearest reference: pocketfft::c2c<float> over one axis
  * (pocketfft_hdronly.h:3272-3282, impulse's FFTDesc[Complex32] through
  * cpp_pocketfft/pocketfft.nim:235-277), here with the precision stated:
@@ -367,9 +367,9 @@ int impulse_fft_ragged_destroy(impulse_fft_ragged batch);
  *                           bf16 MFMA and combined in FP64; FP64 twiddles;
  *                           complex128 output (error ~1e-12 relative).
  * forward = 1 computes exp(-2 pi i jk/n); fct scales the output (in FP64).
- * Buffers are DEVICE pointers; in and out must not overlap.  Plans of 8192
- * points and longer hold a workspace of one intermediate (batch * length
- * elements of the output type). */
+ * Buffers are DEVICE pointers; in and out must not overlap.  Two-pass plans
+ * (longer than 4096 points, or 8192 with IMPULSE_FFT_MFMA_F32) hold a workspace
+ * of one intermediate (batch * length elements of the output type). */
 typedef enum impulse_fft_mfma_precision {
     IMPULSE_FFT_MFMA_F32 = 0,
     IMPULSE_FFT_MFMA_BF16X = 1

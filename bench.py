@@ -667,7 +667,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU runs are not part of this build", file=sys.stderr)
+              "the engine has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
     import torch
 
@@ -770,7 +770,7 @@ def main():
                                        upfirdn_form(torch, args.form_reps), fftpack_form(torch, args.form_reps)]
     for f in forms:
         if f.get("form") == "ragged":
-            # the north-star comparison: ragged-batch throughput against the FP64 drop-in's
+            # ragged-batch throughput (FP64 and matrix-core stages) against the FP64 drop-in's
             f["vs_fp64_dropin_gflops"] = round(f["gflops"] / value, 3)
             f["mfma_bf16x"]["vs_fp64_dropin_gflops"] = round(f["mfma_bf16x"]["gflops"] / value, 3)
     vendor = None if args.no_vendor else vendor_fft(torch, plist, bufs, ms_per_step, args.steps)

@@ -255,7 +255,7 @@ def test_gradients_through_conjugate_views():
         w = torch.randn_like(z)
         loss2 = (z * w).sum()
         (gZ,) = torch.autograd.grad(loss2, Z)
-        assert abs(float(loss2) - float((Z.detach().conj() * gZ).real.sum())) <= \
+        assert abs(float(loss2.detach()) - float((Z.detach().conj() * gZ).real.sum())) <= \
             1e-10 * float(Z.detach().abs().norm() * gZ.abs().norm()), n
     with pytest.raises(ValueError):
         from impulse_amd import fft as F

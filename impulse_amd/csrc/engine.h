@@ -24,7 +24,7 @@ hipError_t launch_unpack_halfcomplex(const double *hc, double2 *bins, long long 
 bool pow2_supported(int N);
 int pow2_lds_bytes(int N, int map);
 int pow2_tb_host(int N, int map);
-int pow2_io(const FftParams &P);
+int pow2_io(const FftParams &P, int map, unsigned grid, unsigned block, size_t lds);
 long long pow2_rowpair_grid(long long signals, long long L2, int TB);
 hipError_t launch_fft_pow2(const FftParams &P, int map, unsigned grid, unsigned block, size_t lds, hipStream_t s);
 

@@ -148,6 +148,11 @@ __device__ __forceinline__ double2 bload2(__amdgpu_buffer_rsrc_t r, int off)
     return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
 }
 
+__device__ __forceinline__ double bload1(__amdgpu_buffer_rsrc_t r, int off)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
 // a transform's input; NT = true marks the load non-temporal (streamed past the L2), for
 // passes whose lines no other workgroup reads
 template <bool NT>

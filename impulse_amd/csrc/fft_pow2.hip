@@ -63,7 +63,13 @@ template <int N, int MAP> constexpr int pow2_ld() { return N + N / 16 + (MAP == 
 // W^m, W^2m, W^4m, W^8m come from the table, every other power is one to three
 // products of those, and each twiddle is used as soon as it is formed so that
 // at most five are live at a time.
-enum Pow2Io { IO_GENERIC = 0, IO_PLAIN = 1, IO_TWIDDLE = 2, IO_R2C_ROWPAIR = 3, IO_CONV = 4 };
+// IO_BLUE_IN / IO_BLUE_OUT: the Bluestein column passes with direct buffer
+// accesses -- chirp-in loads (complex or odd real pairs) with a twiddled store,
+// and plain loads with a chirp-out complex store (rows and columns mappings)
+// IO_W4 (added to IO_CONV or a Bluestein mode) selects the instance held to four waves per
+// SIMD, for grids of a few rounds.
+enum Pow2Io { IO_GENERIC = 0, IO_PLAIN = 1, IO_TWIDDLE = 2, IO_R2C_ROWPAIR = 3, IO_CONV = 4, IO_BLUE_IN = 5,
+              IO_BLUE_OUT = 6, IO_W4 = 8 };
 
 // Row assignment of the fused real post-process (ST_R2C_ROWPAIR, rows of L1 = N
 // points, L2 rows per signal, Z[j + L2 n] in row j).  Z[M - k] for k = j + L2 n
@@ -162,14 +168,16 @@ __device__ __forceinline__ void dft_groups(double2 *v, double s)
 #ifndef IFFT_COL_WPE
 #define IFFT_COL_WPE 3
 #endif
-template <int MAP> constexpr int pow2_wpe()
+template <int MAP, int IOW> constexpr int pow2_wpe()
 {
+    if (IOW & IO_W4) return 4;
     return MAP == MAP_ROW ? 3 : (MAP == MAP_COL ? IFFT_COL_WPE : IFFT_ROWCOL_WPE);
 }
 
-template <int N, int MAP, int IO>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_wpe<MAP>()))) k_fft_pow2(const FftParams P)
+template <int N, int MAP, int IOW>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_wpe<MAP, IOW>()))) k_fft_pow2(const FftParams P)
 {
+    constexpr int IO = IOW & (IO_W4 - 1);
     extern __shared__ double lds[];
     __shared__ TfInfo info[MAP == MAP_ROWCOL ? kMaxTB : 1];
     constexpr bool COL = MAP == MAP_COL;
@@ -199,7 +207,37 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
 
     double2 v[16];
     // ---- stage 0 (radix 16, p = 1): inputs x[i + q N/16] from global memory
-    if constexpr (FAST) {
+    if constexpr (IO == IO_BLUE_IN) {
+        // z[pos] = (x[pos] or conj x[pos]) w[pos] for pos < ilim, zero beyond; x complex or the
+        // odd real pair xa + i xb.  Every load is issued (position 0 stands in for padding and
+        // idle slots) and the zero is selected afterwards, so no load waits on a branch.
+        // Positions and byte offsets fit 31 bits (checked at plan time).
+        const __amdgpu_buffer_rsrc_t rin = make_rsrc((const char *)P.src + P.src_lo, P.src_span);
+        const __amdgpu_buffer_rsrc_t rch = make_rsrc(P.chirp, P.ilim * 16);
+        const bool pairs = P.load_mode == LD_R2C_PAIR;
+        const int es = pairs ? 8 : 16;
+        const int ba = (int)(T.ia * es - P.src_lo), bb = (int)(T.ib * es - P.src_lo);
+        const int estep = (int)P.in.sax * es;
+        const int ilim = (int)P.ilim;
+        const double cj = P.conj_in ? -1.0 : 1.0;
+        const bool hasb = T.flags & 2;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int pos = (int)T.ip0 + (i + q * (N / 16)) * (int)P.in.pst;
+            const bool ok = valid && pos < ilim;
+            const int pc = ok ? pos : 0;
+            double2 z;
+            if (pairs) {
+                const double a = bload1(rin, ba + pc * estep), b = bload1(rin, bb + pc * estep);
+                z = make_double2(a, hasb ? b : 0.0);
+            } else {
+                z = bload2(rin, ba + pc * estep);
+            }
+            z.y *= cj;
+            const double2 w = bload2(rch, pc * 16);
+            v[q] = ok ? cmul(z, w) : make_double2(0.0, 0.0);
+        }
+    } else if constexpr (FAST) {
         // The r2c row-pair pass reads whole rows that no other workgroup touches: its loads are
         // non-temporal, which keeps the L2 for merging its short strided stores (pass time
         // 0.337 -> 0.330 ms at r2c 2^22 x16).  The column passes read 32- to 128-byte pieces of
@@ -318,7 +356,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
             const __amdgpu_buffer_rsrc_t rout = make_rsrc((char *)P.dst + P.dst_lo, P.dst_span);
             const int ostep = (int)(P.out.pst * P.out.sax * 16);
             const int obase = (int)((T.oa + T.op0 * P.out.sax) * 16 - P.dst_lo);
-            constexpr bool twid = IO == IO_TWIDDLE || IO == IO_CONV;
+            constexpr bool twid = IO == IO_TWIDDLE || IO == IO_CONV || IO == IO_BLUE_IN;
             const double so = IO == IO_CONV ? -s : s;   // the row pass stores conj(W^{j n})
             if constexpr (twid) {
                 // W_L^{j n}, n = b + q N/RL: W^{j b} times powers of W^{j N/RL}
@@ -343,6 +381,26 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
                         for (int e = 0; e < 4; ++e)
                             if (q & (1 << e)) w = cmul(w, pw[e]);
                         bstore2(rout, obase + n * ostep, cmul(v[c * RL + q], tw_sign(w, so)));
+                    }
+                }
+            } else if constexpr (IO == IO_BLUE_OUT) {
+                // X[pos] = (c[pos] w[pos], conjugated on request) fct for pos < olim; the chirp
+                // loads are issued for every point (position 0 for those past the limit) so that
+                // only the stores sit under the branch
+                const __amdgpu_buffer_rsrc_t rch = make_rsrc(P.chirp, P.olim * 16);
+                const int olim = (int)P.olim, op0 = (int)T.op0, opst = (int)P.out.pst;
+                const double cj = P.conj_out ? -1.0 : 1.0;
+#pragma unroll
+                for (int c = 0; c < NB; ++c) {
+                    const int b = i + c * NT;
+#pragma unroll
+                    for (int q = 0; q < RL; ++q) {
+                        const int n = b + q * (N / RL);
+                        const int pos = op0 + n * opst;
+                        const double2 w = bload2(rch, (pos < olim ? pos : 0) * 16);
+                        double2 z = cmul(v[c * RL + q], w);
+                        z.y *= cj;
+                        if (pos < olim) bstore2(rout, obase + n * ostep, cscale(z, P.fct));
                     }
                 }
             } else {
@@ -478,17 +536,28 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
 template <int N, int IO>
 static hipError_t launch_io(const FftParams &P, int map, unsigned grid, unsigned block, size_t lds, hipStream_t s)
 {
+    // the Bluestein modes are built for the row and column mappings only (pow2_io)
+    constexpr bool ROWCOL = (IO & (IO_W4 - 1)) != IO_BLUE_IN && (IO & (IO_W4 - 1)) != IO_BLUE_OUT;
     static bool attr_done = false;
     if (!attr_done) {
-        for (const void *f : {(const void *)k_fft_pow2<N, MAP_ROW, IO>, (const void *)k_fft_pow2<N, MAP_COL, IO>,
-                              (const void *)k_fft_pow2<N, MAP_ROWCOL, IO>})
+        for (const void *f : {(const void *)k_fft_pow2<N, MAP_ROW, IO>, (const void *)k_fft_pow2<N, MAP_COL, IO>})
             if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 8192) != hipSuccess)
+                (void)hipGetLastError();
+        if constexpr (ROWCOL)
+            if (hipFuncSetAttribute((const void *)k_fft_pow2<N, MAP_ROWCOL, IO>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 8192) != hipSuccess)
                 (void)hipGetLastError();
         attr_done = true;
     }
     switch (map) {
         case MAP_COL: hipLaunchKernelGGL((k_fft_pow2<N, MAP_COL, IO>), dim3(grid), dim3(block), lds, s, P); break;
-        case MAP_ROWCOL: hipLaunchKernelGGL((k_fft_pow2<N, MAP_ROWCOL, IO>), dim3(grid), dim3(block), lds, s, P); break;
+        case MAP_ROWCOL:
+            if constexpr (ROWCOL) {
+                hipLaunchKernelGGL((k_fft_pow2<N, MAP_ROWCOL, IO>), dim3(grid), dim3(block), lds, s, P);
+                break;
+            } else {
+                return hipErrorInvalidValue;
+            }
         default: hipLaunchKernelGGL((k_fft_pow2<N, MAP_ROW, IO>), dim3(grid), dim3(block), lds, s, P); break;
     }
     return hipGetLastError();
@@ -516,17 +585,17 @@ static hipError_t launch_rowpair(const FftParams &P, unsigned grid, unsigned blo
 
 // the four-step Bluestein row pass (transform, spectrum product, inverse transform)
 // on the row mapping
-template <int N>
+template <int N, int IOW>
 static hipError_t launch_conv(const FftParams &P, unsigned grid, unsigned block, size_t lds, hipStream_t s)
 {
     static bool attr_done = false;
     if (!attr_done) {
-        if (hipFuncSetAttribute((const void *)k_fft_pow2<N, MAP_ROW, IO_CONV>,
+        if (hipFuncSetAttribute((const void *)k_fft_pow2<N, MAP_ROW, IOW>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 8192) != hipSuccess)
             (void)hipGetLastError();
         attr_done = true;
     }
-    hipLaunchKernelGGL((k_fft_pow2<N, MAP_ROW, IO_CONV>), dim3(grid), dim3(block), lds, s, P);
+    hipLaunchKernelGGL((k_fft_pow2<N, MAP_ROW, IOW>), dim3(grid), dim3(block), lds, s, P);
     return hipGetLastError();
 }
 
@@ -535,10 +604,18 @@ static hipError_t launch_n(const FftParams &P, int map, int io, unsigned grid, u
                            hipStream_t s)
 {
     if (io == IO_R2C_ROWPAIR) return map == MAP_ROWCOL ? launch_rowpair<N>(P, grid, block, lds, s) : hipErrorInvalidValue;
-    if (io == IO_CONV) return map == MAP_ROW ? launch_conv<N>(P, grid, block, lds, s) : hipErrorInvalidValue;
+    if ((io & (IO_W4 - 1)) == IO_CONV) {
+        if (map != MAP_ROW) return hipErrorInvalidValue;
+        return (io & IO_W4) ? launch_conv<N, IO_CONV | IO_W4>(P, grid, block, lds, s)
+                            : launch_conv<N, IO_CONV>(P, grid, block, lds, s);
+    }
     switch (io) {
         case IO_PLAIN: return launch_io<N, IO_PLAIN>(P, map, grid, block, lds, s);
         case IO_TWIDDLE: return launch_io<N, IO_TWIDDLE>(P, map, grid, block, lds, s);
+        case IO_BLUE_IN: return launch_io<N, IO_BLUE_IN>(P, map, grid, block, lds, s);
+        case IO_BLUE_OUT: return launch_io<N, IO_BLUE_OUT>(P, map, grid, block, lds, s);
+        case IO_BLUE_IN | IO_W4: return launch_io<N, IO_BLUE_IN | IO_W4>(P, map, grid, block, lds, s);
+        case IO_BLUE_OUT | IO_W4: return launch_io<N, IO_BLUE_OUT | IO_W4>(P, map, grid, block, lds, s);
         default: return launch_io<N, IO_GENERIC>(P, map, grid, block, lds, s);
     }
 }
@@ -570,11 +647,77 @@ bool pow2_fast_load(const FftParams &P)
            P.src_span <= 0x7fffffffLL;
 }
 
-int pow2_io(const FftParams &P)
+// the Bluestein column passes: f64 data, every position and byte offset within 31 bits
+static bool blue_windows(const FftParams &P)
+{
+    const long long win = 0x7fffffffLL;
+    const long long imax = (P.ni - 1) * P.in.ipos + (long long)(P.N - 1) * P.in.pst;
+    const long long omax = (P.ni - 1) * P.out.ipos + (long long)(P.N - 1) * P.out.pst;
+    return P.chirp && !P.src32 && !P.dst32 && P.src_span > 0 && P.src_span <= win && P.dst_span > 0 &&
+           P.dst_span <= win && imax >= 0 && omax >= 0 && (imax + 1) * 16 <= win && (omax + 1) * 16 <= win &&
+           P.in.ipos >= 0 && P.in.pst > 0 && P.out.ipos >= 0 && P.out.pst > 0;
+}
+
+// chirp-in loads (complex or odd real pairs) with the plain four-step twiddle store
+static bool pow2_blue_in(const FftParams &P)
+{
+    const long long omax = (P.ni - 1) * P.out.ipos + (long long)(P.N - 1) * P.out.pst;
+    return blue_windows(P) && P.chirp_in && (P.load_mode == LD_C2C || P.load_mode == LD_R2C_PAIR) &&
+           P.ilim <= 0x7fffffffLL / 16 && P.store_mode == ST_TWIDDLE && P.tw_mul == 1 && !P.chirp_out &&
+           !P.conj_out && omax < P.olim;
+}
+
+// plain loads with the chirp-out complex store
+static bool pow2_blue_out(const FftParams &P)
+{
+    return blue_windows(P) && pow2_fast_load(P) && P.chirp_out && P.olim <= 0x7fffffffLL / 16 &&
+           P.store_mode == ST_C2C;
+}
+
+// compute units of the current device (read once per device)
+static int device_cus()
+{
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();
+        return 256;
+    }
+    if (!cus[dev]) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) {
+            (void)hipGetLastError();
+            v = 256;
+        }
+        cus[dev] = v;
+    }
+    return cus[dev];
+}
+
+// A grid of at most four rounds of resident workgroups (three waves per SIMD) runs its
+// Bluestein passes on the four-wave instances: a second workgroup per CU overlaps one's loads
+// with the other's stages, which the register spills of those instances do not outweigh.  On
+// r2c 1,000,003 x4 the three passes took 0.164 -> 0.155 ms; on larger batches (c2c 512 x
+// 60,013: 1.50 -> 1.78 ms) the spills cost more than the overlap gains.
+static bool few_rounds(unsigned grid, unsigned block, size_t lds)
+{
+    const int waves = (int)((block + 63) / 64);
+    int per_cu = 12 / std::max(1, waves);
+    const size_t lds_wg = lds + 2048;   // the kernel's static shared memory
+    per_cu = std::min<int>(per_cu, (int)((size_t)160 * 1024 / lds_wg));
+    return per_cu >= 1 && (long long)grid <= 4LL * per_cu * device_cus();
+}
+
+int pow2_io(const FftParams &P, int map, unsigned grid, unsigned block, size_t lds)
 {
     if (P.store_mode == ST_R2C_ROWPAIR) return IO_R2C_ROWPAIR;
-    if (P.store_mode == ST_CONV_TWIDDLE) return IO_CONV;
-    return pow2_fast(P) ? (P.store_mode == ST_TWIDDLE ? IO_TWIDDLE : IO_PLAIN) : IO_GENERIC;
+    int io = IO_GENERIC;
+    if (P.store_mode == ST_CONV_TWIDDLE) io = IO_CONV;
+    else if (pow2_fast(P)) return P.store_mode == ST_TWIDDLE ? IO_TWIDDLE : IO_PLAIN;
+    else if (map != MAP_ROWCOL && pow2_blue_in(P)) io = IO_BLUE_IN;
+    else if (map != MAP_ROWCOL && pow2_blue_out(P)) io = IO_BLUE_OUT;
+    else return IO_GENERIC;
+    return few_rounds(grid, block, lds) ? (io | IO_W4) : io;
 }
 
 // workgroups of the fused row-pair pass: pair groups plus one self-paired group per signal
@@ -586,7 +729,7 @@ long long pow2_rowpair_grid(long long signals, long long L2, int TB)
 
 hipError_t launch_fft_pow2(const FftParams &P, int map, unsigned grid, unsigned block, size_t lds, hipStream_t s)
 {
-    const int fast = pow2_io(P);
+    const int fast = pow2_io(P, map, grid, block, lds);
     switch (P.N) {
         case 64: return launch_n<64>(P, map, fast, grid, block, lds, s);
         case 128: return launch_n<128>(P, map, fast, grid, block, lds, s);

@@ -1324,7 +1324,7 @@ static int plan_execute_ws(Plan &P, const void *in, void *out, double fct, hipSt
             if (ps.kind == Pass::FFT_POW2) {
                 e = launch_fft_pow2(fp, ps.map, ps.grid, ps.block, ps.lds, stream);
                 ps.kname = "k_fft_pow2<" + std::to_string(fp.N) + ", " + std::to_string(ps.map) + ", " +
-                           std::to_string(pow2_io(fp)) + ">";
+                           std::to_string(pow2_io(fp, ps.map, ps.grid, ps.block, ps.lds)) + ">";
             } else {
                 e = launch_fft_lds(fp, ps.grid, ps.block, ps.lds, stream);
                 ps.kname = "k_fft_lds<" + std::to_string(fft_lds_wpe(fp)) + ">";

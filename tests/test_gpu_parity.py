@@ -369,6 +369,49 @@ def test_bluestein_three_pass(n):
             assert rel_l2(y, oracle.c2r(h, m, [1], fwd, 1.0 / m)) < TOL, ("c2r", m, fwd)
 
 
+@pytest.mark.parametrize("kind,batch,w4", [("c2c", 2, True), ("c2c", 512, False), ("r2c", 3, True),
+                                           ("r2c", 1024, False)])
+def test_bluestein_pass_instances(kind, batch, w4):
+    """The three-pass Bluestein of n = 16381 (n2 = 256 x 128) with direct buffer accesses in its
+    column passes: chirp-in loads (complex, or odd real pairs) with the twiddled store, and the
+    chirp-out store.  Small batches (a few rounds of workgroups) run the passes on the four-wave
+    instances (IO code + 8), large batches on the three-wave ones; both against the oracle on
+    the first and last signals, checking which instances ran."""
+    torch = _torch()
+    n = 16381
+    print(f"case Bluestein pass instances {kind} [{batch},{n}]", flush=True)
+    sp = torch.cuda.current_stream().cuda_stream
+    if kind == "c2c":
+        x = synth.complex_normal((batch, n), seed=batch)
+        yd = torch.empty((batch, n), dtype=torch.complex128, device="cuda")
+        lk = _lib.C2C
+    else:
+        x = synth.uniform((batch, n), -1, 1, seed=batch)
+        yd = torch.empty((batch, n // 2 + 1), dtype=torch.complex128, device="cuda")
+        lk = _lib.R2C
+    xd = _dev(x)
+    st_in = [s * xd.element_size() for s in xd.stride()]
+    st_out = [s * yd.element_size() for s in yd.stride()]
+    for fwd in (True, False):
+        pl = F.Plan(lk, [batch, n], st_in, st_out, [1], fwd)
+        pl.execute(xd.data_ptr(), yd.data_ptr(), 0.5, sp)
+        torch.cuda.synchronize()
+        y = yd.cpu().numpy()
+        for row in (0, batch - 1):
+            ref = oracle.c2c(x[row], [0], fwd, 0.5) if kind == "c2c" else oracle.r2c(x[row], [0], fwd, 0.5)
+            assert rel_l2(y[row], ref) < TOL, (kind, batch, fwd, row)
+        names = []
+        for k in range(pl.num_launches):
+            name = ctypes.create_string_buffer(64)
+            z, u = ctypes.c_size_t(), ctypes.c_uint()
+            _lib.check(_lib.lib().impulse_fft_plan_pass_info(pl.handle, k, name, 64, ctypes.byref(z), ctypes.byref(z),
+                                                          ctypes.byref(u), ctypes.byref(u), ctypes.byref(z)))
+            names.append(name.value.decode())
+        print(kind, batch, fwd, names)
+        io = [int(s.rsplit(",", 1)[1].rstrip(">")) for s in names if s.startswith("k_fft_pow2")]
+        assert io[:3] == ([13, 12, 14] if w4 else [5, 4, 6]), names
+
+
 @pytest.mark.parametrize("kind,n,batch", [("c2c", 16381, 5), ("c2c", 65536, 3), ("r2c", 65536, 3),
                                           ("r2c", 32749, 5), ("c2r", 32762, 3)])
 def test_four_step_batch_chunks(kind, n, batch):

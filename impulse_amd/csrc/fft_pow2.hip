@@ -165,13 +165,16 @@ __device__ __forceinline__ void dft_groups(double2 *v, double s)
 #ifndef IFFT_ROWCOL_WPE
 #define IFFT_ROWCOL_WPE 3
 #endif
+#ifndef IFFT_ROW_WPE
+#define IFFT_ROW_WPE 3
+#endif
 #ifndef IFFT_COL_WPE
 #define IFFT_COL_WPE 3
 #endif
 template <int MAP, int IOW> constexpr int pow2_wpe()
 {
     if (IOW & IO_W4) return 4;
-    return MAP == MAP_ROW ? 3 : (MAP == MAP_COL ? IFFT_COL_WPE : IFFT_ROWCOL_WPE);
+    return MAP == MAP_ROW ? IFFT_ROW_WPE : (MAP == MAP_COL ? IFFT_COL_WPE : IFFT_ROWCOL_WPE);
 }
 
 template <int N, int MAP, int IOW>

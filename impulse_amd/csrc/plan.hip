@@ -1122,10 +1122,22 @@ int plan_create(std::unique_ptr<Plan> &out, int kind, int dtype, size_t ndim, co
 
     std::vector<Job> jobs;
     if (kind == J_C2C) {
+        // The axes are separable, so their order changes only the rounding.  The axis with the
+        // smallest input stride runs first: its pass reads the input and writes the output in
+        // whole lines, and the strided passes that follow run in place on lines they have just
+        // read, which stay in the caches for their stores.  Timed alone (20 executes of one plan),
+        // c2c 4096^2 over [0, 1] took 0.297 ms columns first and 0.282 ms rows first, 2048 x 8192
+        // 0.301 -> 0.285 ms and 8192^2 1.78 -> 1.71 ms; between other transforms the gain is
+        // within run-to-run noise.
+        std::vector<size_t> order(axes, axes + naxes);
+        size_t lead = 0;
+        for (size_t a = 1; a < naxes; ++a)
+            if (std::llabs(is[order[a]]) < std::llabs(is[order[lead]])) lead = a;
+        std::rotate(order.begin(), order.begin() + lead, order.begin() + lead + 1);
         for (size_t a = 0; a < naxes; ++a) {
             const bool first = a == 0;
-            jobs.push_back(make_job(J_C2C, (long long)shape[axes[a]], P.forward, first, first ? REF_IN : REF_OUT,
-                                    REF_OUT, axes[a], P.shape, first ? is : os, os));
+            jobs.push_back(make_job(J_C2C, (long long)shape[order[a]], P.forward, first, first ? REF_IN : REF_OUT,
+                                    REF_OUT, order[a], P.shape, first ? is : os, os));
         }
     } else if (kind == J_R2C) {
         jobs.push_back(make_job(J_R2C, (long long)shape[last], P.forward, true, REF_IN, REF_OUT, last, P.shape, is, os));

@@ -153,6 +153,11 @@ __device__ __forceinline__ double bload1(__amdgpu_buffer_rsrc_t r, int off)
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
 }
 
+__device__ __forceinline__ float2 bloadf2(__amdgpu_buffer_rsrc_t r, int off)
+{
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
 // a transform's input; NT = true marks the load non-temporal (streamed past the L2), for
 // passes whose lines no other workgroup reads
 template <bool NT>
@@ -165,6 +170,14 @@ __device__ __forceinline__ void bstore2(__amdgpu_buffer_rsrc_t r, int off, doubl
 {
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned int, v),
                                            r, off, 0, 0);
+}
+
+// v rounded to complex float
+__device__ __forceinline__ void bstoref2(__amdgpu_buffer_rsrc_t r, int off, double2 v)
+{
+    const float2 f = make_float2((float)v.x, (float)v.y);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned int, f),
+                                          r, off, 0, 0);
 }
 
 __device__ __forceinline__ double2 twolevel_b(__amdgpu_buffer_rsrc_t lo, __amdgpu_buffer_rsrc_t hi, int logS,

@@ -18,7 +18,8 @@
 // Bluestein chirp) and store modes (plain, four-step twiddle, Bluestein
 // spectrum product or chirp, c2r outputs) are fused into the global accesses;
 // FAST instantiations cover plain complex loads and plain / twiddled complex
-// stores with direct pointer arithmetic.
+// stores (complex double, or complex float widened on load and rounded on
+// store) with direct pointer arithmetic.
 //
 // MAP selects the lane mapping of the global accesses:
 //   MAP_ROW    : lanes walk along a transform (contiguous axis)
@@ -67,9 +68,13 @@ template <int N, int MAP> constexpr int pow2_ld() { return N + N / 16 + (MAP == 
 // accesses -- chirp-in loads (complex or odd real pairs) with a twiddled store,
 // and plain loads with a chirp-out complex store (rows and columns mappings)
 // IO_W4 (added to IO_CONV or a Bluestein mode) selects the instance held to four waves per
-// SIMD, for grids of a few rounds.
+// SIMD, for grids of a few rounds.  IO_S32 / IO_D32 (added to IO_PLAIN, or IO_S32 to
+// IO_TWIDDLE) select the instances whose source / destination array holds complex float: widened
+// to FP64 on load, rounded to nearest on store.  They are instances of their own so that the
+// FP64 instances carry no float code (a runtime branch in the shared instances cost the FP64
+// passes 1-2 %).
 enum Pow2Io { IO_GENERIC = 0, IO_PLAIN = 1, IO_TWIDDLE = 2, IO_R2C_ROWPAIR = 3, IO_CONV = 4, IO_BLUE_IN = 5,
-              IO_BLUE_OUT = 6, IO_W4 = 8 };
+              IO_BLUE_OUT = 6, IO_W4 = 8, IO_S32 = 16, IO_D32 = 32 };
 
 // Row assignment of the fused real post-process (ST_R2C_ROWPAIR, rows of L1 = N
 // points, L2 rows per signal, Z[j + L2 n] in row j).  Z[M - k] for k = j + L2 n
@@ -181,6 +186,7 @@ template <int N, int MAP, int IOW>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_wpe<MAP, IOW>()))) k_fft_pow2(const FftParams P)
 {
     constexpr int IO = IOW & (IO_W4 - 1);
+    constexpr bool S32 = IOW & IO_S32, D32 = IOW & IO_D32;
     extern __shared__ double lds[];
     __shared__ TfInfo info[MAP == MAP_ROWCOL ? kMaxTB : 1];
     constexpr bool COL = MAP == MAP_COL;
@@ -248,12 +254,23 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
         constexpr bool NTLOAD = IO == IO_R2C_ROWPAIR;
         // byte offsets inside the source window fit 31 bits (checked at plan time)
         const __amdgpu_buffer_rsrc_t rin = make_rsrc((const char *)P.src + P.src_lo, P.src_span);
-        const int step = (int)(P.in.pst * P.in.sax * 16);
-        const int base = (int)((T.ia + T.ip0 * P.in.sax) * 16 - P.src_lo) + i * step;
+        if constexpr (S32) {
+            // complex float: 8-byte loads, widened to FP64
+            const int step = (int)(P.in.pst * P.in.sax * 8);
+            const int base = (int)((T.ia + T.ip0 * P.in.sax) * 8 - P.src_lo) + i * step;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const double2 x = bload2_data<NTLOAD>(rin, base + q * (N / 16) * step);
-            v[q] = valid ? x : make_double2(0.0, 0.0);
+            for (int q = 0; q < 16; ++q) {
+                const float2 x = bloadf2(rin, base + q * (N / 16) * step);
+                v[q] = valid ? make_double2((double)x.x, (double)x.y) : make_double2(0.0, 0.0);
+            }
+        } else {
+            const int step = (int)(P.in.pst * P.in.sax * 16);
+            const int base = (int)((T.ia + T.ip0 * P.in.sax) * 16 - P.src_lo) + i * step;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const double2 x = bload2_data<NTLOAD>(rin, base + q * (N / 16) * step);
+                v[q] = valid ? x : make_double2(0.0, 0.0);
+            }
         }
     } else {
 #pragma unroll
@@ -357,8 +374,14 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
         if (!valid) return;
         if constexpr (FAST) {
             const __amdgpu_buffer_rsrc_t rout = make_rsrc((char *)P.dst + P.dst_lo, P.dst_span);
-            const int ostep = (int)(P.out.pst * P.out.sax * 16);
-            const int obase = (int)((T.oa + T.op0 * P.out.sax) * 16 - P.dst_lo);
+            // complex float outputs: rounded to nearest, 8-byte stores
+            constexpr int oes = D32 ? 8 : 16;
+            const int ostep = (int)(P.out.pst * P.out.sax * oes);
+            const int obase = (int)((T.oa + T.op0 * P.out.sax) * oes - P.dst_lo);
+            auto put = [&](int off, double2 z) {
+                if constexpr (D32) bstoref2(rout, off, z);
+                else bstore2(rout, off, z);
+            };
             constexpr bool twid = IO == IO_TWIDDLE || IO == IO_CONV || IO == IO_BLUE_IN;
             const double so = IO == IO_CONV ? -s : s;   // the row pass stores conj(W^{j n})
             if constexpr (twid) {
@@ -383,7 +406,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
 #pragma unroll
                         for (int e = 0; e < 4; ++e)
                             if (q & (1 << e)) w = cmul(w, pw[e]);
-                        bstore2(rout, obase + n * ostep, cmul(v[c * RL + q], tw_sign(w, so)));
+                        put(obase + n * ostep, cmul(v[c * RL + q], tw_sign(w, so)));
                     }
                 }
             } else if constexpr (IO == IO_BLUE_OUT) {
@@ -403,7 +426,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
                         const double2 w = bload2(rch, (pos < olim ? pos : 0) * 16);
                         double2 z = cmul(v[c * RL + q], w);
                         z.y *= cj;
-                        if (pos < olim) bstore2(rout, obase + n * ostep, cscale(z, P.fct));
+                        if (pos < olim) put(obase + n * ostep, cscale(z, P.fct));
                     }
                 }
             } else {
@@ -413,7 +436,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
 #pragma unroll
                     for (int q = 0; q < RL; ++q) {
                         const int n = b + q * (N / RL);
-                        bstore2(rout, obase + n * ostep, cscale(v[c * RL + q], P.fct));
+                        put(obase + n * ostep, cscale(v[c * RL + q], P.fct));
                     }
                 }
             }
@@ -527,7 +550,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
                 if constexpr (IO == IO_PLAIN) {
                     const __amdgpu_buffer_rsrc_t rout = make_rsrc((char *)P.dst + P.dst_lo, P.dst_span);
                     const long long pos = U.op0 + (long long)(n + half * H) * P.out.pst;
-                    bstore2(rout, (int)((U.oa + pos * P.out.sax) * 16 - P.dst_lo), cscale(ldc[tt * SH + padx(n)], P.fct));
+                    const double2 z = cscale(ldc[tt * SH + padx(n)], P.fct);
+                    if constexpr (D32) bstoref2(rout, (int)((U.oa + pos * P.out.sax) * 8 - P.dst_lo), z);
+                    else bstore2(rout, (int)((U.oa + pos * P.out.sax) * 16 - P.dst_lo), z);
                 } else {
                     store_value(P, U, n + half * H, ldc[tt * SH + padx(n)]);
                 }
@@ -618,6 +643,10 @@ static hipError_t launch_n(const FftParams &P, int map, int io, unsigned grid, u
         case IO_BLUE_IN: return launch_io<N, IO_BLUE_IN>(P, map, grid, block, lds, s);
         case IO_BLUE_OUT: return launch_io<N, IO_BLUE_OUT>(P, map, grid, block, lds, s);
         case IO_BLUE_IN | IO_W4: return launch_io<N, IO_BLUE_IN | IO_W4>(P, map, grid, block, lds, s);
+        case IO_PLAIN | IO_S32: return launch_io<N, IO_PLAIN | IO_S32>(P, map, grid, block, lds, s);
+        case IO_PLAIN | IO_D32: return launch_io<N, IO_PLAIN | IO_D32>(P, map, grid, block, lds, s);
+        case IO_PLAIN | IO_S32 | IO_D32: return launch_io<N, IO_PLAIN | IO_S32 | IO_D32>(P, map, grid, block, lds, s);
+        case IO_TWIDDLE | IO_S32: return launch_io<N, IO_TWIDDLE | IO_S32>(P, map, grid, block, lds, s);
         case IO_BLUE_OUT | IO_W4: return launch_io<N, IO_BLUE_OUT | IO_W4>(P, map, grid, block, lds, s);
         default: return launch_io<N, IO_GENERIC>(P, map, grid, block, lds, s);
     }
@@ -628,14 +657,14 @@ int pow2_tb_host(int N, int map) { return pow2_tb(N, map); }
 // LDS bytes per transform (one padded component row of doubles)
 int pow2_lds_bytes(int N, int map) { return (N + N / 16 + (map == MAP_ROWCOL ? 2 : 1)) * 8; }
 
-// plain complex loads and plain / twiddled complex stores take the FAST path
-// when no position can reach the zero-padding or store limits
+// plain complex loads and plain / twiddled complex stores (complex double or complex float
+// arrays) take the FAST path when no position can reach the zero-padding or store limits
 bool pow2_fast(const FftParams &P)
 {
     const long long imax = (P.ni - 1) * P.in.ipos + (long long)(P.N - 1) * P.in.pst;
     const long long omax = (P.ni - 1) * P.out.ipos + (long long)(P.N - 1) * P.out.pst;
     const long long win = 0x7fffffffLL;   // buffer windows are addressed with 31-bit byte offsets
-    return P.load_mode == LD_C2C && !P.src32 && !P.dst32 && !P.conj_in && !P.chirp_in && imax < P.ilim &&
+    return P.load_mode == LD_C2C && !P.conj_in && !P.chirp_in && imax < P.ilim &&
            (P.store_mode == ST_C2C || (P.store_mode == ST_TWIDDLE && P.tw_mul == 1)) && !P.chirp_out &&
            !P.conj_out && omax < P.olim && P.src_span > 0 && P.src_span <= win && P.dst_span > 0 &&
            P.dst_span <= win;
@@ -716,7 +745,11 @@ int pow2_io(const FftParams &P, int map, unsigned grid, unsigned block, size_t l
     if (P.store_mode == ST_R2C_ROWPAIR) return IO_R2C_ROWPAIR;
     int io = IO_GENERIC;
     if (P.store_mode == ST_CONV_TWIDDLE) io = IO_CONV;
-    else if (pow2_fast(P)) return P.store_mode == ST_TWIDDLE ? IO_TWIDDLE : IO_PLAIN;
+    else if (pow2_fast(P)) {
+        io = (P.store_mode == ST_TWIDDLE ? IO_TWIDDLE : IO_PLAIN) | (P.src32 ? IO_S32 : 0) | (P.dst32 ? IO_D32 : 0);
+        // the twiddled store writes a workspace: no complex float instance for it
+        return (io & IO_D32) && (io & (IO_W4 - 1)) == IO_TWIDDLE ? IO_GENERIC : io;
+    }
     else if (map != MAP_ROWCOL && pow2_blue_in(P)) io = IO_BLUE_IN;
     else if (map != MAP_ROWCOL && pow2_blue_out(P)) io = IO_BLUE_OUT;
     else return IO_GENERIC;

@@ -660,6 +660,46 @@ def vendor_fft(torch, plist, bufs, ms_per_step, reps):
             "parts": rows, "step_ms": round(total, 4), "this_engine_speedup": round(total / ms_per_step, 2)}
 
 
+def f32_arrays(torch, plist, bufs, ms64, steps, warmup):
+    """The same three-part workload with float32 / complex64 arrays (the reference's
+    FFTDesc[float32], pocketfft::c2c/r2c<float>): FP64 arithmetic inside, the arrays half the
+    bytes.  Inputs are the FP64 inputs rounded to float; the error is the rel-L2 of each part's
+    output against its FP64 output."""
+    from impulse_amd import _lib
+    from impulse_amd import fft as F
+    f32of = {"c128": torch.complex64, "f64": torch.float32}
+    b32, pl32 = [], []
+    for p, (x, y) in zip(plist, bufs):
+        x32 = x.to(f32of[p.in_dtype])
+        y32 = torch.empty(p.out_shape, dtype=f32of[p.out_dtype], device="cuda")
+        ib = 8 if p.in_dtype == "c128" else 4
+        pl32.append(F.Plan(p.kind, p.shape, c_strides(p.shape, ib), c_strides(p.out_shape, 8), p.axes, True,
+                           dtype=_lib.F32))
+        b32.append((x32, y32))
+    sp = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        for pl, (x, y) in zip(pl32, b32):
+            pl.execute(x.data_ptr(), y.data_ptr(), 1.0, sp)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    errs = []
+    for (x, y), (_, y32) in zip(bufs, b32):
+        errs.append(float(torch.linalg.vector_norm(y32.to(y.dtype) - y) / torch.linalg.vector_norm(y)))
+    return {"workload": "the same three parts with float32 / complex64 arrays, FP64 arithmetic",
+            "ms_per_step": round(ms, 4), "gflops": round(sum(p.flops for p in plist) / (ms * 1e-3) / 1e9, 1),
+            "vs_fp64_arrays": round(ms64 / ms, 3), "max_rel_l2_vs_fp64": max(errs)}
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -774,6 +814,7 @@ def main():
             f["vs_fp64_dropin_gflops"] = round(f["gflops"] / value, 3)
             f["mfma_bf16x"]["vs_fp64_dropin_gflops"] = round(f["mfma_bf16x"]["gflops"] / value, 3)
     vendor = None if args.no_vendor else vendor_fft(torch, plist, bufs, ms_per_step, args.steps)
+    f32 = f32_arrays(torch, plist, bufs, ms_per_step, args.steps, args.warmup)
     cpu = None if args.no_cpu else cpu_baseline(plist, args.cpu_seconds)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
@@ -783,7 +824,7 @@ def main():
                                "r2c f64 1000003 x4", "parts": [p.name for p in plist]},
         "roofline": roofline, "cpu_baseline": cpu, "parts": part_rows, "launches": launches, "c1": c1,
         "forms": forms,
-        "vendor_fft": vendor,
+        "vendor_fft": vendor, "f32_arrays": f32,
     }
     print(json.dumps(out))
 

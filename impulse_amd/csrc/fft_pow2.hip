@@ -79,25 +79,54 @@ enum Pow2Io { IO_GENERIC = 0, IO_PLAIN = 1, IO_TWIDDLE = 2, IO_R2C_ROWPAIR = 3, 
 // Row assignment of the fused real post-process (ST_R2C_ROWPAIR, rows of L1 = N
 // points, L2 rows per signal, Z[j + L2 n] in row j).  Z[M - k] for k = j + L2 n
 // sits in row L2 - j at point L1 - 1 - n, so a workgroup of TB rows takes TB/2
-// pairs (j, L2 - j), j = 1 .. L2/2 - 1 (rows j in slots 0 .. TB/2-1, partners in
-// slots TB/2 ..); one more workgroup per signal takes the self-paired rows 0
-// and L2/2.  Returns the logical transform of slot t, or -1 for an idle slot.
-__device__ __forceinline__ long long rowpair_transform(long long blk, int t, int TB, long long L2)
+// pairs (j, L2 - j), j = 1 .. L2/2 - 1: pair slot p holds row j in slot p and its
+// partner in slot p + TB/2.  The self-paired rows 0 and L2/2 take the first free
+// pair slot of the signal's last group (slots cnt and cnt + TB/2), or a group of
+// their own when the pairs fill every group; then each signal has as many groups
+// as its rows fill, and the grid no round with a few groups left over.
+struct RowpairGroup {
+    long long u;   // signal
+    long long w;   // group within the signal
+    int cnt;       // pair slots holding pairs
+    bool self;     // the self-paired rows sit in pair slot cnt
+};
+
+__host__ __device__ __forceinline__ long long rowpair_groups(long long L2, int TB)
+{
+    const long long pp = TB / 2, npairs = L2 / 2 - 1;
+    const long long W = (npairs + pp - 1) / pp;
+    return W * pp > npairs ? W : W + 1;
+}
+
+__device__ __forceinline__ RowpairGroup rowpair_group(long long blk, int TB, long long L2)
 {
     const int pp = TB / 2;
     const long long npairs = L2 / 2 - 1;
-    const long long W = (npairs + pp - 1) / pp;   // pair workgroups per signal
-    const long long u = blk / (W + 1), w = blk - u * (W + 1);
+    const long long G = rowpair_groups(L2, TB);
+    RowpairGroup R;
+    R.u = blk / G;
+    R.w = blk - R.u * G;
+    const long long left = npairs - R.w * pp;
+    R.cnt = (int)(left < 0 ? 0 : (left < pp ? left : pp));
+    R.self = R.w == G - 1;
+    return R;
+}
+
+// logical transform of slot t, or -1 for an idle slot
+__device__ __forceinline__ long long rowpair_transform(const RowpairGroup &R, int t, int TB, long long L2)
+{
+    const int pp = TB / 2, p = t % pp;
+    const bool partner = t >= pp;
     long long row;
-    if (w < W) {
-        const long long j = w * pp + 1 + (t % pp);
-        if (j > npairs) return -1;
-        row = t < pp ? j : L2 - j;
+    if (p < R.cnt) {
+        const long long j = R.w * pp + 1 + p;
+        row = partner ? L2 - j : j;
+    } else if (R.self && p == R.cnt) {
+        row = partner ? L2 / 2 : 0;
     } else {
-        if (t > 1) return -1;
-        row = t == 0 ? 0 : L2 / 2;
+        return -1;
     }
-    return u * L2 + row;
+    return R.u * L2 + row;
 }
 
 // X[k] and X[M-k] of an even real transform from Z[k], Z[M-k] (M complex
@@ -200,8 +229,10 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
     const int i = COL ? tid / TB : tid % NT;
     const double s = (double)P.sign;
     long long gidx;
+    RowpairGroup RG{};
     if constexpr (IO == IO_R2C_ROWPAIR) {
-        const long long g = rowpair_transform(xcd_block(blockIdx.x, gridDim.x), t, TB, P.ni);
+        RG = rowpair_group(xcd_block(blockIdx.x, gridDim.x), TB, P.ni);
+        const long long g = rowpair_transform(RG, t, TB, P.ni);
         gidx = g < 0 ? P.ntf : g;
     } else {
         gidx = (long long)xcd_block(blockIdx.x, gridDim.x) * TB + t;
@@ -449,11 +480,12 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
             }
         }
     } else if constexpr (IO == IO_R2C_ROWPAIR) {
-        // rows j (slots t < TB/2) and L2 - j (slots t + TB/2): point n of row j
+        // rows j (slots p < cnt) and L2 - j (slots p + TB/2): point n of row j
         // pairs with point L1 - 1 - n of row L2 - j.  Round r keeps half r of the
         // j rows and half 1 - r of their partners in LDS, so every pair of a
-        // round is resident; the self-paired rows 0 and L2/2 keep both halves
-        // (slots t and t + 2).
+        // round is resident.  Then the self-paired rows 0 (slot cnt) and L2/2
+        // (slot cnt + TB/2), where the group holds them, keep both halves in LDS
+        // slots 0, 1 and 2, 3.
         if (i == 0) info[t] = T;
         double2 *ldc = (double2 *)lds;
         constexpr int H = N / 2;
@@ -462,37 +494,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
         constexpr int nthr = TB * NT;
         static_assert(TB >= 4 && TB % 2 == 0, "row pairs and the two self-paired rows need four slots");
         const long long L2 = P.ni;
+        const int cnt = RG.cnt;
         __syncthreads();
-        const bool selfwg = (info[0].flags & 1) && info[0].j == 0;
-        if (selfwg) {
-            if (t < 2 && valid) {
-#pragma unroll
-                for (int c = 0; c < NB; ++c) {
-                    const int b = i + c * NT;
-#pragma unroll
-                    for (int q = 0; q < RL; ++q) {
-                        const int n = b + q * (N / RL);
-                        const int slot = n < H ? t : t + 2;
-                        ldc[slot * SH + padx(n < H ? n : n - H)] = v[c * RL + q];
-                    }
-                }
-            }
-            __syncthreads();
-            auto zat = [&](int tt, int n) { return n < H ? ldc[tt * SH + padx(n)] : ldc[(tt + 2) * SH + padx(n - H)]; };
-            // row 0: pairs (n, L1 - n), n = 0 .. L1/2; row L2/2: pairs (n, L1 - 1 - n), n < L1/2
-            for (int e = tid; e < N + 1; e += nthr) {
-                if (e <= N / 2) {
-                    const int n = e;
-                    const TfInfo &U = info[0];
-                    real_split_store(P, U, L2 * (long long)n, zat(0, n), zat(0, (N - n) % N));
-                } else {
-                    const int n = e - (N / 2 + 1);
-                    const TfInfo &U = info[1];
-                    if (U.flags & 1)
-                        real_split_store(P, U, L2 / 2 + L2 * (long long)n, zat(1, n), zat(1, N - 1 - n));
-                }
-            }
-        } else {
+        if (cnt > 0) {
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
                 __syncthreads();
@@ -515,9 +519,40 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
                     const int e = tid + it * nthr;
                     const int pp = e % PP, nl = e / PP;
                     const TfInfo &U = info[pp];
-                    if (e >= PP * H || !(U.flags & 1)) continue;
+                    if (e >= PP * H || pp >= cnt || !(U.flags & 1)) continue;
                     const long long k = U.j + L2 * (long long)(r * H + nl);
                     real_split_store(P, U, k, ldc[pp * SH + padx(nl)], ldc[(pp + PP) * SH + padx(H - 1 - nl)]);
+                }
+            }
+        }
+        if (RG.self) {
+            // row 0 (slot cnt) into LDS slots 0 and 1, row L2/2 (slot cnt + PP) into 2 and 3
+            __syncthreads();
+            if ((t == cnt || t == cnt + PP) && valid) {
+                const int s0 = t == cnt ? 0 : 2;
+#pragma unroll
+                for (int c = 0; c < NB; ++c) {
+                    const int b = i + c * NT;
+#pragma unroll
+                    for (int q = 0; q < RL; ++q) {
+                        const int n = b + q * (N / RL);
+                        ldc[(s0 + (n < H ? 0 : 1)) * SH + padx(n < H ? n : n - H)] = v[c * RL + q];
+                    }
+                }
+            }
+            __syncthreads();
+            auto zat = [&](int s0, int n) { return n < H ? ldc[s0 * SH + padx(n)] : ldc[(s0 + 1) * SH + padx(n - H)]; };
+            // row 0: pairs (n, L1 - n), n = 0 .. L1/2; row L2/2: pairs (n, L1 - 1 - n), n < L1/2
+            for (int e = tid; e < N + 1; e += nthr) {
+                if (e <= N / 2) {
+                    const int n = e;
+                    const TfInfo &U = info[cnt];
+                    if (U.flags & 1) real_split_store(P, U, L2 * (long long)n, zat(0, n), zat(0, (N - n) % N));
+                } else {
+                    const int n = e - (N / 2 + 1);
+                    const TfInfo &U = info[cnt + PP];
+                    if (U.flags & 1)
+                        real_split_store(P, U, L2 / 2 + L2 * (long long)n, zat(2, n), zat(2, N - 1 - n));
                 }
             }
         }
@@ -756,11 +791,11 @@ int pow2_io(const FftParams &P, int map, unsigned grid, unsigned block, size_t l
     return few_rounds(grid, block, lds) ? (io | IO_W4) : io;
 }
 
-// workgroups of the fused row-pair pass: pair groups plus one self-paired group per signal
+// workgroups of the fused row-pair pass: the pair groups of every signal, the self-paired rows
+// in the last one's free slot or in one more group
 long long pow2_rowpair_grid(long long signals, long long L2, int TB)
 {
-    const long long pp = TB / 2, npairs = L2 / 2 - 1;
-    return signals * ((npairs + pp - 1) / pp + 1);
+    return signals * rowpair_groups(L2, TB);
 }
 
 hipError_t launch_fft_pow2(const FftParams &P, int map, unsigned grid, unsigned block, size_t lds, hipStream_t s)

@@ -153,6 +153,11 @@ __device__ __forceinline__ double bload1(__amdgpu_buffer_rsrc_t r, int off)
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
 }
 
+__device__ __forceinline__ float bloadf1(__amdgpu_buffer_rsrc_t r, int off)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
 __device__ __forceinline__ float2 bloadf2(__amdgpu_buffer_rsrc_t r, int off)
 {
     return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));

@@ -255,7 +255,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
         const __amdgpu_buffer_rsrc_t rin = make_rsrc((const char *)P.src + P.src_lo, P.src_span);
         const __amdgpu_buffer_rsrc_t rch = make_rsrc(P.chirp, P.ilim * 16);
         const bool pairs = P.load_mode == LD_R2C_PAIR;
-        const int es = pairs ? 8 : 16;
+        const int es = (pairs ? 8 : 16) / (S32 ? 2 : 1);   // float / complex float sources: half
         const int ba = (int)(T.ia * es - P.src_lo), bb = (int)(T.ib * es - P.src_lo);
         const int estep = (int)P.in.sax * es;
         const int ilim = (int)P.ilim;
@@ -267,7 +267,15 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
             const bool ok = valid && pos < ilim;
             const int pc = ok ? pos : 0;
             double2 z;
-            if (pairs) {
+            if constexpr (S32) {
+                if (pairs) {
+                    const float a = bloadf1(rin, ba + pc * estep), b = bloadf1(rin, bb + pc * estep);
+                    z = make_double2((double)a, hasb ? (double)b : 0.0);
+                } else {
+                    const float2 f = bloadf2(rin, ba + pc * estep);
+                    z = make_double2((double)f.x, (double)f.y);
+                }
+            } else if (pairs) {
                 const double a = bload1(rin, ba + pc * estep), b = bload1(rin, bb + pc * estep);
                 z = make_double2(a, hasb ? b : 0.0);
             } else {
@@ -281,7 +289,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
         // The r2c row-pair pass reads whole rows that no other workgroup touches: its loads are
         // non-temporal, which keeps the L2 for merging its short strided stores (pass time
         // 0.337 -> 0.330 ms at r2c 2^22 x16).  The column passes read 32- to 128-byte pieces of
-        // lines their neighbours read too, and lose 4-17 % with the same hint.
+        // lines their neighbours read too, and lose 4-17 % with the same hint.  The r2c pass A,
+        // whose 128-byte pieces are whole lines, gains 5 % with it (0.243 -> 0.232 ms) but leaves
+        // pass B 5 % slower, so it keeps cached loads too.
         constexpr bool NTLOAD = IO == IO_R2C_ROWPAIR;
         // byte offsets inside the source window fit 31 bits (checked at plan time)
         const __amdgpu_buffer_rsrc_t rin = make_rsrc((const char *)P.src + P.src_lo, P.src_span);
@@ -678,6 +688,10 @@ static hipError_t launch_n(const FftParams &P, int map, int io, unsigned grid, u
         case IO_BLUE_IN: return launch_io<N, IO_BLUE_IN>(P, map, grid, block, lds, s);
         case IO_BLUE_OUT: return launch_io<N, IO_BLUE_OUT>(P, map, grid, block, lds, s);
         case IO_BLUE_IN | IO_W4: return launch_io<N, IO_BLUE_IN | IO_W4>(P, map, grid, block, lds, s);
+        case IO_BLUE_IN | IO_S32: return launch_io<N, IO_BLUE_IN | IO_S32>(P, map, grid, block, lds, s);
+        case IO_BLUE_IN | IO_S32 | IO_W4: return launch_io<N, IO_BLUE_IN | IO_S32 | IO_W4>(P, map, grid, block, lds, s);
+        case IO_BLUE_OUT | IO_D32: return launch_io<N, IO_BLUE_OUT | IO_D32>(P, map, grid, block, lds, s);
+        case IO_BLUE_OUT | IO_D32 | IO_W4: return launch_io<N, IO_BLUE_OUT | IO_D32 | IO_W4>(P, map, grid, block, lds, s);
         case IO_PLAIN | IO_S32: return launch_io<N, IO_PLAIN | IO_S32>(P, map, grid, block, lds, s);
         case IO_PLAIN | IO_D32: return launch_io<N, IO_PLAIN | IO_D32>(P, map, grid, block, lds, s);
         case IO_PLAIN | IO_S32 | IO_D32: return launch_io<N, IO_PLAIN | IO_S32 | IO_D32>(P, map, grid, block, lds, s);
@@ -720,7 +734,7 @@ static bool blue_windows(const FftParams &P)
     const long long win = 0x7fffffffLL;
     const long long imax = (P.ni - 1) * P.in.ipos + (long long)(P.N - 1) * P.in.pst;
     const long long omax = (P.ni - 1) * P.out.ipos + (long long)(P.N - 1) * P.out.pst;
-    return P.chirp && !P.src32 && !P.dst32 && P.src_span > 0 && P.src_span <= win && P.dst_span > 0 &&
+    return P.chirp && P.src_span > 0 && P.src_span <= win && P.dst_span > 0 &&
            P.dst_span <= win && imax >= 0 && omax >= 0 && (imax + 1) * 16 <= win && (omax + 1) * 16 <= win &&
            P.in.ipos >= 0 && P.in.pst > 0 && P.out.ipos >= 0 && P.out.pst > 0;
 }
@@ -729,7 +743,7 @@ static bool blue_windows(const FftParams &P)
 static bool pow2_blue_in(const FftParams &P)
 {
     const long long omax = (P.ni - 1) * P.out.ipos + (long long)(P.N - 1) * P.out.pst;
-    return blue_windows(P) && P.chirp_in && (P.load_mode == LD_C2C || P.load_mode == LD_R2C_PAIR) &&
+    return blue_windows(P) && !P.dst32 && P.chirp_in && (P.load_mode == LD_C2C || P.load_mode == LD_R2C_PAIR) &&
            P.ilim <= 0x7fffffffLL / 16 && P.store_mode == ST_TWIDDLE && P.tw_mul == 1 && !P.chirp_out &&
            !P.conj_out && omax < P.olim;
 }
@@ -785,8 +799,8 @@ int pow2_io(const FftParams &P, int map, unsigned grid, unsigned block, size_t l
         // the twiddled store writes a workspace: no complex float instance for it
         return (io & IO_D32) && (io & (IO_W4 - 1)) == IO_TWIDDLE ? IO_GENERIC : io;
     }
-    else if (map != MAP_ROWCOL && pow2_blue_in(P)) io = IO_BLUE_IN;
-    else if (map != MAP_ROWCOL && pow2_blue_out(P)) io = IO_BLUE_OUT;
+    else if (map != MAP_ROWCOL && pow2_blue_in(P)) io = IO_BLUE_IN | (P.src32 ? IO_S32 : 0);
+    else if (map != MAP_ROWCOL && pow2_blue_out(P)) io = IO_BLUE_OUT | (P.dst32 ? IO_D32 : 0);
     else return IO_GENERIC;
     return few_rounds(grid, block, lds) ? (io | IO_W4) : io;
 }

@@ -198,3 +198,48 @@ def test_f32_autograd_adjoints():
         lhs = (z.detach().double() * w.double()).sum()
         rhs = (Z.detach().conj() * gZ).real.double().sum()
         assert abs(float(lhs - rhs)) <= 1e-5 * float(Z.detach().abs().double().norm() * gZ.abs().double().norm()), n
+
+
+@pytest.mark.parametrize("kind,batch,w4", [("c2c", 2, True), ("c2c", 512, False), ("r2c", 3, True),
+                                           ("r2c", 1024, False)])
+def test_f32_bluestein_pass_instances(kind, batch, w4):
+    """The three-pass Bluestein of n = 16381 on float arrays: the chirp-in pass reads the float
+    (or complex float) input directly (IO code 5 + 16, + 8 on the four-wave instances), and for c2c
+    the chirp-out pass writes the complex float output directly (6 + 32); r2c writes the workspace
+    and its real post-process rounds to float.  First and last signals against the oracle."""
+    import ctypes
+    import torch
+    n = 16381
+    print(f"case f32 Bluestein pass instances {kind} [{batch},{n}]", flush=True)
+    sp = torch.cuda.current_stream().cuda_stream
+    if kind == "c2c":
+        x = synth.complex_normal((batch, n), seed=batch).astype(np.complex64)
+        yd = torch.empty((batch, n), dtype=torch.complex64, device="cuda")
+        lk = _lib.C2C
+    else:
+        x = synth.uniform((batch, n), -1, 1, seed=batch).astype(np.float32)
+        yd = torch.empty((batch, n // 2 + 1), dtype=torch.complex64, device="cuda")
+        lk = _lib.R2C
+    xd = torch.from_numpy(x).cuda()
+    st_in = [s * xd.element_size() for s in xd.stride()]
+    st_out = [s * yd.element_size() for s in yd.stride()]
+    for fwd in (True, False):
+        pl = F.Plan(lk, [batch, n], st_in, st_out, [1], fwd, dtype=_lib.F32)
+        pl.execute(xd.data_ptr(), yd.data_ptr(), 0.5, sp)
+        torch.cuda.synchronize()
+        y = yd.cpu().numpy()
+        for row in (0, batch - 1):
+            xr = _w(x[row])
+            ref = oracle.c2c(xr, [0], fwd, 0.5) if kind == "c2c" else oracle.r2c(xr, [0], fwd, 0.5)
+            assert rel_l2(y[row], ref) <= F32_TOL, (kind, batch, fwd, row)
+        names = []
+        for k in range(pl.num_launches):
+            name = ctypes.create_string_buffer(64)
+            z, u = ctypes.c_size_t(), ctypes.c_uint()
+            _lib.check(_lib.lib().impulse_fft_plan_pass_info(pl.handle, k, name, 64, ctypes.byref(z), ctypes.byref(z),
+                                                          ctypes.byref(u), ctypes.byref(u), ctypes.byref(z)))
+            names.append(name.value.decode())
+        print(kind, batch, fwd, names)
+        io = [int(s.rsplit(",", 1)[1].rstrip(">")) for s in names if s.startswith("k_fft_pow2")]
+        want = [21, 4, 38] if kind == "c2c" else [21, 4, 6]
+        assert io[:3] == ([c + 8 for c in want] if w4 else want), names

@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--only-forms", default="",
                     help="comma list of forms to run alone (ragged,mfma,conv,dct,upfirdn,fftpack), no drop-in step")
     ap.add_argument("--form-reps", type=int, default=10)
+    ap.add_argument("--no-f32", action="store_true",
+                    help="skip the float32-array run of the drop-in step (profiles of the FP64 launches)")
     ap.add_argument("--no-vendor", action="store_true",
                     help="skip timing torch.fft (hipFFT/rocFFT) on the same inputs as a comparator")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -814,7 +816,7 @@ def main():
             f["vs_fp64_dropin_gflops"] = round(f["gflops"] / value, 3)
             f["mfma_bf16x"]["vs_fp64_dropin_gflops"] = round(f["mfma_bf16x"]["gflops"] / value, 3)
     vendor = None if args.no_vendor else vendor_fft(torch, plist, bufs, ms_per_step, args.steps)
-    f32 = f32_arrays(torch, plist, bufs, ms_per_step, args.steps, args.warmup)
+    f32 = None if args.no_f32 else f32_arrays(torch, plist, bufs, ms_per_step, args.steps, args.warmup)
     cpu = None if args.no_cpu else cpu_baseline(plist, args.cpu_seconds)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,

@@ -73,8 +73,10 @@ template <int N, int MAP> constexpr int pow2_ld() { return N + N / 16 + (MAP == 
 // to FP64 on load, rounded to nearest on store.  They are instances of their own so that the
 // FP64 instances carry no float code (a runtime branch in the shared instances cost the FP64
 // passes 1-2 %).
+// IO_R2C_HALF: a whole even-length r2c signal per transform (its half-length complex sequence,
+// per-point loads), the real post-process fused into the store.
 enum Pow2Io { IO_GENERIC = 0, IO_PLAIN = 1, IO_TWIDDLE = 2, IO_R2C_ROWPAIR = 3, IO_CONV = 4, IO_BLUE_IN = 5,
-              IO_BLUE_OUT = 6, IO_W4 = 8, IO_S32 = 16, IO_D32 = 32 };
+              IO_BLUE_OUT = 6, IO_R2C_HALF = 7, IO_W4 = 8, IO_S32 = 16, IO_D32 = 32 };
 
 // Row assignment of the fused real post-process (ST_R2C_ROWPAIR, rows of L1 = N
 // points, L2 rows per signal, Z[j + L2 n] in row j).  Z[M - k] for k = j + L2 n
@@ -135,7 +137,7 @@ __device__ __forceinline__ void real_split_store(const FftParams &P, const TfInf
                                                  double2 zm)
 {
     const long long M = P.M;
-    const double2 w = twolevel(P.rtw_lo, P.rtw_hi, P.rtw_logS, k);
+    const double2 w = real_tw(P, k);   // W_{2M}^k: the direct table or the two-level one
     const double2 e1 = cscale(cadd(zk, cconj(zm)), 0.5), d1 = csub(zk, cconj(zm));
     const double2 o1 = make_double2(0.5 * d1.y, -0.5 * d1.x);
     const double2 e2 = cscale(cadd(zm, cconj(zk)), 0.5), d2 = csub(zm, cconj(zk));
@@ -239,7 +241,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
     }
     const TfInfo T = make_tfinfo(P, gidx);
     const bool valid = T.flags & 1;
-    constexpr bool FAST = IO != IO_GENERIC;
+    constexpr bool FAST = IO != IO_GENERIC && IO != IO_R2C_HALF;
     // twiddle table through a buffer resource: 32-bit offsets, scalar base
     const __amdgpu_buffer_rsrc_t rtw = make_rsrc(P.tw, (long long)N * 16);
 
@@ -312,6 +314,18 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
                 const double2 x = bload2_data<NTLOAD>(rin, base + q * (N / 16) * step);
                 v[q] = valid ? x : make_double2(0.0, 0.0);
             }
+        }
+    } else if (IO == IO_R2C_HALF && P.load_mode == LD_C2C && !P.src32 && P.src_span > 0 &&
+               P.src_span <= 0x7fffffffLL) {
+        // the real-half load of an aligned contiguous row seen as complex words (the plan's
+        // alternative parameters): direct 16-byte loads
+        const __amdgpu_buffer_rsrc_t rin = make_rsrc((const char *)P.src + P.src_lo, P.src_span);
+        const int step = (int)(P.in.pst * P.in.sax * 16);
+        const int base = (int)((T.ia + T.ip0 * P.in.sax) * 16 - P.src_lo) + i * step;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const double2 x = bload2(rin, base + q * (N / 16) * step);
+            v[q] = valid ? x : make_double2(0.0, 0.0);
         }
     } else {
 #pragma unroll
@@ -411,6 +425,38 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
         stages(-s);
     }
 
+    if constexpr (IO == IO_R2C_HALF) {
+        // X[k] and X[M - k] from Z[k] and Z[M - k] (M = N): the lower half of Z goes to LDS
+        // (the last stage ended with a barrier); the holder of Z[n], n > N/2, reads Z[N - n] and
+        // writes X[N - n] and X[n]; the holders of Z[0] and Z[N/2] write X[0], X[N] and X[N/2].
+        double2 *ldc = (double2 *)lds;
+        constexpr int H = N / 2;
+        constexpr int SH = LDP / 2;
+        static_assert(SH >= H + H / 16, "half a transform must fit its LDS row");
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            const int b = i + c * NT;
+#pragma unroll
+            for (int q = 0; q < RL; ++q) {
+                const int n = b + q * (N / RL);
+                if (n < H) ldc[t * SH + padx(n)] = v[c * RL + q];
+            }
+        }
+        __syncthreads();
+        if (!valid) return;
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            const int b = i + c * NT;
+#pragma unroll
+            for (int q = 0; q < RL; ++q) {
+                const int n = b + q * (N / RL);
+                const double2 z = v[c * RL + q];
+                if (n > H) real_split_store(P, T, N - n, ldc[t * SH + padx(N - n)], z);
+                else if (n == 0 || n == H) real_split_store(P, T, n, z, z);
+            }
+        }
+        return;
+    }
     if constexpr (MAP != MAP_ROWCOL) {
         if (!valid) return;
         if constexpr (FAST) {
@@ -609,8 +655,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
 template <int N, int IO>
 static hipError_t launch_io(const FftParams &P, int map, unsigned grid, unsigned block, size_t lds, hipStream_t s)
 {
-    // the Bluestein modes are built for the row and column mappings only (pow2_io)
-    constexpr bool ROWCOL = (IO & (IO_W4 - 1)) != IO_BLUE_IN && (IO & (IO_W4 - 1)) != IO_BLUE_OUT;
+    // the Bluestein and real-half modes are built for the row and column mappings only (pow2_io)
+    constexpr bool ROWCOL = (IO & (IO_W4 - 1)) != IO_BLUE_IN && (IO & (IO_W4 - 1)) != IO_BLUE_OUT &&
+                            (IO & (IO_W4 - 1)) != IO_R2C_HALF;
     static bool attr_done = false;
     if (!attr_done) {
         for (const void *f : {(const void *)k_fft_pow2<N, MAP_ROW, IO>, (const void *)k_fft_pow2<N, MAP_COL, IO>})
@@ -686,6 +733,7 @@ static hipError_t launch_n(const FftParams &P, int map, int io, unsigned grid, u
         case IO_PLAIN: return launch_io<N, IO_PLAIN>(P, map, grid, block, lds, s);
         case IO_TWIDDLE: return launch_io<N, IO_TWIDDLE>(P, map, grid, block, lds, s);
         case IO_BLUE_IN: return launch_io<N, IO_BLUE_IN>(P, map, grid, block, lds, s);
+        case IO_R2C_HALF: return launch_io<N, IO_R2C_HALF>(P, map, grid, block, lds, s);
         case IO_BLUE_OUT: return launch_io<N, IO_BLUE_OUT>(P, map, grid, block, lds, s);
         case IO_BLUE_IN | IO_W4: return launch_io<N, IO_BLUE_IN | IO_W4>(P, map, grid, block, lds, s);
         case IO_BLUE_IN | IO_S32: return launch_io<N, IO_BLUE_IN | IO_S32>(P, map, grid, block, lds, s);
@@ -792,6 +840,7 @@ static bool few_rounds(unsigned grid, unsigned block, size_t lds)
 int pow2_io(const FftParams &P, int map, unsigned grid, unsigned block, size_t lds)
 {
     if (P.store_mode == ST_R2C_ROWPAIR) return IO_R2C_ROWPAIR;
+    if (P.store_mode == ST_R2C_HALF) return map == MAP_ROWCOL ? IO_GENERIC : IO_R2C_HALF;   // plan routes rows only
     int io = IO_GENERIC;
     if (P.store_mode == ST_CONV_TWIDDLE) io = IO_CONV;
     else if (pow2_fast(P)) {

@@ -338,13 +338,17 @@ struct Builder {
     // the compile-time power-of-two kernel takes per-point load/store modes
     bool try_pow2(FftParams P, BufRef src, BufRef dst, bool uses_fct, bool colL, bool colS)
     {
-        if (!pow2_supported(P.N) || P.conv || P.nload != P.N || P.nstore != P.N) return false;
-        if (P.store_mode == ST_R2C_HALF || P.store_mode == ST_R2C_PAIR) return false;
+        // an even-length r2c signal per transform (real post-process fused into the store) runs on
+        // the row mapping; its N + 1 outputs are not a transform's N points
+        const bool rhalf = P.store_mode == ST_R2C_HALF && P.load_mode == LD_REAL_HALF && P.nstore == P.N + 1;
+        if (!pow2_supported(P.N) || P.conv || P.nload != P.N || (P.nstore != P.N && !rhalf)) return false;
+        if ((P.store_mode == ST_R2C_HALF && !rhalf) || P.store_mode == ST_R2C_PAIR) return false;
         int map;
         if (colL && colS) map = 1;
         else if (!colL && colS) map = 2;
         else if (!colL && !colS) map = 0;
         else return false;
+        if (rhalf && map != 0) return false;
         const int NT = P.N / 16;
         const long long per = (long long)pow2_lds_bytes(P.N, map);
         const long long cap = max_lds_bytes() - (long long)sizeof(long long) * 8 * kMaxTB - 1024;

@@ -486,3 +486,49 @@ def test_error_codes():
     assert rc == 1                                                    # in-place r2c is rejected
     assert not L.impulse_make_rfft_plan(0)
     assert not L.impulse_make_cfft_plan(0)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_real_rows_register_kernel(dtype):
+    """Even-length r2c and c2r rows whose half length is a power of two run on the register kernel:
+    r2c with the real post-process fused into the store (IO code 7), c2r with its pre-process in
+    the per-point loads (IO code 0): [256, 4096] both ways and both signs, against the oracle,
+    naming the instances that ran."""
+    torch = _torch()
+    rows, n = 256, 4096
+    f32 = dtype == "f32"
+    bar = 2.0 ** -24 + 2e-15 if f32 else TOL
+    lk_dt = _lib.F32 if f32 else _lib.F64
+    rdt, cdt = (np.float32, np.complex64) if f32 else (np.float64, np.complex128)
+    x = synth.uniform((rows, n), -1.0, 1.0, seed=5).astype(rdt)
+    h = synth.complex_normal((rows, n // 2 + 1), seed=6).astype(cdt)
+    sp = torch.cuda.current_stream().cuda_stream
+
+    def io_codes(pl):
+        out = []
+        for k in range(pl.num_launches):
+            name = ctypes.create_string_buffer(64)
+            z, u = ctypes.c_size_t(), ctypes.c_uint()
+            _lib.check(_lib.lib().impulse_fft_plan_pass_info(pl.handle, k, name, 64, ctypes.byref(z), ctypes.byref(z),
+                                                          ctypes.byref(u), ctypes.byref(u), ctypes.byref(z)))
+            out.append(name.value.decode())
+        return out
+
+    xd = torch.from_numpy(x).cuda()
+    hd = torch.from_numpy(h).cuda()
+    es = x.itemsize
+    for fwd in (True, False):
+        yd = torch.empty((rows, n // 2 + 1), dtype=torch.complex64 if f32 else torch.complex128, device="cuda")
+        pl = F.Plan(_lib.R2C, [rows, n], [n * es, es], [(n // 2 + 1) * 2 * es, 2 * es], [1], fwd, dtype=lk_dt)
+        pl.execute(xd.data_ptr(), yd.data_ptr(), 0.5, sp)
+        torch.cuda.synchronize()
+        ref = oracle.r2c(x.astype(np.float64), [1], fwd, 0.5)
+        assert rel_l2(yd.cpu().numpy(), ref) <= bar, ("r2c", dtype, fwd)
+        assert io_codes(pl) == ["k_fft_pow2<2048, 0, 7>"], io_codes(pl)
+        zd = torch.empty((rows, n), dtype=torch.float32 if f32 else torch.float64, device="cuda")
+        pc = F.Plan(_lib.C2R, [rows, n], [(n // 2 + 1) * 2 * es, 2 * es], [n * es, es], [1], fwd, dtype=lk_dt)
+        pc.execute(hd.data_ptr(), zd.data_ptr(), 1.0 / n, sp)
+        torch.cuda.synchronize()
+        ref = oracle.c2r(h.astype(np.complex128), n, [1], fwd, 1.0 / n)
+        assert rel_l2(zd.cpu().numpy(), ref) <= bar, ("c2r", dtype, fwd)
+        assert io_codes(pc) == ["k_fft_pow2<2048, 0, 0>"], io_codes(pc)

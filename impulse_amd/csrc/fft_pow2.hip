@@ -315,17 +315,25 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(pow2_w
                 v[q] = valid ? x : make_double2(0.0, 0.0);
             }
         }
-    } else if (IO == IO_R2C_HALF && P.load_mode == LD_C2C && !P.src32 && P.src_span > 0 &&
-               P.src_span <= 0x7fffffffLL) {
+    } else if (IO == IO_R2C_HALF && P.load_mode == LD_C2C && P.src_span > 0 && P.src_span <= 0x7fffffffLL) {
         // the real-half load of an aligned contiguous row seen as complex words (the plan's
-        // alternative parameters): direct 16-byte loads
+        // alternative parameters): direct 16-byte loads, or 8-byte loads of float pairs
         const __amdgpu_buffer_rsrc_t rin = make_rsrc((const char *)P.src + P.src_lo, P.src_span);
-        const int step = (int)(P.in.pst * P.in.sax * 16);
-        const int base = (int)((T.ia + T.ip0 * P.in.sax) * 16 - P.src_lo) + i * step;
+        const int es = P.src32 ? 8 : 16;
+        const int step = (int)(P.in.pst * P.in.sax * es);
+        const int base = (int)((T.ia + T.ip0 * P.in.sax) * es - P.src_lo) + i * step;
+        if (P.src32) {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const double2 x = bload2(rin, base + q * (N / 16) * step);
-            v[q] = valid ? x : make_double2(0.0, 0.0);
+            for (int q = 0; q < 16; ++q) {
+                const float2 x = bloadf2(rin, base + q * (N / 16) * step);
+                v[q] = valid ? make_double2((double)x.x, (double)x.y) : make_double2(0.0, 0.0);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const double2 x = bload2(rin, base + q * (N / 16) * step);
+                v[q] = valid ? x : make_double2(0.0, 0.0);
+            }
         }
     } else {
 #pragma unroll
